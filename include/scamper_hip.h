@@ -1,0 +1,152 @@
+/*
+ * scamper_hip.h -- C ABI of libscamper_hip.so, the MI355X (gfx950) classical
+ * lattice sweep.
+ *
+ * The entry points replace the reference's CPU sweep surface in
+ * UynajGI/scamper (CMC.rs / Carlo.rs).  Each declaration cites the reference
+ * interface it stands in for.  Conventions:
+ *   - every function returns an int status (SCAMPER_OK = 0) and never
+ *     unwinds; scamper_last_error() returns the message of the calling
+ *     thread's last failure (the reference's panics and CarloError values
+ *     become status codes);
+ *   - handles are not thread-safe: one handle per host thread, or external
+ *     locking (the reference's Algorithm/System are Send, not Sync);
+ *   - host pointers are borrowed for the duration of the call only; device
+ *     buffers belong to the handle;
+ *   - spins cross the ABI in the reference's layout (flat, site-major f64:
+ *     lattice/state.rs:19-28) or as site-major int8 / f32 for speed.
+ */
+#ifndef SCAMPER_HIP_H
+#define SCAMPER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ statuses */
+enum {
+  SCAMPER_OK = 0,
+  SCAMPER_EINVAL = 1,       /* invalid argument (CarloError::InvalidConfig) */
+  SCAMPER_ENOMEM = 2,       /* device or host allocation failed */
+  SCAMPER_EDEVICE = 3,      /* HIP runtime error */
+  SCAMPER_EINVARIANT = 4,   /* invariant violation (the reference panics) */
+  SCAMPER_EUNSUPPORTED = 5, /* valid in the reference, not on this path */
+  SCAMPER_EBUDGET = 6,      /* admission refused: HBM budget exhausted */
+};
+
+/* Message of the calling thread's last failure; returns its full length. */
+size_t scamper_last_error(char *buf, size_t len);
+/* Library version "major.minor.patch" and build target. */
+const char *scamper_version(void);
+
+/* ------------------------------------------------------------- lattices */
+typedef struct scamper_lattice scamper_lattice;
+
+/* build_hypercubic (CMC.rs/src/lattice/graph.rs:417-451); build_chain and
+ * build_square (graph.rs:402-414) are the 1-D and 2-D cases.  Site index
+ * x + Lx*(y + Ly*z); bonds to +1 along each axis, wrapped when pbc. */
+int scamper_lattice_hypercubic(const uint64_t *dims, int n_dims, int pbc, scamper_lattice **out);
+/* build_triangular / build_honeycomb / build_kagome (graph.rs:454-543). */
+int scamper_lattice_triangular(uint64_t lx, uint64_t ly, scamper_lattice **out);
+int scamper_lattice_honeycomb(uint64_t lx, uint64_t ly, scamper_lattice **out);
+int scamper_lattice_kagome(uint64_t lx, uint64_t ly, scamper_lattice **out);
+/* CsrLattice::try_from_edges (graph.rs:131-184): physical bonds stored once;
+ * parallel bonds and self-loops allowed; weight may be NULL (all 1.0). */
+int scamper_lattice_from_edges(uint64_t n_sites, const uint64_t *src, const uint64_t *dst,
+                               const double *weight, uint64_t n_edges, scamper_lattice **out);
+/* n_sites, n_edges (CsrLattice::n_edges), n_bonds (incidences, graph.rs:
+ * 112-124) and the number of colour classes of the greedy colouring. */
+int scamper_lattice_info(const scamper_lattice *lat, uint64_t *n_sites, uint64_t *n_edges,
+                         uint64_t *n_bonds, uint32_t *n_colours);
+/* CSR arrays in the reference's order (offsets[n+1], neighbors/edge_ids[n_bonds]). */
+int scamper_lattice_csr(const scamper_lattice *lat, uint64_t *offsets, uint64_t *neighbors,
+                        uint64_t *edge_ids);
+/* Colour of each site and its rank inside its colour class. */
+int scamper_lattice_colours(const scamper_lattice *lat, uint32_t *colour, uint64_t *class_index);
+void scamper_lattice_free(scamper_lattice *lat);
+
+/* --------------------------------------------------------------- models */
+enum { SCAMPER_ISING = 0, SCAMPER_POTTS = 1, SCAMPER_ON = 2 };
+
+/* IsingModel{j} (models.rs:29-33), PottsModel{j,q} (models.rs:190-194),
+ * ONModel<D>{j} (models.rs:384-388; spin_dim = D, only D = 3 on the GPU). */
+typedef struct {
+  int32_t kind;
+  int32_t q;
+  int32_t spin_dim;
+  int32_t reserved;
+  double j;
+} scamper_model;
+
+/* ---------------------------------------------------------------- state */
+typedef struct scamper_state scamper_state;
+
+/* A batch of n_replicas independent System values (lattice/state.rs:19-28)
+ * of one lattice and model, device-resident.  Replica r draws from the
+ * Philox key RngStreamKey(seed).with_replica(r) (Carlo.rs/src/rng_stream.rs:
+ * 126-154).  Every replica starts cold (ordered_spin) at beta = 1.0
+ * (classical_mc.rs:447-449 default). */
+int scamper_state_create(const scamper_lattice *lat, const scamper_model *model,
+                         uint32_t n_replicas, uint64_t seed, scamper_state **out);
+void scamper_state_free(scamper_state *st);
+/* System::set_beta (state.rs:67-73). */
+int scamper_state_set_beta(scamper_state *st, uint32_t replica, double beta);
+int scamper_state_get_beta(const scamper_state *st, uint32_t replica, double *beta);
+/* initial_state "cold" / "hot" (classical_mc.rs:452-466): ordered_spin, or
+ * a random start keyed by (seed, replica, site). */
+int scamper_state_init(scamper_state *st, uint32_t replica, int random);
+/* Upload / download one replica in the reference's layout (site-major f64,
+ * spin_dim values per site); uploads are validated like
+ * Hamiltonian::validate_configuration (interaction.rs:40-58). */
+int scamper_state_upload_f64(scamper_state *st, uint32_t replica, const double *spins);
+int scamper_state_download_f64(scamper_state *st, uint32_t replica, double *spins);
+/* Site-major int8 (Ising +-1, Potts 0..q-1) or f32 (O(3), 3 per site). */
+int scamper_state_upload_i8(scamper_state *st, uint32_t replica, const int8_t *spins);
+int scamper_state_download_i8(scamper_state *st, uint32_t replica, int8_t *spins);
+int scamper_state_upload_f32(scamper_state *st, uint32_t replica, const float *spins);
+int scamper_state_download_f32(scamper_state *st, uint32_t replica, float *spins);
+/* Sweeps done so far (the counter in every Philox draw). */
+int scamper_state_sweeps(const scamper_state *st, uint64_t *sweeps);
+
+/* ------------------------------------------------------------- sweeping */
+enum {
+  SCAMPER_METROPOLIS = 0,        /* MetropolisCore (algorithms/metropolis.rs:16-111) */
+  SCAMPER_HEAT_BATH = 1,         /* HeatBathCore (algorithms/heat_bath.rs:13-72) */
+  SCAMPER_CONT_HEAT_BATH = 2,    /* ContinuousHeatBathCore (heat_bath.rs:75-134) */
+  SCAMPER_OVER_RELAXATION = 3,   /* MicrocanonicalCore (algorithms/microcanonical.rs:13-85) */
+  SCAMPER_HYBRID_HB_OR = 4,      /* HybridCore<ContinuousHeatBath, Microcanonical> (hybrid.rs:10-64) */
+};
+
+/* Algorithm::sweep_with_phase (algorithms/common.rs:35-55) for every
+ * replica, n_sweeps times, visiting colour classes in ascending order.  With
+ * measure != 0 each sweep's energy and magnetisation of every replica are
+ * appended to the device-resident series (ClassicalMC::measure,
+ * classical_mc.rs:187-190).  Asynchronous on the state's stream. */
+int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure);
+/* Block until the state's queued work is done. */
+int scamper_synchronize(scamper_state *st);
+/* scamper_sweep bracketed by HIP events on the state's stream: *ms is the
+ * device time of the n_sweeps (the reference's Run::step sweep timer,
+ * Carlo.rs/src/run.rs:197-211). */
+int scamper_sweep_timed(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure, float *ms);
+
+/* Exact energy (compute_total_energy, interaction.rs:183-212) and
+ * magnetisation (Measurable::magnetization, models.rs:103-110, 276-292,
+ * 500-514) of one replica, recomputed from its spins. */
+int scamper_state_observables(scamper_state *st, uint32_t replica, double *energy,
+                              double *magnetization);
+/* Number of measured sweeps in the series, and the per-sweep "Energy" and
+ * "Magnetization" samples of one replica (observables/energy.rs:7-25,
+ * magnetization.rs:7-31).  Either output may be NULL. */
+int scamper_series_length(const scamper_state *st, uint64_t *n);
+int scamper_series_get(scamper_state *st, uint32_t replica, double *energy,
+                       double *magnetization, uint64_t max_n);
+int scamper_series_clear(scamper_state *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCAMPER_HIP_H */

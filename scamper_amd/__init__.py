@@ -1,0 +1,256 @@
+"""scamper_amd -- MI355X (gfx950) classical lattice sweep.
+
+Python access to libscamper_hip.so through its C ABI (include/scamper_hip.h).
+The library holds all of the work: HIP kernels, decision tables, lattice
+construction and the sweep driver.  This module only marshals arguments and
+turns status codes into exceptions; there is no fallback path, and importing
+the bindings fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libscamper_hip.so")
+
+ISING, POTTS, ON = 0, 1, 2
+METROPOLIS, HEAT_BATH, CONT_HEAT_BATH, OVER_RELAXATION, HYBRID_HB_OR = 0, 1, 2, 3, 4
+
+STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EINVARIANT",
+          5: "EUNSUPPORTED", 6: "EBUDGET"}
+
+
+class ScamperError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS.get(code, code)}: {message}")
+        self.code = code
+
+
+class Model(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("q", C.c_int32), ("spin_dim", C.c_int32),
+                ("reserved", C.c_int32), ("j", C.c_double)]
+
+
+_lib = None
+u64, u32, i32, dbl, vp, sz = C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_void_p, C.c_size_t
+PP = C.POINTER(C.c_void_p)
+
+# name -> (restype, argtypes); every function the header declares.
+SIGNATURES = {
+    "scamper_last_error": (sz, [C.c_char_p, sz]),
+    "scamper_version": (C.c_char_p, []),
+    "scamper_lattice_hypercubic": (i32, [vp, i32, i32, PP]),
+    "scamper_lattice_triangular": (i32, [u64, u64, PP]),
+    "scamper_lattice_honeycomb": (i32, [u64, u64, PP]),
+    "scamper_lattice_kagome": (i32, [u64, u64, PP]),
+    "scamper_lattice_from_edges": (i32, [u64, vp, vp, vp, u64, PP]),
+    "scamper_lattice_info": (i32, [vp, vp, vp, vp, vp]),
+    "scamper_lattice_csr": (i32, [vp, vp, vp, vp]),
+    "scamper_lattice_colours": (i32, [vp, vp, vp]),
+    "scamper_lattice_free": (None, [vp]),
+    "scamper_state_create": (i32, [vp, C.POINTER(Model), u32, u64, PP]),
+    "scamper_state_free": (None, [vp]),
+    "scamper_state_set_beta": (i32, [vp, u32, dbl]),
+    "scamper_state_get_beta": (i32, [vp, u32, vp]),
+    "scamper_state_init": (i32, [vp, u32, i32]),
+    "scamper_state_upload_f64": (i32, [vp, u32, vp]),
+    "scamper_state_download_f64": (i32, [vp, u32, vp]),
+    "scamper_state_upload_i8": (i32, [vp, u32, vp]),
+    "scamper_state_download_i8": (i32, [vp, u32, vp]),
+    "scamper_state_upload_f32": (i32, [vp, u32, vp]),
+    "scamper_state_download_f32": (i32, [vp, u32, vp]),
+    "scamper_state_sweeps": (i32, [vp, vp]),
+    "scamper_sweep": (i32, [vp, i32, u64, i32]),
+    "scamper_synchronize": (i32, [vp]),
+    "scamper_sweep_timed": (i32, [vp, i32, u64, i32, vp]),
+    "scamper_state_observables": (i32, [vp, u32, vp, vp]),
+    "scamper_series_length": (i32, [vp, vp]),
+    "scamper_series_get": (i32, [vp, u32, vp, vp, u64]),
+    "scamper_series_clear": (i32, [vp]),
+}
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m scamper_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().scamper_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise ScamperError(rc, last_error())
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+class Lattice:
+    """CsrLattice handle (CMC.rs/src/lattice/graph.rs:112-124)."""
+
+    def __init__(self, handle: int):
+        self.h = handle
+        n_sites, n_edges, n_bonds, n_col = u64(), u64(), u64(), u32()
+        check(lib().scamper_lattice_info(self.h, C.byref(n_sites), C.byref(n_edges),
+                                         C.byref(n_bonds), C.byref(n_col)))
+        self.n_sites, self.n_edges = n_sites.value, n_edges.value
+        self.n_bonds, self.n_colours = n_bonds.value, n_col.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.scamper_lattice_free(self.h)
+            self.h = None
+
+    @staticmethod
+    def _make(fn, *args) -> "Lattice":
+        out = C.c_void_p()
+        check(fn(*args, C.byref(out)))
+        return Lattice(out.value)
+
+    @classmethod
+    def hypercubic(cls, dims, pbc: bool = True) -> "Lattice":
+        d = np.ascontiguousarray(dims, dtype=np.uint64)
+        lat = cls._make(lib().scamper_lattice_hypercubic, _p(d), d.size, int(pbc))
+        lat.dims = tuple(int(x) for x in dims)
+        return lat
+
+    @classmethod
+    def square(cls, lx: int, ly: int, pbc: bool = True) -> "Lattice":
+        return cls.hypercubic([lx, ly], pbc)
+
+    @classmethod
+    def cubic(cls, l: int, pbc: bool = True) -> "Lattice":
+        return cls.hypercubic([l, l, l], pbc)
+
+    @classmethod
+    def chain(cls, n: int, pbc: bool = True) -> "Lattice":
+        return cls.hypercubic([n], pbc)
+
+    @classmethod
+    def triangular(cls, lx, ly) -> "Lattice":
+        return cls._make(lib().scamper_lattice_triangular, lx, ly)
+
+    @classmethod
+    def honeycomb(cls, lx, ly) -> "Lattice":
+        return cls._make(lib().scamper_lattice_honeycomb, lx, ly)
+
+    @classmethod
+    def kagome(cls, lx, ly) -> "Lattice":
+        return cls._make(lib().scamper_lattice_kagome, lx, ly)
+
+    @classmethod
+    def from_edges(cls, n_sites, src, dst, weight=None) -> "Lattice":
+        s = np.ascontiguousarray(src, dtype=np.uint64)
+        t = np.ascontiguousarray(dst, dtype=np.uint64)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+        return cls._make(lib().scamper_lattice_from_edges, n_sites, _p(s), _p(t), _p(w), s.size)
+
+    def csr(self):
+        off = np.zeros(self.n_sites + 1, np.uint64)
+        nb = np.zeros(max(1, self.n_bonds), np.uint64)
+        eid = np.zeros(max(1, self.n_bonds), np.uint64)
+        check(lib().scamper_lattice_csr(self.h, _p(off), _p(nb), _p(eid)))
+        return off, nb[: self.n_bonds], eid[: self.n_bonds]
+
+    def colours(self):
+        col = np.zeros(self.n_sites, np.uint32)
+        idx = np.zeros(self.n_sites, np.uint64)
+        check(lib().scamper_lattice_colours(self.h, _p(col), _p(idx)))
+        return col, idx
+
+
+def model(kind: int, j: float = 1.0, q: int = 2, spin_dim: int = 1) -> Model:
+    return Model(kind, q, spin_dim, 0, j)
+
+
+class State:
+    """Device-resident batch of System values (CMC.rs/src/lattice/state.rs:19-28)."""
+
+    def __init__(self, lattice: Lattice, m: Model, n_replicas: int = 1, seed: int = 0):
+        out = C.c_void_p()
+        check(lib().scamper_state_create(lattice.h, C.byref(m), n_replicas, seed, C.byref(out)))
+        self.h = out.value
+        self.lattice, self.model, self.n_replicas, self.seed = lattice, m, n_replicas, seed
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.scamper_state_free(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_beta(self, beta: float, replica: int = 0):
+        check(lib().scamper_state_set_beta(self.h, replica, beta))
+
+    def init(self, random: bool, replica: int = 0):
+        check(lib().scamper_state_init(self.h, replica, int(random)))
+
+    def upload_i8(self, spins, replica: int = 0):
+        a = np.ascontiguousarray(spins, dtype=np.int8)
+        check(lib().scamper_state_upload_i8(self.h, replica, _p(a)))
+
+    def download_i8(self, replica: int = 0) -> np.ndarray:
+        a = np.zeros(self.lattice.n_sites, np.int8)
+        check(lib().scamper_state_download_i8(self.h, replica, _p(a)))
+        return a
+
+    def upload_f64(self, spins, replica: int = 0):
+        a = np.ascontiguousarray(spins, dtype=np.float64)
+        check(lib().scamper_state_upload_f64(self.h, replica, _p(a)))
+
+    def download_f64(self, replica: int = 0) -> np.ndarray:
+        a = np.zeros(self.lattice.n_sites * max(1, self.model.spin_dim), np.float64)
+        check(lib().scamper_state_download_f64(self.h, replica, _p(a)))
+        return a
+
+    @property
+    def sweeps(self) -> int:
+        v = u64()
+        check(lib().scamper_state_sweeps(self.h, C.byref(v)))
+        return v.value
+
+    def sweep(self, algorithm: int = METROPOLIS, n_sweeps: int = 1, measure: bool = False):
+        check(lib().scamper_sweep(self.h, algorithm, n_sweeps, int(measure)))
+
+    def sweep_timed(self, algorithm: int = METROPOLIS, n_sweeps: int = 1,
+                    measure: bool = False) -> float:
+        """Device milliseconds of n_sweeps, from HIP events on the state's stream."""
+        ms = C.c_float()
+        check(lib().scamper_sweep_timed(self.h, algorithm, n_sweeps, int(measure), C.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        check(lib().scamper_synchronize(self.h))
+
+    def observables(self, replica: int = 0):
+        e, m = dbl(), dbl()
+        check(lib().scamper_state_observables(self.h, replica, C.byref(e), C.byref(m)))
+        return e.value, m.value
+
+    def series(self, replica: int = 0):
+        n = u64()
+        check(lib().scamper_series_length(self.h, C.byref(n)))
+        e = np.zeros(n.value, np.float64)
+        m = np.zeros(n.value, np.float64)
+        check(lib().scamper_series_get(self.h, replica, _p(e), _p(m), n.value))
+        return e, m
+
+    def clear_series(self):
+        check(lib().scamper_series_clear(self.h))

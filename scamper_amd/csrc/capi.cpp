@@ -1,0 +1,509 @@
+// C ABI of libscamper_hip.so: lattice handles, device-resident replica
+// batches and the sweep driver.  Declarations and reference citations are in
+// include/scamper_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "host.h"
+
+using namespace scamper;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define SC_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return fail(SCAMPER_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// RngStreamKey::seed (Carlo.rs/src/rng_stream.rs:126-154): SplitMix64
+// rounds fold every logical coordinate into the 64-bit stream seed.
+uint64_t mix64(uint64_t v) {
+  v += 0x9E3779B97F4A7C15ull;
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+  return v ^ (v >> 31);
+}
+uint64_t fold(uint64_t s, uint64_t domain, uint64_t value) {
+  return mix64(s ^ domain ^ mix64(value + ((domain << 17) | (domain >> 47))));
+}
+uint64_t stream_seed(uint64_t base, uint64_t replica, uint64_t phase) {
+  uint64_t s = mix64(base ^ 0x53435554544C4552ull);
+  s = fold(s, 0x5441534Bull, 0);               // task
+  s = fold(s, 0x52554E00ull, 0);               // run
+  s = fold(s, 0x434841494E00ull, 0);           // chain
+  s = fold(s, 0x5245504C49434100ull, replica); // replica
+  s = fold(s, 0x544852454144ull, 0);           // thread
+  s = fold(s, 0x504841534500ull, phase);       // phase
+  return fold(s, 0x535542535452ull, 0);        // substream
+}
+
+}  // namespace
+
+struct scamper_state {
+  Lattice lat;
+  scamper_model model{};
+  uint32_t n_rep = 0;
+  uint64_t seed = 0;
+  uint64_t sweeps = 0;
+  std::vector<double> beta;
+  std::vector<uint64_t> keys;
+  StencilGeom geom{};
+  int z = 0;
+  hipStream_t stream = nullptr;
+  int8_t *d_spins = nullptr;
+  ReplicaParams *d_params = nullptr;
+  double *d_hbexp = nullptr;
+  long long *d_em = nullptr;      // scratch for exact recomputation
+  long long *d_series = nullptr;  // [slot][replica][kEmStride]
+  uint64_t series_cap = 0, series_len = 0;
+  int tables_for = -1;            // algorithm the uploaded tables serve
+  int device = 0;
+
+  ~scamper_state() {
+    if (stream) hipStreamSynchronize(stream);
+    hipFree(d_spins);
+    hipFree(d_params);
+    hipFree(d_hbexp);
+    hipFree(d_em);
+    hipFree(d_series);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int check_state(const scamper_state *st, uint32_t replica) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  if (replica >= st->n_rep) return fail(SCAMPER_EINVAL, "replica %u out of range (%u replicas)", replica, st->n_rep);
+  return SCAMPER_OK;
+}
+
+// Build and upload the per-replica decision tables for `alg`.
+int upload_tables(scamper_state *st, int alg) {
+  std::vector<ReplicaParams> p(st->n_rep);
+  std::vector<double> hb;
+  const int z = st->z;
+  if (st->model.kind == kPotts && alg == kHeatBath) hb.assign((size_t)st->n_rep * (kMaxZ + 1) * (kMaxZ + 1), 0.0);
+  for (uint32_t r = 0; r < st->n_rep; ++r) {
+    memset(&p[r], 0, sizeof(ReplicaParams));
+    p[r].key = st->keys[r];
+    p[r].beta = st->beta[r];
+    if (st->model.kind == kIsing) {
+      if (alg == kMetropolis) ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
+      else ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
+    } else if (st->model.kind == kPotts) {
+      if (alg == kMetropolis) potts_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
+      else potts_heat_bath_exp(st->beta[r], st->model.j, z, hb.data() + (size_t)r * (kMaxZ + 1) * (kMaxZ + 1));
+    }
+  }
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(st->d_params, p.data(), p.size() * sizeof(ReplicaParams), hipMemcpyHostToDevice));
+  if (!hb.empty()) {
+    if (!st->d_hbexp) SC_HIP(hipMalloc(&st->d_hbexp, hb.size() * sizeof(double)));
+    SC_HIP(hipMemcpy(st->d_hbexp, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  st->tables_for = alg;
+  return SCAMPER_OK;
+}
+
+int ensure_series(scamper_state *st, uint64_t extra) {
+  const uint64_t need = st->series_len + extra;
+  if (need <= st->series_cap) return SCAMPER_OK;
+  uint64_t cap = std::max<uint64_t>(need, st->series_cap * 2);
+  cap = std::max<uint64_t>(cap, 1024);
+  long long *fresh = nullptr;
+  const size_t slot = (size_t)st->n_rep * kEmStride * sizeof(long long);
+  if (hipMalloc(&fresh, cap * slot) != hipSuccess) return fail(SCAMPER_ENOMEM, "series buffer of %llu sweeps", (unsigned long long)cap);
+  if (st->series_len) SC_HIP(hipMemcpyAsync(fresh, st->d_series, st->series_len * slot, hipMemcpyDeviceToDevice, st->stream));
+  SC_HIP(hipStreamSynchronize(st->stream));
+  hipFree(st->d_series);
+  st->d_series = fresh;
+  st->series_cap = cap;
+  return SCAMPER_OK;
+}
+
+// Site (x, row) of the stencil geometry -> colour-split byte offset.
+inline uint64_t split_index(const StencilGeom &g, uint64_t site, uint32_t *colour) {
+  const uint64_t lx = 2ull * g.lx2;
+  const uint64_t x = site % lx, row = site / lx;
+  const uint64_t y = row % g.ly, zz = row / g.ly;
+  *colour = (uint32_t)((x + y + zz) & 1);
+  return row * g.lx2 + (x >> 1);
+}
+
+int upload_i8_impl(scamper_state *st, uint32_t replica, const int8_t *spins) {
+  const uint64_t n = st->lat.n_sites;
+  for (uint64_t s = 0; s < n; ++s) {
+    const int8_t v = spins[s];
+    if (st->model.kind == kIsing ? (v != 1 && v != -1) : (v < 0 || v >= st->model.q))
+      return fail(SCAMPER_EINVAL, "invalid spin at site %llu: %d", (unsigned long long)s, (int)v);
+  }
+  std::vector<int8_t> split(n);
+  for (uint64_t s = 0; s < n; ++s) {
+    uint32_t c;
+    const uint64_t i = split_index(st->geom, s, &c);
+    split[c * st->geom.nh + i] = spins[s];
+  }
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(st->d_spins + 2ull * replica * st->geom.nh, split.data(), n, hipMemcpyHostToDevice));
+  return SCAMPER_OK;
+}
+
+int download_i8_impl(scamper_state *st, uint32_t replica, int8_t *spins) {
+  const uint64_t n = st->lat.n_sites;
+  std::vector<int8_t> split(n);
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(split.data(), st->d_spins + 2ull * replica * st->geom.nh, n, hipMemcpyDeviceToHost));
+  for (uint64_t s = 0; s < n; ++s) {
+    uint32_t c;
+    const uint64_t i = split_index(st->geom, s, &c);
+    spins[s] = split[c * st->geom.nh + i];
+  }
+  return SCAMPER_OK;
+}
+
+// Energy and magnetisation from the accumulated counters of one replica.
+void observables_from_em(const scamper_state *st, const long long *em, double *e, double *m) {
+  const double n = (double)st->lat.n_sites;
+  if (e) *e = -st->model.j * (double)em[0];
+  if (!m) return;
+  if (st->model.kind == kIsing) {
+    *m = std::fabs((double)em[1]) / n;
+  } else {
+    long long largest = 0;
+    for (int v = 0; v < st->model.q; ++v) largest = std::max(largest, em[2 + v]);
+    *m = ((double)st->model.q * (double)largest - n) / (n * (double)(st->model.q - 1));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t scamper_last_error(char *buf, size_t len) {
+  if (buf && len) {
+    const size_t k = std::min(len - 1, g_last_error.size());
+    memcpy(buf, g_last_error.data(), k);
+    buf[k] = 0;
+  }
+  return g_last_error.size();
+}
+
+const char *scamper_version(void) { return "0.1.0 gfx950"; }
+
+// ------------------------------------------------------------- lattices
+static int wrap_lattice(Lattice &&l, scamper_lattice **out) {
+  auto *h = new (std::nothrow) scamper_lattice;
+  if (!h) return fail(SCAMPER_ENOMEM, "lattice handle");
+  h->lat = std::move(l);
+  *out = h;
+  return SCAMPER_OK;
+}
+
+int scamper_lattice_hypercubic(const uint64_t *dims, int n_dims, int pbc, scamper_lattice **out) {
+  if (!dims || !out) return fail(SCAMPER_EINVAL, "null argument");
+  Lattice l;
+  std::string err;
+  if (!lattice_hypercubic(dims, n_dims, pbc != 0, l, err)) return fail(SCAMPER_EINVAL, "%s", err.c_str());
+  return wrap_lattice(std::move(l), out);
+}
+
+int scamper_lattice_triangular(uint64_t lx, uint64_t ly, scamper_lattice **out) {
+  if (!out) return fail(SCAMPER_EINVAL, "null argument");
+  Lattice l;
+  std::string err;
+  if (!lattice_triangular(lx, ly, l, err)) return fail(SCAMPER_EINVAL, "%s", err.c_str());
+  return wrap_lattice(std::move(l), out);
+}
+
+int scamper_lattice_honeycomb(uint64_t lx, uint64_t ly, scamper_lattice **out) {
+  if (!out) return fail(SCAMPER_EINVAL, "null argument");
+  Lattice l;
+  std::string err;
+  if (!lattice_honeycomb(lx, ly, l, err)) return fail(SCAMPER_EINVAL, "%s", err.c_str());
+  return wrap_lattice(std::move(l), out);
+}
+
+int scamper_lattice_kagome(uint64_t lx, uint64_t ly, scamper_lattice **out) {
+  if (!out) return fail(SCAMPER_EINVAL, "null argument");
+  Lattice l;
+  std::string err;
+  if (!lattice_kagome(lx, ly, l, err)) return fail(SCAMPER_EINVAL, "%s", err.c_str());
+  return wrap_lattice(std::move(l), out);
+}
+
+int scamper_lattice_from_edges(uint64_t n_sites, const uint64_t *src, const uint64_t *dst,
+                               const double *weight, uint64_t n_edges, scamper_lattice **out) {
+  if (!out || (n_edges && (!src || !dst))) return fail(SCAMPER_EINVAL, "null argument");
+  std::vector<Bond> e(n_edges);
+  for (uint64_t k = 0; k < n_edges; ++k) e[k] = Bond{src[k], dst[k], 0xFF, weight ? weight[k] : 1.0};
+  Lattice l;
+  std::string err;
+  if (!lattice_from_edges(n_sites, std::move(e), l, err)) return fail(SCAMPER_EINVAL, "%s", err.c_str());
+  return wrap_lattice(std::move(l), out);
+}
+
+int scamper_lattice_info(const scamper_lattice *h, uint64_t *n_sites, uint64_t *n_edges,
+                         uint64_t *n_bonds, uint32_t *n_colours) {
+  if (!h) return fail(SCAMPER_EINVAL, "null lattice");
+  if (n_sites) *n_sites = h->lat.n_sites;
+  if (n_edges) *n_edges = h->lat.n_edge_count;
+  if (n_bonds) *n_bonds = h->lat.n_bond_count;
+  if (n_colours) *n_colours = h->lat.n_colours;
+  return SCAMPER_OK;
+}
+
+int scamper_lattice_csr(const scamper_lattice *h, uint64_t *offsets, uint64_t *neighbors, uint64_t *edge_ids) {
+  if (!h) return fail(SCAMPER_EINVAL, "null lattice");
+  std::string err;
+  if (!materialize(const_cast<scamper_lattice *>(h)->lat, err)) return fail(SCAMPER_ENOMEM, "%s", err.c_str());
+  const Lattice &l = h->lat;
+  if (offsets) std::copy(l.offsets.begin(), l.offsets.end(), offsets);
+  if (neighbors) std::copy(l.neighbors.begin(), l.neighbors.end(), neighbors);
+  if (edge_ids) std::copy(l.edge_ids.begin(), l.edge_ids.end(), edge_ids);
+  return SCAMPER_OK;
+}
+
+int scamper_lattice_colours(const scamper_lattice *h, uint32_t *colour, uint64_t *class_index) {
+  if (!h) return fail(SCAMPER_EINVAL, "null lattice");
+  std::string err;
+  if (!materialize(const_cast<scamper_lattice *>(h)->lat, err)) return fail(SCAMPER_ENOMEM, "%s", err.c_str());
+  if (colour) std::copy(h->lat.colour.begin(), h->lat.colour.end(), colour);
+  if (class_index) std::copy(h->lat.class_index.begin(), h->lat.class_index.end(), class_index);
+  return SCAMPER_OK;
+}
+
+void scamper_lattice_free(scamper_lattice *h) { delete h; }
+
+// ---------------------------------------------------------------- state
+int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, uint32_t n_replicas,
+                         uint64_t seed, scamper_state **out) {
+  if (!lh || !model || !out) return fail(SCAMPER_EINVAL, "null argument");
+  if (n_replicas == 0 || n_replicas > 65535) return fail(SCAMPER_EINVAL, "n_replicas must be in 1..65535");
+  if (!std::isfinite(model->j)) return fail(SCAMPER_EINVAL, "coupling must be finite");
+  if (model->kind == kPotts && (model->q < 2 || model->q > kMaxQ))
+    return fail(model->q < 2 ? SCAMPER_EINVAL : SCAMPER_EUNSUPPORTED, "Potts q=%d (GPU path supports 2..%d)", model->q, kMaxQ);
+  if (model->kind != kIsing && model->kind != kPotts)
+    return fail(SCAMPER_EUNSUPPORTED, "model kind %d is not available on this lattice path", model->kind);
+  const Lattice &l = lh->lat;
+  if (!l.stencil)
+    return fail(SCAMPER_EUNSUPPORTED, "only periodic hypercubic lattices with even sides (<= 3-D) run on the stencil path");
+  auto st = std::unique_ptr<scamper_state>(new (std::nothrow) scamper_state);
+  if (!st) return fail(SCAMPER_ENOMEM, "state handle");
+  st->lat = l;
+  st->model = *model;
+  if (st->model.kind != kON) st->model.spin_dim = 1;
+  st->n_rep = n_replicas;
+  st->seed = seed;
+  st->beta.assign(n_replicas, 1.0);
+  st->keys.resize(n_replicas);
+  for (uint32_t r = 0; r < n_replicas; ++r) st->keys[r] = stream_seed(seed, r, 0);
+  st->geom.nd = l.nd;
+  st->geom.lx2 = (uint32_t)(l.dims[0] / 2);
+  st->geom.ly = (uint32_t)l.dims[1];
+  st->geom.lz = (uint32_t)l.dims[2];
+  st->geom.nh = l.n_sites / 2;
+  st->z = 2 * l.nd;
+  SC_HIP(hipGetDevice(&st->device));
+  SC_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+  const size_t bytes = (size_t)n_replicas * l.n_sites;
+  if (hipMalloc(&st->d_spins, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
+  SC_HIP(hipMalloc(&st->d_params, sizeof(ReplicaParams) * n_replicas));
+  SC_HIP(hipMalloc(&st->d_em, sizeof(long long) * kEmStride * n_replicas));
+  for (uint32_t r = 0; r < n_replicas; ++r) {
+    hipError_t e = launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream);
+    if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
+  }
+  SC_HIP(hipStreamSynchronize(st->stream));
+  *out = st.release();
+  return SCAMPER_OK;
+}
+
+void scamper_state_free(scamper_state *st) { delete st; }
+
+int scamper_state_set_beta(scamper_state *st, uint32_t replica, double beta) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!std::isfinite(beta) || beta < 0.0) return fail(SCAMPER_EINVAL, "beta must be finite and non-negative");
+  st->beta[replica] = beta;
+  st->tables_for = -1;
+  return SCAMPER_OK;
+}
+
+int scamper_state_get_beta(const scamper_state *st, uint32_t replica, double *beta) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!beta) return fail(SCAMPER_EINVAL, "null argument");
+  *beta = st->beta[replica];
+  return SCAMPER_OK;
+}
+
+int scamper_state_init(scamper_state *st, uint32_t replica, int random) {
+  if (int rc = check_state(st, replica)) return rc;
+  hipError_t e = launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream);
+  if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
+  return SCAMPER_OK;
+}
+
+int scamper_state_upload_i8(scamper_state *st, uint32_t replica, const int8_t *spins) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  return upload_i8_impl(st, replica, spins);
+}
+
+int scamper_state_download_i8(scamper_state *st, uint32_t replica, int8_t *spins) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  return download_i8_impl(st, replica, spins);
+}
+
+int scamper_state_upload_f64(scamper_state *st, uint32_t replica, const double *spins) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  const uint64_t n = st->lat.n_sites;
+  std::vector<int8_t> v(n);
+  for (uint64_t s = 0; s < n; ++s) {
+    const double x = spins[s];
+    // IsingModel::validate_spin / PottsModel::validate_spin (models.rs:62-77, 238-255)
+    const bool ok = st->model.kind == kIsing ? (x == 1.0 || x == -1.0)
+                                             : (std::isfinite(x) && x == std::floor(x) && x >= 0.0 && x < st->model.q);
+    if (!ok) return fail(SCAMPER_EINVAL, "invalid spin at site %llu: %g", (unsigned long long)s, x);
+    v[s] = (int8_t)x;
+  }
+  return upload_i8_impl(st, replica, v.data());
+}
+
+int scamper_state_download_f64(scamper_state *st, uint32_t replica, double *spins) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  std::vector<int8_t> v(st->lat.n_sites);
+  if (int rc = download_i8_impl(st, replica, v.data())) return rc;
+  for (uint64_t s = 0; s < st->lat.n_sites; ++s) spins[s] = (double)v[s];
+  return SCAMPER_OK;
+}
+
+int scamper_state_upload_f32(scamper_state *st, uint32_t, const float *) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
+}
+
+int scamper_state_download_f32(scamper_state *st, uint32_t, float *) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
+}
+
+int scamper_state_sweeps(const scamper_state *st, uint64_t *sweeps) {
+  if (!st || !sweeps) return fail(SCAMPER_EINVAL, "null argument");
+  *sweeps = st->sweeps;
+  return SCAMPER_OK;
+}
+
+int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  if (algorithm != kMetropolis && algorithm != kHeatBath)
+    return fail(SCAMPER_EUNSUPPORTED, "algorithm %d needs continuous spins", algorithm);
+  if (st->tables_for != algorithm)
+    if (int rc = upload_tables(st, algorithm)) return rc;
+  if (measure) {
+    if (int rc = ensure_series(st, n_sweeps)) return rc;
+    const size_t slot = (size_t)st->n_rep * kEmStride;
+    SC_HIP(hipMemsetAsync(st->d_series + st->series_len * slot, 0, n_sweeps * slot * sizeof(long long), st->stream));
+  }
+  for (uint64_t t = 0; t < n_sweeps; ++t) {
+    const uint64_t sweep = st->sweeps + t;
+    for (uint32_t c = 0; c < 2; ++c) {
+      long long *em = (measure && c == 1) ? st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride : nullptr;
+      hipError_t e = launch_stencil_discrete(st->geom, st->model.kind, st->model.q, algorithm, st->d_spins,
+                                             st->d_params, st->n_rep, c, sweep, st->d_hbexp, em, st->stream);
+      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+    }
+  }
+  st->sweeps += n_sweeps;
+  if (measure) st->series_len += n_sweeps;
+  return SCAMPER_OK;
+}
+
+int scamper_sweep_timed(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure, float *ms) {
+  if (!st || !ms) return fail(SCAMPER_EINVAL, "null argument");
+  if (st->tables_for != algorithm && (algorithm == kMetropolis || algorithm == kHeatBath))
+    if (int rc = upload_tables(st, algorithm)) return rc;
+  hipEvent_t a, b;
+  SC_HIP(hipEventCreate(&a));
+  SC_HIP(hipEventCreate(&b));
+  SC_HIP(hipEventRecord(a, st->stream));
+  int rc = scamper_sweep(st, algorithm, n_sweeps, measure);
+  if (rc == SCAMPER_OK) {
+    SC_HIP(hipEventRecord(b, st->stream));
+    SC_HIP(hipEventSynchronize(b));
+    SC_HIP(hipEventElapsedTime(ms, a, b));
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return rc;
+}
+
+int scamper_synchronize(scamper_state *st) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  SC_HIP(hipStreamSynchronize(st->stream));
+  return SCAMPER_OK;
+}
+
+int scamper_state_observables(scamper_state *st, uint32_t replica, double *energy, double *magnetization) {
+  if (int rc = check_state(st, replica)) return rc;
+  SC_HIP(hipMemsetAsync(st->d_em, 0, sizeof(long long) * kEmStride * st->n_rep, st->stream));
+  hipError_t e = launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);
+  if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure: %s", hipGetErrorString(e));
+  long long em[kEmStride];
+  SC_HIP(hipMemcpyAsync(em, st->d_em + (size_t)replica * kEmStride, sizeof em, hipMemcpyDeviceToHost, st->stream));
+  SC_HIP(hipStreamSynchronize(st->stream));
+  observables_from_em(st, em, energy, magnetization);
+  return SCAMPER_OK;
+}
+
+int scamper_series_length(const scamper_state *st, uint64_t *n) {
+  if (!st || !n) return fail(SCAMPER_EINVAL, "null argument");
+  *n = st->series_len;
+  return SCAMPER_OK;
+}
+
+int scamper_series_get(scamper_state *st, uint32_t replica, double *energy, double *magnetization, uint64_t max_n) {
+  if (int rc = check_state(st, replica)) return rc;
+  const uint64_t n = std::min(max_n, st->series_len);
+  if (n == 0) return SCAMPER_OK;
+  const size_t slot = (size_t)st->n_rep * kEmStride;
+  std::vector<long long> buf(n * slot);
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(buf.data(), st->d_series, buf.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  for (uint64_t t = 0; t < n; ++t)
+    observables_from_em(st, buf.data() + t * slot + (size_t)replica * kEmStride, energy ? energy + t : nullptr,
+                        magnetization ? magnetization + t : nullptr);
+  return SCAMPER_OK;
+}
+
+int scamper_series_clear(scamper_state *st) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  st->series_len = 0;
+  return SCAMPER_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// Internal interfaces between the host library (capi.cpp, host_*.cpp) and
+// the HIP kernels (k_*.hip).  Nothing here crosses the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.h"
+
+namespace scamper {
+
+// Model kinds (scamper_model.kind).
+enum ModelKind : int32_t { kIsing = 0, kPotts = 1, kON = 2 };
+
+// Update algorithms (scamper_sweep's `algorithm`).
+enum Algorithm : int32_t {
+  kMetropolis = 0,
+  kHeatBath = 1,
+  kContinuousHeatBath = 2,
+  kOverRelaxation = 3,
+  kHybridHeatBathOverRelaxation = 4,
+};
+
+constexpr int kMaxZ = 32;          // largest coordination the discrete kernels accept
+constexpr int kMaxTable = 2 * kMaxZ + 1;
+constexpr int kMaxQ = 16;          // largest Potts q on the GPU path
+
+// Per-replica device parameters.  `thr` is indexed by the kernel's local
+// energy class (see each kernel); `hb_exp` holds the heat-bath Potts weights
+// exp(lw(n) - lw(nmax)) indexed [n * (z+1) + nmax].
+struct alignas(16) ReplicaParams {
+  uint64_t key;
+  double beta;
+  Threshold thr[kMaxTable];
+};
+
+// Regular periodic hypercubic geometry with even sides, stored colour-split:
+// colour c of replica r occupies spins[(2r + c) * nh ...], rows of lx2 bytes,
+// row index = y + ly * z, element i of a row is site x = 2i + ((c+y+z)&1).
+struct StencilGeom {
+  int nd;        // 1, 2 or 3
+  uint32_t lx2;  // Lx / 2
+  uint32_t ly;
+  uint32_t lz;
+  uint64_t nh;   // sites per colour = lx2 * ly * lz
+};
+
+// Graph lattice (any CSR) with a greedy colouring.  Spins are site-major
+// int8 per replica; class_sites lists the sites of every colour class,
+// ascending, starting at class_start[c].
+struct CsrGeom {
+  uint64_t n_sites;
+  uint32_t n_colours;
+  int32_t weight_max;        // max sum of |w| over one site's non-loop incidences
+  const uint32_t *offsets;   // [n_sites+1]
+  const uint32_t *neighbors; // non-loop incidences only
+  const int8_t *weights;     // integer weights per incidence
+  const uint32_t *class_sites;
+};
+
+// Accumulated observables of one measured sweep of one replica:
+// em[0] = sum over bonds of w * s_i s_j (Ising) or w * delta(s_i,s_j) (Potts)
+// em[1] = sum of spins (Ising) or number of sites in state 0 .. q-1 at
+//         em[1 .. q] (Potts).
+constexpr int kEmStride = 2 + kMaxQ;
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_stencil_discrete(const StencilGeom &g, int model_kind, int q, int algorithm,
+                                   int8_t *spins, const ReplicaParams *reps, uint32_t n_rep,
+                                   uint32_t colour, uint64_t sweep, const double *hb_exp,
+                                   long long *em, hipStream_t stream);
+hipError_t launch_stencil_measure(const StencilGeom &g, int model_kind, int q,
+                                  const int8_t *spins, uint32_t n_rep, long long *em,
+                                  hipStream_t stream);
+hipError_t launch_stencil_init(const StencilGeom &g, int model_kind, int q, int8_t *spins,
+                               uint64_t key, uint32_t rep, int random, hipStream_t stream);
+bool stencil_fast_path_ok(const StencilGeom &g);
+
+}  // namespace scamper
