@@ -146,6 +146,40 @@ int scamper_series_get(scamper_state *st, uint32_t replica, double *energy,
                        double *magnetization, uint64_t max_n);
 int scamper_series_clear(scamper_state *st);
 
+/* ------------------------------------------------- statistics and driver */
+/* Estimate (Carlo.rs/src/estimate.rs:9-21). */
+typedef struct {
+  double mean;
+  double stderr_;
+  double autocorr_time;
+  uint64_t n_bins;
+} scamper_estimate;
+
+/* Accumulator::add over a whole series, then Accumulator::finalize
+ * (Carlo.rs/src/measurements.rs:107-123,162-182,244-280): bins of
+ * `binsize` samples, the partial bin included in the mean, stderr over bins,
+ * tau from one rebinning by two. */
+int scamper_binned_estimate(const double *samples, uint64_t n, uint64_t binsize,
+                            scamper_estimate *out);
+
+/* RunConfig (Carlo.rs/src/scheduler.rs:48-79). */
+typedef struct {
+  uint64_t thermalization_sweeps;
+  uint64_t measurement_sweeps;
+  uint64_t binsize;
+  uint64_t base_seed;
+} scamper_run_config;
+
+/* Scheduler::run_one::<ClassicalMC<model, algorithm>> (scheduler.rs:94-155,
+ * classical_mc.rs:427-490): one replica seeded with base_seed, cold or
+ * random start, thermalisation then measured sweeps on the device, and the
+ * DefaultObservableSet estimates out[0..4] = Energy, E2, Magnetization, M2,
+ * M4 (observables/common.rs:81-96).  The per-sweep series may be copied to
+ * energy / magnetization (each measurement_sweeps long) when not NULL. */
+int scamper_run_one(const scamper_lattice *lat, const scamper_model *model, int algorithm,
+                    double beta, int random_start, const scamper_run_config *cfg,
+                    scamper_estimate out[5], double *energy, double *magnetization);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,7 +56,8 @@ def _declare(L: C.CDLL) -> None:
         "orc_philox4x32_10": (None, [vp, vp, vp]),
         "orc_replica_key": (u64, [u64, u64]),
         "orc_exchange_key": (u64, [u64]),
-        "orc_half": (u32, [u64, u64, u64, u32, u32]),
+        "orc_byte": (u32, [u64, u64, u64, u32, u32, u32]),
+        "orc_low": (u32, [u64, u64, u64, u32, u32]),
         "orc_word": (u32, [u64, u64, u64, u32, u32]),
         "orc_cont_words": (None, [u64, u64, u64, u32, vp]),
         "orc_cont_init_words": (None, [u64, u64, vp]),
@@ -124,8 +125,12 @@ def word(key, j, sweep, colour, stream=0) -> int:
     return lib().orc_word(key, j, sweep, colour, stream)
 
 
-def half(key, j, sweep, colour, stream=0) -> int:
-    return lib().orc_half(key, j, sweep, colour, stream)
+def byte(key, j, sweep, colour, stream=0, part=0) -> int:
+    return lib().orc_byte(key, j, sweep, colour, stream, part)
+
+
+def low(key, j, sweep, colour, stream=0) -> int:
+    return lib().orc_low(key, j, sweep, colour, stream)
 
 
 def accept_log_probability(log_a: float, u: float) -> bool:

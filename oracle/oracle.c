@@ -8,16 +8,19 @@
  *
  * DRAW LAYOUT (this project's injected uniforms; the reference draws from a
  * sequential xoshiro256++ stream instead, which no test can replay):
- *   key   = RngStreamKey(seed).with_replica(replica).seed()   (64 bit)
- *   site  = (colour c, rank j of the site inside colour class c)
- *   ctr   = { j>>3, sweep lo32, (sweep>>32 & 0xffff) | c<<16 | stream<<24, j>>35 }
- *   half  = 16-bit lane (j & 7) of Philox4x32-10(ctr, key)
- *   word  = half(stream) << 16 | half(stream + 1)      uniform u = word * 2^-32
- * Streams: 0/1 acceptance (Metropolis, Ising and Potts heat bath),
- * 2/3 Potts proposal, 4 discrete random start (j = site, c = 0, sweep = 0),
- * 5 continuous-spin words (ctr[0] = j>>1, two words per site),
- * 6 continuous fallback (isotropic redraw), 7 replica exchange,
- * 8 continuous random start (ctr[0] = site).
+ *   key    = RngStreamKey(seed).with_replica(replica).seed()   (64 bit)
+ *   site   = (colour c, rank j of the site inside colour class c)
+ *   x(t)   = (sweep>>32 & 0xffff) | c<<16 | t<<24
+ *   top    = byte (j & 15) of Philox4x32-10({ j>>4, sweep lo32, x(s),   j>>36 }, key)
+ *   mid    = byte (j & 15) of Philox4x32-10({ j>>4, sweep lo32, x(s+1), j>>36 }, key)
+ *   low    = half (j & 7)  of Philox4x32-10({ j>>3, sweep lo32, x(s+2), j>>35 }, key)
+ *   word   = top << 24 | mid << 16 | low,   uniform u = word * 2^-32
+ * (byte k of a 32-bit word is bits 8k..8k+7, half k bits 16k..16k+15.)
+ * Streams s: 0 acceptance (Metropolis, Ising and Potts heat bath), 3 Potts
+ * proposal, 6 discrete random start (j = site, c = 0, sweep = 0), 7
+ * continuous-spin words (ctr[0] = j>>1, two words per site), 8 continuous
+ * fallback (isotropic redraw), 9 replica exchange, 10 continuous random
+ * start (ctr[0] = site).
  */
 #define _GNU_SOURCE
 #include "oracle.h"
@@ -32,12 +35,12 @@
 #endif
 
 #define ORC_STREAM_ACCEPT 0u
-#define ORC_STREAM_PROPOSE 2u
-#define ORC_STREAM_INIT 4u
-#define ORC_STREAM_CONT 5u
-#define ORC_STREAM_CONT_FALLBACK 6u
-#define ORC_STREAM_EXCHANGE 7u
-#define ORC_STREAM_CONT_INIT 8u
+#define ORC_STREAM_PROPOSE 3u
+#define ORC_STREAM_INIT 6u
+#define ORC_STREAM_CONT 7u
+#define ORC_STREAM_CONT_FALLBACK 8u
+#define ORC_STREAM_EXCHANGE 9u
+#define ORC_STREAM_CONT_INIT 10u
 
 static const double TWO_M32 = 2.3283064365386963e-10; /* 2^-32 */
 static const double TAU = 6.283185307179586;          /* std::f64::consts::TAU */
@@ -117,15 +120,22 @@ static void block(uint64_t key, uint32_t c0, uint64_t sweep, uint32_t colour, ui
   orc_philox4x32_10(ctr, k, out);
 }
 
-uint32_t orc_half(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream) {
+uint32_t orc_byte(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream,
+                  uint32_t part) {
   uint32_t out[4];
-  block(key, (uint32_t)(j >> 3), sweep, colour, stream, (uint32_t)(j >> 35), out);
-  return (out[(j & 7) >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+  block(key, (uint32_t)(j >> 4), sweep, colour, stream + part, (uint32_t)(j >> 36), out);
+  return (out[(j >> 2) & 3] >> (8 * (j & 3))) & 0xFFu;
 }
 
-uint32_t orc_word(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream_hi) {
-  return (orc_half(key, j, sweep, colour, stream_hi) << 16) |
-         orc_half(key, j, sweep, colour, stream_hi + 1);
+uint32_t orc_low(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream) {
+  uint32_t out[4];
+  block(key, (uint32_t)(j >> 3), sweep, colour, stream + 2, (uint32_t)(j >> 35), out);
+  return (out[(j >> 1) & 3] >> (16 * (j & 1))) & 0xFFFFu;
+}
+
+uint32_t orc_word(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream) {
+  return (orc_byte(key, j, sweep, colour, stream, 0) << 24) |
+         (orc_byte(key, j, sweep, colour, stream, 1) << 16) | orc_low(key, j, sweep, colour, stream);
 }
 
 void orc_cont_words(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t w[2]) {
@@ -795,9 +805,9 @@ void orc_sweep_over_relaxation(const orc_model *m, const orc_lattice *lat, doubl
 void orc_random_start(const orc_model *m, const orc_lattice *lat, uint64_t key, double *spins) {
   for (uint64_t s = 0; s < lat->n_sites; ++s) {
     if (m->kind == ORC_ISING) {
-      spins[s] = (orc_half(key, s, 0, 0, ORC_STREAM_INIT) & 1u) ? 1.0 : -1.0;
+      spins[s] = (orc_byte(key, s, 0, 0, ORC_STREAM_INIT, 0) & 1u) ? 1.0 : -1.0;
     } else if (m->kind == ORC_POTTS) {
-      spins[s] = (double)((orc_half(key, s, 0, 0, ORC_STREAM_INIT) * (uint32_t)m->q) >> 16);
+      spins[s] = (double)(((uint64_t)orc_word(key, s, 0, 0, ORC_STREAM_INIT) * (uint64_t)m->q) >> 32);
     } else {
       uint32_t w[4];
       orc_cont_init_words(key, s, w);
@@ -808,24 +818,27 @@ void orc_random_start(const orc_model *m, const orc_lattice *lat, uint64_t key, 
 
 /* ------------------------------------------- fast int8 stencil restatement */
 
-/* Per-thread cache of one Philox block (8 halves). */
+/* Per-thread cache of the top and mid blocks of 16 sites. */
 typedef struct {
   uint64_t group;
   uint32_t stream;
-  uint32_t out[4];
+  uint32_t top[4], mid[4];
   int valid;
-} half_cache;
+} top_cache;
 
-static uint32_t cached_half(half_cache *hc, uint64_t key, uint64_t j, uint64_t sweep,
-                            uint32_t c, uint32_t stream) {
-  uint64_t g = j >> 3;
-  if (!hc->valid || hc->group != g || hc->stream != stream) {
-    block(key, (uint32_t)g, sweep, c, stream, (uint32_t)(j >> 35), hc->out);
-    hc->group = g;
-    hc->stream = stream;
-    hc->valid = 1;
+static uint32_t cached_word(top_cache *tc, uint64_t key, uint64_t j, uint64_t sweep, uint32_t c,
+                            uint32_t stream) {
+  uint64_t g = j >> 4;
+  if (!tc->valid || tc->group != g || tc->stream != stream) {
+    block(key, (uint32_t)g, sweep, c, stream, (uint32_t)(j >> 36), tc->top);
+    block(key, (uint32_t)g, sweep, c, stream + 1, (uint32_t)(j >> 36), tc->mid);
+    tc->group = g;
+    tc->stream = stream;
+    tc->valid = 1;
   }
-  return (hc->out[(j & 7) >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+  uint32_t top = (tc->top[(j >> 2) & 3] >> (8 * (j & 3))) & 0xFFu;
+  uint32_t mid = (tc->mid[(j >> 2) & 3] >> (8 * (j & 3))) & 0xFFu;
+  return (top << 24) | (mid << 16) | orc_low(key, j, sweep, c, stream);
 }
 
 /* Colour-ordered sweeps on an int8 hypercubic periodic lattice with even
@@ -849,7 +862,7 @@ uint64_t orc_stencil_i8_sweeps(int8_t *spins, const uint64_t *dims, int n_dims, 
     for (uint32_t c = 0; c < 2; ++c) {
 #pragma omp parallel for schedule(static) reduction(+ : accepted)
       for (uint64_t row = 0; row < rows; ++row) {
-        half_cache ha = {0}, hl = {0}, hp = {0}, hq = {0};
+        top_cache ha = {0}, hp = {0};
         uint64_t y = row % ly, z = row / ly;
         uint64_t ym = (y + ly - 1) % ly, yp = (y + 1) % ly;
         uint64_t zm = (z + lz - 1) % lz, zp = (z + 1) % lz;
@@ -877,8 +890,7 @@ uint64_t orc_stencil_i8_sweeps(int8_t *spins, const uint64_t *dims, int n_dims, 
               double field = j * (double)sum;
               double x2 = 2.0 * beta * field;
               double pp = (x2 >= 0.0) ? 1.0 / (1.0 + exp(-x2)) : exp(x2) / (1.0 + exp(x2));
-              uint32_t w = (cached_half(&ha, key, jj, sweep, c, 0) << 16) |
-                           cached_half(&hl, key, jj, sweep, c, 1);
+              uint32_t w = cached_word(&ha, key, jj, sweep, c, ORC_STREAM_ACCEPT);
               spins[site] = ((double)w * TWO_M32 < pp) ? 1 : -1;
             } else {
               double de = 2.0 * j * (double)s * (double)sum;
@@ -887,8 +899,7 @@ uint64_t orc_stencil_i8_sweeps(int8_t *spins, const uint64_t *dims, int n_dims, 
               if (la >= 0.0)
                 acc = 1;
               else {
-                uint32_t w = (cached_half(&ha, key, jj, sweep, c, 0) << 16) |
-                             cached_half(&hl, key, jj, sweep, c, 1);
+                uint32_t w = cached_word(&ha, key, jj, sweep, c, ORC_STREAM_ACCEPT);
                 acc = orc_accept_log_probability(la, (double)w * TWO_M32);
               }
               if (acc) {
@@ -912,8 +923,7 @@ uint64_t orc_stencil_i8_sweeps(int8_t *spins, const uint64_t *dims, int n_dims, 
                 lw[k] = exp(lw[k] - mx);
                 total += lw[k];
               }
-              uint32_t w = (cached_half(&ha, key, jj, sweep, c, 0) << 16) |
-                           cached_half(&hl, key, jj, sweep, c, 1);
+              uint32_t w = cached_word(&ha, key, jj, sweep, c, ORC_STREAM_ACCEPT);
               double thr = (double)w * TWO_M32 * total;
               int ns = q - 1;
               for (int k = 0; k < q; ++k) {
@@ -925,8 +935,7 @@ uint64_t orc_stencil_i8_sweeps(int8_t *spins, const uint64_t *dims, int n_dims, 
               }
               spins[site] = (int8_t)ns;
             } else {
-              uint32_t pw = (cached_half(&hp, key, jj, sweep, c, 2) << 16) |
-                            cached_half(&hq, key, jj, sweep, c, 3);
+              uint32_t pw = cached_word(&hp, key, jj, sweep, c, ORC_STREAM_PROPOSE);
               uint64_t ns = potts_propose((uint64_t)(uint8_t)s, q, pw);
               double de = -j * (double)cnt[ns] - (-j * (double)cnt[(uint8_t)s]);
               double la = orc_log_acceptance(beta, de);
@@ -934,8 +943,7 @@ uint64_t orc_stencil_i8_sweeps(int8_t *spins, const uint64_t *dims, int n_dims, 
               if (la >= 0.0)
                 acc = 1;
               else {
-                uint32_t w = (cached_half(&ha, key, jj, sweep, c, 0) << 16) |
-                             cached_half(&hl, key, jj, sweep, c, 1);
+                uint32_t w = cached_word(&ha, key, jj, sweep, c, ORC_STREAM_ACCEPT);
                 acc = orc_accept_log_probability(la, (double)w * TWO_M32);
               }
               if (acc) {

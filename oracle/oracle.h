@@ -35,10 +35,13 @@ uint64_t orc_replica_key(uint64_t seed, uint64_t replica);
 /* Key of the replica-exchange stream of a job seeded with `seed`. */
 uint64_t orc_exchange_key(uint64_t seed);
 
-/* 16-bit site draw (see DRAW LAYOUT in oracle.c). */
-uint32_t orc_half(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream);
-/* 32-bit site draw = half(stream_hi) << 16 | half(stream_hi + 1). */
-uint32_t orc_word(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream_hi);
+/* Top (part 0) or mid (part 1) byte and low 16 bits of a site draw (see
+ * DRAW LAYOUT in oracle.c). */
+uint32_t orc_byte(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream,
+                  uint32_t part);
+uint32_t orc_low(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream);
+/* 32-bit site draw = top << 24 | mid << 16 | low. */
+uint32_t orc_word(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream);
 /* Two 32-bit words of the continuous-spin stream for site j. */
 void orc_cont_words(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t w[2]);
 /* Four 32-bit words for the isotropic random start of continuous spins. */

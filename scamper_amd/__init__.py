@@ -29,6 +29,16 @@ class ScamperError(RuntimeError):
         self.code = code
 
 
+class Estimate(C.Structure):
+    _fields_ = [("mean", C.c_double), ("stderr", C.c_double),
+                ("autocorr_time", C.c_double), ("n_bins", C.c_uint64)]
+
+
+class RunConfig(C.Structure):
+    _fields_ = [("thermalization_sweeps", C.c_uint64), ("measurement_sweeps", C.c_uint64),
+                ("binsize", C.c_uint64), ("base_seed", C.c_uint64)]
+
+
 class Model(C.Structure):
     _fields_ = [("kind", C.c_int32), ("q", C.c_int32), ("spin_dim", C.c_int32),
                 ("reserved", C.c_int32), ("j", C.c_double)]
@@ -70,6 +80,9 @@ SIGNATURES = {
     "scamper_series_length": (i32, [vp, vp]),
     "scamper_series_get": (i32, [vp, u32, vp, vp, u64]),
     "scamper_series_clear": (i32, [vp]),
+    "scamper_binned_estimate": (i32, [vp, u64, u64, C.POINTER(Estimate)]),
+    "scamper_run_one": (i32, [vp, C.POINTER(Model), i32, dbl, i32, C.POINTER(RunConfig),
+                              vp, vp, vp]),
 }
 
 
@@ -254,3 +267,28 @@ class State:
 
     def clear_series(self):
         check(lib().scamper_series_clear(self.h))
+
+
+OBSERVABLES = ("Energy", "E2", "Magnetization", "M2", "M4")
+
+
+def binned_estimate(samples, binsize: int) -> Estimate:
+    """Accumulator + Estimate (Carlo.rs/src/measurements.rs:60-280)."""
+    a = np.ascontiguousarray(samples, dtype=np.float64)
+    out = Estimate()
+    check(lib().scamper_binned_estimate(_p(a), a.size, binsize, C.byref(out)))
+    return out
+
+
+def run_one(lattice: Lattice, m: Model, algorithm: int, beta: float, random_start: bool,
+            thermalization_sweeps: int, measurement_sweeps: int, binsize: int = 100,
+            base_seed: int = 0, series: bool = False):
+    """Scheduler::run_one (Carlo.rs/src/scheduler.rs:94-155) on the device."""
+    cfg = RunConfig(thermalization_sweeps, measurement_sweeps, binsize, base_seed)
+    est = (Estimate * 5)()
+    e = np.zeros(measurement_sweeps) if series else None
+    mm = np.zeros(measurement_sweeps) if series else None
+    check(lib().scamper_run_one(lattice.h, C.byref(m), algorithm, beta, int(random_start),
+                                C.byref(cfg), est, _p(e), _p(mm)))
+    res = {k: est[i] for i, k in enumerate(OBSERVABLES)}
+    return (res, e, mm) if series else res

@@ -18,10 +18,10 @@
 using namespace scamper;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const char *fmt, ...) {
+int scamper::fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -30,6 +30,8 @@ int fail(int code, const char *fmt, ...) {
   g_last_error = buf;
   return code;
 }
+
+namespace {
 
 #define SC_HIP(call)                                                                   \
   do {                                                                                 \

@@ -40,6 +40,9 @@ struct Lattice {
   uint64_t degree(uint64_t s) const { return offsets[s + 1] - offsets[s]; }
 };
 
+// Record the calling thread's last error (scamper_last_error) and return code.
+int fail(int code, const char *fmt, ...);
+
 // Builders; on failure return false and set `err`.
 bool lattice_from_edges(uint64_t n_sites, std::vector<Bond> edges, Lattice &out, std::string &err);
 bool lattice_hypercubic(const uint64_t *dims, int n_dims, bool pbc, Lattice &out, std::string &err);

@@ -73,6 +73,10 @@ hipError_t launch_stencil_measure(const StencilGeom &g, int model_kind, int q,
                                   hipStream_t stream);
 hipError_t launch_stencil_init(const StencilGeom &g, int model_kind, int q, int8_t *spins,
                                uint64_t key, uint32_t rep, int random, hipStream_t stream);
-bool stencil_fast_path_ok(const StencilGeom &g);
+// Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
+bool ising_fast_path_ok(const StencilGeom &g);
+hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,
+                             uint32_t n_rep, uint32_t colour, uint64_t sweep, long long *em,
+                             hipStream_t stream);
 
 }  // namespace scamper

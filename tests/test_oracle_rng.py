@@ -54,17 +54,20 @@ def test_draw_layout(orc):
     key = orc.replica_key(42, 3)
     k = [key & 0xFFFFFFFF, key >> 32]
     sweep, colour = (5 << 32) | 17, 1
-    for j in [0, 1, 7, 8, 9, 12345, 2**35 + 3]:
-        ctr = [(j >> 3) & 0xFFFFFFFF, sweep & 0xFFFFFFFF,
-               ((sweep >> 32) & 0xFFFF) | (colour << 16) | (0 << 24), (j >> 35) & 0xFFFFFFFF]
-        out = _philox_py(ctr, k)
-        lane = j & 7
-        want = (out[lane >> 1] >> (16 * (lane & 1))) & 0xFFFF
-        assert orc.half(key, j, sweep, colour, 0) == want
-        ctr[2] |= 1 << 24
-        out1 = _philox_py(ctr, k)
-        lo = (out1[lane >> 1] >> (16 * (lane & 1))) & 0xFFFF
-        assert orc.word(key, j, sweep, colour, 0) == (want << 16) | lo
+    s = 3
+    for j in [0, 1, 7, 8, 9, 15, 16, 12345, 2**36 + 3, 2**37 + 19]:
+        x = ((sweep >> 32) & 0xFFFF) | (colour << 16)
+        ctr = [(j >> 4) & 0xFFFFFFFF, sweep & 0xFFFFFFFF, x | (s << 24), (j >> 36) & 0xFFFFFFFF]
+        top = (_philox_py(ctr, k)[(j >> 2) & 3] >> (8 * (j & 3))) & 0xFF
+        ctr[2] = x | ((s + 1) << 24)
+        mid = (_philox_py(ctr, k)[(j >> 2) & 3] >> (8 * (j & 3))) & 0xFF
+        assert orc.byte(key, j, sweep, colour, s, 0) == top
+        assert orc.byte(key, j, sweep, colour, s, 1) == mid
+        out2 = _philox_py([(j >> 3) & 0xFFFFFFFF, sweep & 0xFFFFFFFF, x | ((s + 2) << 24),
+                           (j >> 35) & 0xFFFFFFFF], k)
+        low = (out2[(j >> 1) & 3] >> (16 * (j & 1))) & 0xFFFF
+        assert orc.low(key, j, sweep, colour, s) == low
+        assert orc.word(key, j, sweep, colour, s) == (top << 24) | (mid << 16) | low
 
 
 def test_stream_seed_properties(orc):
@@ -105,10 +108,10 @@ def test_stream_seed_python_restatement(orc):
             assert orc.replica_key(b, r) == seed(b, r)
 
 
-def test_uniformity_of_halves(orc):
+def test_uniformity_of_words(orc):
     key = orc.replica_key(1, 0)
-    h = np.array([orc.half(key, j, 0, 0, 0) for j in range(20000)], dtype=np.float64)
-    assert abs(h.mean() / 65535.0 - 0.5) < 0.01
-    counts = np.bincount((h // 4096).astype(int), minlength=16)
+    h = np.array([orc.word(key, j, 0, 0, 0) for j in range(20000)], dtype=np.float64)
+    assert abs(h.mean() / 2.0**32 - 0.5) < 0.01
+    counts = np.bincount((h // 2.0**28).astype(int), minlength=16)
     chi2 = ((counts - 1250.0) ** 2 / 1250.0).sum()
     assert chi2 < 45  # 15 dof, p ~ 1e-4
