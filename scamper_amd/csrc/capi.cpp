@@ -73,6 +73,15 @@ struct scamper_state {
   std::vector<uint64_t> keys;
   StencilGeom geom{};
   int z = 0;
+  // graph lattices (any CSR): integer weights, greedy colour classes
+  bool csr = false;
+  CsrGeom cgeom{};
+  std::vector<uint64_t> class_start;
+  int w_max = 0;
+  bool unit_weights = true;
+  double self_loop_weight = 0.0;  // sum of w over self-loops (constant energy)
+  uint32_t *d_offsets = nullptr, *d_neighbors = nullptr, *d_class_sites = nullptr;
+  int8_t *d_weights = nullptr;
   hipStream_t stream = nullptr;
   int8_t *d_spins = nullptr;
   ReplicaParams *d_params = nullptr;
@@ -90,6 +99,10 @@ struct scamper_state {
     hipFree(d_hbexp);
     hipFree(d_em);
     hipFree(d_series);
+    hipFree(d_offsets);
+    hipFree(d_neighbors);
+    hipFree(d_class_sites);
+    hipFree(d_weights);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -106,13 +119,18 @@ int check_state(const scamper_state *st, uint32_t replica) {
 int upload_tables(scamper_state *st, int alg) {
   std::vector<ReplicaParams> p(st->n_rep);
   std::vector<double> hb;
-  const int z = st->z;
+  const int z = st->csr ? st->w_max : st->z;
+  if (st->csr && st->model.kind == kPotts && alg == kHeatBath && !st->unit_weights)
+    return fail(SCAMPER_EUNSUPPORTED, "Potts heat bath on a graph lattice needs unit bond weights");
   if (st->model.kind == kPotts && alg == kHeatBath) hb.assign((size_t)st->n_rep * (kMaxZ + 1) * (kMaxZ + 1), 0.0);
   for (uint32_t r = 0; r < st->n_rep; ++r) {
     memset(&p[r], 0, sizeof(ReplicaParams));
     p[r].key = st->keys[r];
     p[r].beta = st->beta[r];
-    if (st->model.kind == kIsing) {
+    if (st->csr && st->model.kind == kIsing) {
+      if (alg == kMetropolis) csr_ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
+      else csr_ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
+    } else if (st->model.kind == kIsing) {
       if (alg == kMetropolis) ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
       else ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
     } else if (st->model.kind == kPotts) {
@@ -162,21 +180,29 @@ int upload_i8_impl(scamper_state *st, uint32_t replica, const int8_t *spins) {
     if (st->model.kind == kIsing ? (v != 1 && v != -1) : (v < 0 || v >= st->model.q))
       return fail(SCAMPER_EINVAL, "invalid spin at site %llu: %d", (unsigned long long)s, (int)v);
   }
+  SC_HIP(hipStreamSynchronize(st->stream));
+  if (st->csr) {
+    SC_HIP(hipMemcpy(st->d_spins + (uint64_t)replica * n, spins, n, hipMemcpyHostToDevice));
+    return SCAMPER_OK;
+  }
   std::vector<int8_t> split(n);
   for (uint64_t s = 0; s < n; ++s) {
     uint32_t c;
     const uint64_t i = split_index(st->geom, s, &c);
     split[c * st->geom.nh + i] = spins[s];
   }
-  SC_HIP(hipStreamSynchronize(st->stream));
   SC_HIP(hipMemcpy(st->d_spins + 2ull * replica * st->geom.nh, split.data(), n, hipMemcpyHostToDevice));
   return SCAMPER_OK;
 }
 
 int download_i8_impl(scamper_state *st, uint32_t replica, int8_t *spins) {
   const uint64_t n = st->lat.n_sites;
-  std::vector<int8_t> split(n);
   SC_HIP(hipStreamSynchronize(st->stream));
+  if (st->csr) {
+    SC_HIP(hipMemcpy(spins, st->d_spins + (uint64_t)replica * n, n, hipMemcpyDeviceToHost));
+    return SCAMPER_OK;
+  }
+  std::vector<int8_t> split(n);
   SC_HIP(hipMemcpy(split.data(), st->d_spins + 2ull * replica * st->geom.nh, n, hipMemcpyDeviceToHost));
   for (uint64_t s = 0; s < n; ++s) {
     uint32_t c;
@@ -187,9 +213,16 @@ int download_i8_impl(scamper_state *st, uint32_t replica, int8_t *spins) {
 }
 
 // Energy and magnetisation from the accumulated counters of one replica.
+// On the stencil every bond is counted once (from its colour-1 end); on a
+// graph both ends count it, and self-loops add the constant -J w.
 void observables_from_em(const scamper_state *st, const long long *em, double *e, double *m) {
   const double n = (double)st->lat.n_sites;
-  if (e) *e = -st->model.j * (double)em[0];
+  if (e) {
+    if (st->csr)
+      *e = -st->model.j * ((double)(em[0] / 2) + st->self_loop_weight);
+    else
+      *e = -st->model.j * (double)em[0];
+  }
   if (!m) return;
   if (st->model.kind == kIsing) {
     *m = std::fabs((double)em[1]) / n;
@@ -310,11 +343,8 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   if (model->kind != kIsing && model->kind != kPotts)
     return fail(SCAMPER_EUNSUPPORTED, "model kind %d is not available on this lattice path", model->kind);
   const Lattice &l = lh->lat;
-  if (!l.stencil)
-    return fail(SCAMPER_EUNSUPPORTED, "only periodic hypercubic lattices with even sides (<= 3-D) run on the stencil path");
   auto st = std::unique_ptr<scamper_state>(new (std::nothrow) scamper_state);
   if (!st) return fail(SCAMPER_ENOMEM, "state handle");
-  st->lat = l;
   st->model = *model;
   if (st->model.kind != kON) st->model.spin_dim = 1;
   st->n_rep = n_replicas;
@@ -322,20 +352,67 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   st->beta.assign(n_replicas, 1.0);
   st->keys.resize(n_replicas);
   for (uint32_t r = 0; r < n_replicas; ++r) st->keys[r] = stream_seed(seed, r, 0);
-  st->geom.nd = l.nd;
-  st->geom.lx2 = (uint32_t)(l.dims[0] / 2);
-  st->geom.ly = (uint32_t)l.dims[1];
-  st->geom.lz = (uint32_t)l.dims[2];
-  st->geom.nh = l.n_sites / 2;
-  st->z = 2 * l.nd;
   SC_HIP(hipGetDevice(&st->device));
   SC_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+  if (l.stencil) {
+    st->lat = l;
+    st->geom.nd = l.nd;
+    st->geom.lx2 = (uint32_t)(l.dims[0] / 2);
+    st->geom.ly = (uint32_t)l.dims[1];
+    st->geom.lz = (uint32_t)l.dims[2];
+    st->geom.nh = l.n_sites / 2;
+    st->z = 2 * l.nd;
+  } else {
+    // Graph lattice: integer weights |w| <= 127, weighted degree <= 32,
+    // self-loops kept aside (constant energy, never in a local field).
+    if (l.n_sites >= (1ull << 32) || l.neighbors.size() >= (1ull << 32))
+      return fail(SCAMPER_EUNSUPPORTED, "graph lattices are limited to 2^32 sites and incidences");
+    st->csr = true;
+    st->lat = l;
+    std::vector<uint32_t> off(l.n_sites + 1, 0), nb;
+    std::vector<int8_t> w;
+    nb.reserve(l.neighbors.size());
+    w.reserve(l.neighbors.size());
+    for (uint64_t s = 0; s < l.n_sites; ++s) {
+      int wsum = 0;
+      for (uint64_t i = l.offsets[s]; i < l.offsets[s + 1]; ++i) {
+        const Bond &b = l.edges[l.edge_ids[i]];
+        if (b.weight != std::floor(b.weight) || std::fabs(b.weight) > 127.0)
+          return fail(SCAMPER_EUNSUPPORTED, "bond weights must be integers with |w| <= 127 on the GPU path");
+        if (b.weight != 1.0) st->unit_weights = false;
+        if (b.source == b.target) continue;
+        nb.push_back((uint32_t)l.neighbors[i]);
+        w.push_back((int8_t)b.weight);
+        wsum += (int)std::fabs(b.weight);
+      }
+      off[s + 1] = (uint32_t)nb.size();
+      st->w_max = std::max(st->w_max, wsum);
+    }
+    if (st->w_max > kMaxZ) return fail(SCAMPER_EUNSUPPORTED, "weighted degree %d exceeds %d", st->w_max, kMaxZ);
+    for (const Bond &b : l.edges)
+      if (b.source == b.target) st->self_loop_weight += b.weight;
+    std::vector<uint32_t> cls(l.class_sites.begin(), l.class_sites.end());
+    st->class_start = l.class_start;
+    SC_HIP(hipMalloc(&st->d_offsets, off.size() * 4));
+    SC_HIP(hipMalloc(&st->d_neighbors, std::max<size_t>(nb.size(), 1) * 4));
+    SC_HIP(hipMalloc(&st->d_weights, std::max<size_t>(w.size(), 1)));
+    SC_HIP(hipMalloc(&st->d_class_sites, cls.size() * 4));
+    SC_HIP(hipMemcpy(st->d_offsets, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    if (!nb.empty()) {
+      SC_HIP(hipMemcpy(st->d_neighbors, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
+      SC_HIP(hipMemcpy(st->d_weights, w.data(), w.size(), hipMemcpyHostToDevice));
+    }
+    SC_HIP(hipMemcpy(st->d_class_sites, cls.data(), cls.size() * 4, hipMemcpyHostToDevice));
+    st->cgeom = CsrGeom{l.n_sites, l.n_colours, st->w_max, st->d_offsets, st->d_neighbors, st->d_weights,
+                        st->d_class_sites};
+  }
   const size_t bytes = (size_t)n_replicas * l.n_sites;
   if (hipMalloc(&st->d_spins, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
   SC_HIP(hipMalloc(&st->d_params, sizeof(ReplicaParams) * n_replicas));
   SC_HIP(hipMalloc(&st->d_em, sizeof(long long) * kEmStride * n_replicas));
   for (uint32_t r = 0; r < n_replicas; ++r) {
-    hipError_t e = launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream);
+    hipError_t e = st->csr ? launch_csr_init(l.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream)
+                           : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream);
     if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
   }
   SC_HIP(hipStreamSynchronize(st->stream));
@@ -362,7 +439,8 @@ int scamper_state_get_beta(const scamper_state *st, uint32_t replica, double *be
 
 int scamper_state_init(scamper_state *st, uint32_t replica, int random) {
   if (int rc = check_state(st, replica)) return rc;
-  hipError_t e = launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream);
+  hipError_t e = st->csr ? launch_csr_init(st->lat.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream)
+                         : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
   return SCAMPER_OK;
 }
@@ -433,6 +511,15 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
   }
   for (uint64_t t = 0; t < n_sweeps; ++t) {
     const uint64_t sweep = st->sweeps + t;
+    if (st->csr) {
+      hipError_t e = launch_csr_discrete(st->cgeom, st->class_start.data(), st->model.kind, st->model.q, algorithm,
+                                         st->d_spins, st->d_params, st->n_rep, sweep, st->d_hbexp, st->stream);
+      if (e == hipSuccess && measure)
+        e = launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep,
+                               st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride, st->stream);
+      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+      continue;
+    }
     for (uint32_t c = 0; c < 2; ++c) {
       long long *em = (measure && c == 1) ? st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride : nullptr;
       hipError_t e = launch_stencil_discrete(st->geom, st->model.kind, st->model.q, algorithm, st->d_spins,
@@ -473,7 +560,8 @@ int scamper_synchronize(scamper_state *st) {
 int scamper_state_observables(scamper_state *st, uint32_t replica, double *energy, double *magnetization) {
   if (int rc = check_state(st, replica)) return rc;
   SC_HIP(hipMemsetAsync(st->d_em, 0, sizeof(long long) * kEmStride * st->n_rep, st->stream));
-  hipError_t e = launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);
+  hipError_t e = st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream)
+                         : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure: %s", hipGetErrorString(e));
   long long em[kEmStride];
   SC_HIP(hipMemcpyAsync(em, st->d_em + (size_t)replica * kEmStride, sizeof em, hipMemcpyDeviceToHost, st->stream));

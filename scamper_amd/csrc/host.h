@@ -65,6 +65,8 @@ void ising_metropolis_table(double beta, double j, int z, Threshold *thr);
 void ising_heat_bath_table(double beta, double j, int z, Threshold *thr);
 void potts_metropolis_table(double beta, double j, int z, Threshold *thr);
 void potts_heat_bath_exp(double beta, double j, int z, double *table /* (z+1)^2 */);
+void csr_ising_metropolis_table(double beta, double j, int w_max, Threshold *thr);
+void csr_ising_heat_bath_table(double beta, double j, int w_max, Threshold *thr);
 
 }  // namespace scamper
 

@@ -73,6 +73,14 @@ hipError_t launch_stencil_measure(const StencilGeom &g, int model_kind, int q,
                                   hipStream_t stream);
 hipError_t launch_stencil_init(const StencilGeom &g, int model_kind, int q, int8_t *spins,
                                uint64_t key, uint32_t rep, int random, hipStream_t stream);
+hipError_t launch_csr_discrete(const CsrGeom &g, const uint64_t *class_start_host, int kind, int q,
+                               int alg, int8_t *spins, const ReplicaParams *reps, uint32_t n_rep,
+                               uint64_t sweep, const double *hb_exp, hipStream_t stream);
+hipError_t launch_csr_measure(const CsrGeom &g, int kind, int q, const int8_t *spins, uint32_t n_rep,
+                              long long *em, hipStream_t stream);
+hipError_t launch_csr_init(uint64_t n_sites, int kind, int q, int8_t *spins, uint64_t key, uint32_t rep,
+                           int random, hipStream_t stream);
+
 // Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
 bool ising_fast_path_ok(const StencilGeom &g);
 hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,

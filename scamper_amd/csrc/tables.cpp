@@ -90,6 +90,26 @@ void ising_heat_bath_table(double beta, double j, int z, Threshold *thr) {
   }
 }
 
+// Graph lattices with integer bond weights: a site's weighted field
+// h = sum w s_nb lies in [-W, W].  Metropolis is indexed by x = s h + W
+// (ΔE = 2 J s h: every partial sum of ±J w is an integer multiple of J),
+// heat bath by h + W.
+void csr_ising_metropolis_table(double beta, double j, int w_max, Threshold *thr) {
+  for (int x = -w_max; x <= w_max; ++x) {
+    const double e_old = -j * (double)x, e_new = j * (double)x;  // local energies of s and -s
+    thr[x + w_max] = make_threshold(accept_threshold(log_acceptance(beta, e_new - e_old)));
+  }
+}
+
+void csr_ising_heat_bath_table(double beta, double j, int w_max, Threshold *thr) {
+  for (int h = -w_max; h <= w_max; ++h) {
+    const double field = j * (double)h;
+    const double x = 2.0 * beta * field;
+    const double p = x >= 0.0 ? 1.0 / (1.0 + std::exp(-x)) : std::exp(x) / (1.0 + std::exp(x));
+    thr[h + w_max] = make_threshold(below_threshold(p));
+  }
+}
+
 // Potts Metropolis (models.rs:230-236): ΔE = -J (n_new - n_old), indexed by
 // d = n_new - n_old + z.
 void potts_metropolis_table(double beta, double j, int z, Threshold *thr) {
