@@ -146,6 +146,48 @@ int scamper_series_get(scamper_state *st, uint32_t replica, double *energy,
                        double *magnetization, uint64_t max_n);
 int scamper_series_clear(scamper_state *st);
 
+/* ----------------------------------------------- multi-spin + tempering */
+typedef struct scamper_multispin scamper_multispin;
+
+/* MultiSpinIsing::new (CMC.rs/src/multi_spin.rs:50-101) for 64 * n_bundles
+ * Ising replicas packed one bit per replica in u64 site words (bit set =
+ * spin +1), with one temperature label per bit instead of the reference's
+ * single shared beta.  Replica 64 b + r draws from the key of replica
+ * 64 b + r, so each bit reproduces the single-replica state's trajectory.
+ * The lattice must be a periodic hypercube with even sides (the reference
+ * requires uniform degree and unit, loop-free bonds).  Starts cold. */
+int scamper_multispin_create(const scamper_lattice *lat, double j, uint32_t n_bundles, uint64_t seed,
+                             scamper_multispin **out);
+void scamper_multispin_free(scamper_multispin *ms);
+/* beta of every temperature label (64 * n_bundles values); replica r holds
+ * label r until exchanges move labels (change_parameter, multi_spin.rs:360-375). */
+int scamper_multispin_set_betas(scamper_multispin *ms, const double *beta_by_label);
+/* Cold (all +1) or random start, bit r of site s = start of replica r. */
+int scamper_multispin_init(scamper_multispin *ms, int random);
+/* Site words, [bundle][site] site-major. */
+int scamper_multispin_upload(scamper_multispin *ms, const uint64_t *words);
+int scamper_multispin_download(scamper_multispin *ms, uint64_t *words);
+/* n_steps of: one colour-ordered Metropolis sweep of every bit, then, when
+ * exchange != 0, one replica-exchange round over the label ladder with
+ * pairing offset (step-1)&1 and accept_log_probability(w_lower + w_upper)
+ * (Carlo.rs/src/parallel_tempering.rs:362-419; log_weight_ratio
+ * (beta_self - beta_partner) E_self, multi_spin.rs:352-358 per bit).  With
+ * measure != 0 each step's per-replica energy and |m| are recorded. */
+int scamper_multispin_step(scamper_multispin *ms, uint64_t n_steps, int exchange, int measure);
+int scamper_multispin_step_timed(scamper_multispin *ms, uint64_t n_steps, int exchange, int measure,
+                                 float *ms_out);
+/* Temperature label held by each replica. */
+int scamper_multispin_labels(scamper_multispin *ms, int32_t *labels);
+/* Accepted exchanges, decisions whose ln(u) fell within 1e-13 relative of
+ * the log ratio (where the device and host libm could disagree), and
+ * exchange rounds so far. */
+int scamper_multispin_stats(scamper_multispin *ms, uint64_t *accepted, uint64_t *near_ties,
+                            uint64_t *exchange_steps);
+/* Recorded steps: energy[step][replica], magnetization[step][replica]
+ * (multi_spin.rs:220-286 "Energy_replica", "Magnetization_replica"). */
+int scamper_multispin_series(scamper_multispin *ms, double *energy, double *magnetization,
+                             uint64_t max_steps, uint64_t *n_steps);
+
 /* ------------------------------------------------- statistics and driver */
 /* Estimate (Carlo.rs/src/estimate.rs:9-21). */
 typedef struct {

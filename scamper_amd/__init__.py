@@ -80,6 +80,17 @@ SIGNATURES = {
     "scamper_series_length": (i32, [vp, vp]),
     "scamper_series_get": (i32, [vp, u32, vp, vp, u64]),
     "scamper_series_clear": (i32, [vp]),
+    "scamper_multispin_create": (i32, [vp, dbl, u32, u64, PP]),
+    "scamper_multispin_free": (None, [vp]),
+    "scamper_multispin_set_betas": (i32, [vp, vp]),
+    "scamper_multispin_init": (i32, [vp, i32]),
+    "scamper_multispin_upload": (i32, [vp, vp]),
+    "scamper_multispin_download": (i32, [vp, vp]),
+    "scamper_multispin_step": (i32, [vp, u64, i32, i32]),
+    "scamper_multispin_step_timed": (i32, [vp, u64, i32, i32, vp]),
+    "scamper_multispin_labels": (i32, [vp, vp]),
+    "scamper_multispin_stats": (i32, [vp, vp, vp, vp]),
+    "scamper_multispin_series": (i32, [vp, vp, vp, u64, vp]),
     "scamper_binned_estimate": (i32, [vp, u64, u64, C.POINTER(Estimate)]),
     "scamper_run_one": (i32, [vp, C.POINTER(Model), i32, dbl, i32, C.POINTER(RunConfig),
                               vp, vp, vp]),
@@ -292,3 +303,65 @@ def run_one(lattice: Lattice, m: Model, algorithm: int, beta: float, random_star
                                 C.byref(cfg), est, _p(e), _p(mm)))
     res = {k: est[i] for i, k in enumerate(OBSERVABLES)}
     return (res, e, mm) if series else res
+
+
+class MultiSpin:
+    """64 * n_bundles Ising replicas, one bit each, with device-side replica
+    exchange (MultiSpinIsing, CMC.rs/src/multi_spin.rs:28-376)."""
+
+    def __init__(self, lattice: Lattice, j: float = 1.0, n_bundles: int = 1, seed: int = 0):
+        out = C.c_void_p()
+        check(lib().scamper_multispin_create(lattice.h, j, n_bundles, seed, C.byref(out)))
+        self.h = out.value
+        self.lattice, self.n = lattice, 64 * n_bundles
+        self.n_bundles = n_bundles
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.scamper_multispin_free(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_betas(self, betas):
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        assert b.size == self.n
+        check(lib().scamper_multispin_set_betas(self.h, _p(b)))
+
+    def init(self, random: bool):
+        check(lib().scamper_multispin_init(self.h, int(random)))
+
+    def upload(self, words):
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        check(lib().scamper_multispin_upload(self.h, _p(w)))
+
+    def download(self) -> np.ndarray:
+        w = np.zeros(self.n_bundles * self.lattice.n_sites, np.uint64)
+        check(lib().scamper_multispin_download(self.h, _p(w)))
+        return w
+
+    def step(self, n_steps: int = 1, exchange: bool = False, measure: bool = False):
+        check(lib().scamper_multispin_step(self.h, n_steps, int(exchange), int(measure)))
+
+    def step_timed(self, n_steps: int = 1, exchange: bool = False, measure: bool = False) -> float:
+        ms = C.c_float()
+        check(lib().scamper_multispin_step_timed(self.h, n_steps, int(exchange), int(measure), C.byref(ms)))
+        return ms.value
+
+    def labels(self) -> np.ndarray:
+        a = np.zeros(self.n, np.int32)
+        check(lib().scamper_multispin_labels(self.h, _p(a)))
+        return a
+
+    def stats(self):
+        a, b, c = u64(), u64(), u64()
+        check(lib().scamper_multispin_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def series(self):
+        n = u64()
+        check(lib().scamper_multispin_series(self.h, None, None, 0, C.byref(n)))
+        e = np.zeros((n.value, self.n))
+        m = np.zeros((n.value, self.n))
+        check(lib().scamper_multispin_series(self.h, _p(e), _p(m), n.value, None))
+        return e, m

@@ -33,12 +33,6 @@ int scamper::fail(int code, const char *fmt, ...) {
 
 namespace {
 
-#define SC_HIP(call)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess) return fail(SCAMPER_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
-
 // RngStreamKey::seed (Carlo.rs/src/rng_stream.rs:126-154): SplitMix64
 // rounds fold every logical coordinate into the 64-bit stream seed.
 uint64_t mix64(uint64_t v) {
@@ -50,7 +44,9 @@ uint64_t mix64(uint64_t v) {
 uint64_t fold(uint64_t s, uint64_t domain, uint64_t value) {
   return mix64(s ^ domain ^ mix64(value + ((domain << 17) | (domain >> 47))));
 }
-uint64_t stream_seed(uint64_t base, uint64_t replica, uint64_t phase) {
+}  // namespace
+
+uint64_t scamper::stream_seed(uint64_t base, uint64_t replica, uint64_t phase) {
   uint64_t s = mix64(base ^ 0x53435554544C4552ull);
   s = fold(s, 0x5441534Bull, 0);               // task
   s = fold(s, 0x52554E00ull, 0);               // run
@@ -60,8 +56,6 @@ uint64_t stream_seed(uint64_t base, uint64_t replica, uint64_t phase) {
   s = fold(s, 0x504841534500ull, phase);       // phase
   return fold(s, 0x535542535452ull, 0);        // substream
 }
-
-}  // namespace
 
 struct scamper_state {
   Lattice lat;

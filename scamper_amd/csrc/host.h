@@ -43,6 +43,16 @@ struct Lattice {
 // Record the calling thread's last error (scamper_last_error) and return code.
 int fail(int code, const char *fmt, ...);
 
+#define SC_HIP(call)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess) return ::scamper::fail(SCAMPER_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// RngStreamKey(base).with_replica(replica).with_phase(phase).seed()
+// (Carlo.rs/src/rng_stream.rs:126-154).
+uint64_t stream_seed(uint64_t base, uint64_t replica, uint64_t phase);
+
 // Builders; on failure return false and set `err`.
 bool lattice_from_edges(uint64_t n_sites, std::vector<Bond> edges, Lattice &out, std::string &err);
 bool lattice_hypercubic(const uint64_t *dims, int n_dims, bool pbc, Lattice &out, std::string &err);

@@ -81,6 +81,16 @@ hipError_t launch_csr_measure(const CsrGeom &g, int kind, int q, const int8_t *s
 hipError_t launch_csr_init(uint64_t n_sites, int kind, int q, int8_t *spins, uint64_t key, uint32_t rep,
                            int random, hipStream_t stream);
 
+// Multi-spin coded Ising (u64 words, bit r = replica 64b + r).
+hipError_t launch_ms_init(uint64_t *words, const uint64_t *keys, const StencilGeom &g, uint32_t bundle, int random,
+                          hipStream_t stream);
+hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const int32_t *labels, const uint64_t *keys,
+                           const StencilGeom &g, uint32_t n_bundles, uint32_t colour, uint64_t sweep,
+                           unsigned long long *counts, hipStream_t stream);
+hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const double *beta_by_label, uint32_t n,
+                              double n_bonds, double n_sites, double j, uint64_t ex_key, uint64_t step, int do_exchange,
+                              double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t stream);
+
 // Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
 bool ising_fast_path_ok(const StencilGeom &g);
 hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,
