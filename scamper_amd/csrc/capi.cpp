@@ -78,6 +78,8 @@ struct scamper_state {
   int8_t *d_weights = nullptr;
   hipStream_t stream = nullptr;
   int8_t *d_spins = nullptr;
+  bool continuous = false;  // O(3): FP32 component planes in d_vec
+  float *d_vec = nullptr;
   ReplicaParams *d_params = nullptr;
   double *d_hbexp = nullptr;
   long long *d_em = nullptr;      // scratch for exact recomputation
@@ -89,6 +91,7 @@ struct scamper_state {
   ~scamper_state() {
     if (stream) hipStreamSynchronize(stream);
     hipFree(d_spins);
+    hipFree(d_vec);
     hipFree(d_params);
     hipFree(d_hbexp);
     hipFree(d_em);
@@ -127,6 +130,8 @@ int upload_tables(scamper_state *st, int alg) {
     } else if (st->model.kind == kIsing) {
       if (alg == kMetropolis) ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
       else ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
+    } else if (st->continuous) {
+      // the O(3) kernels read only key and beta
     } else if (st->model.kind == kPotts) {
       if (alg == kMetropolis) potts_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
       else potts_heat_bath_exp(st->beta[r], st->model.j, z, hb.data() + (size_t)r * (kMaxZ + 1) * (kMaxZ + 1));
@@ -211,6 +216,15 @@ int download_i8_impl(scamper_state *st, uint32_t replica, int8_t *spins) {
 // graph both ends count it, and self-loops add the constant -J w.
 void observables_from_em(const scamper_state *st, const long long *em, double *e, double *m) {
   const double n = (double)st->lat.n_sites;
+  if (st->continuous) {
+    // double sums {s.h (J included), Mx, My, Mz}; ONModel::magnetization
+    // is |sum s| / N (models.rs:500-514)
+    double d[4];
+    std::memcpy(d, em, sizeof d);
+    if (e) *e = -d[0];
+    if (m) *m = std::sqrt(d[1] * d[1] + d[2] * d[2] + d[3] * d[3]) / n;
+    return;
+  }
   if (e) {
     if (st->csr)
       *e = -st->model.j * ((double)(em[0] / 2) + st->self_loop_weight);
@@ -334,9 +348,13 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   if (!std::isfinite(model->j)) return fail(SCAMPER_EINVAL, "coupling must be finite");
   if (model->kind == kPotts && (model->q < 2 || model->q > kMaxQ))
     return fail(model->q < 2 ? SCAMPER_EINVAL : SCAMPER_EUNSUPPORTED, "Potts q=%d (GPU path supports 2..%d)", model->q, kMaxQ);
-  if (model->kind != kIsing && model->kind != kPotts)
-    return fail(SCAMPER_EUNSUPPORTED, "model kind %d is not available on this lattice path", model->kind);
+  if (model->kind != kIsing && model->kind != kPotts && model->kind != kON)
+    return fail(SCAMPER_EINVAL, "unknown model kind %d", model->kind);
   const Lattice &l = lh->lat;
+  if (model->kind == kON && (model->spin_dim != 3 || !l.stencil))
+    return fail(SCAMPER_EUNSUPPORTED,
+                "O(N) spins run with N = 3 on periodic hypercubic lattices with even sides (got N = %d)",
+                model->spin_dim);
   auto st = std::unique_ptr<scamper_state>(new (std::nothrow) scamper_state);
   if (!st) return fail(SCAMPER_ENOMEM, "state handle");
   st->model = *model;
@@ -400,13 +418,20 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
     st->cgeom = CsrGeom{l.n_sites, l.n_colours, st->w_max, st->d_offsets, st->d_neighbors, st->d_weights,
                         st->d_class_sites};
   }
-  const size_t bytes = (size_t)n_replicas * l.n_sites;
-  if (hipMalloc(&st->d_spins, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
+  st->continuous = model->kind == kON;
+  if (st->continuous) {
+    const size_t bytes = (size_t)n_replicas * l.n_sites * 3 * sizeof(float);
+    if (hipMalloc(&st->d_vec, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
+  } else {
+    const size_t bytes = (size_t)n_replicas * l.n_sites;
+    if (hipMalloc(&st->d_spins, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
+  }
   SC_HIP(hipMalloc(&st->d_params, sizeof(ReplicaParams) * n_replicas));
   SC_HIP(hipMalloc(&st->d_em, sizeof(long long) * kEmStride * n_replicas));
   for (uint32_t r = 0; r < n_replicas; ++r) {
-    hipError_t e = st->csr ? launch_csr_init(l.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream)
-                           : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream);
+    hipError_t e = st->continuous ? launch_on3_init(st->geom, st->d_vec, st->keys[r], r, 0, st->stream)
+                   : st->csr ? launch_csr_init(l.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream)
+                             : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream);
     if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
   }
   SC_HIP(hipStreamSynchronize(st->stream));
@@ -433,21 +458,61 @@ int scamper_state_get_beta(const scamper_state *st, uint32_t replica, double *be
 
 int scamper_state_init(scamper_state *st, uint32_t replica, int random) {
   if (int rc = check_state(st, replica)) return rc;
-  hipError_t e = st->csr ? launch_csr_init(st->lat.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream)
-                         : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream);
+  hipError_t e = st->continuous ? launch_on3_init(st->geom, st->d_vec, st->keys[replica], replica, random != 0, st->stream)
+                 : st->csr ? launch_csr_init(st->lat.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream)
+                           : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
+  return SCAMPER_OK;
+}
+
+static int vec_upload(scamper_state *st, uint32_t replica, const float *xyz) {
+  const uint64_t n = st->lat.n_sites, nh = st->geom.nh;
+  std::vector<float> planes(6 * nh);
+  for (uint64_t s = 0; s < n; ++s) {
+    uint32_t c;
+    const uint64_t i = split_index(st->geom, s, &c);
+    for (int k = 0; k < 3; ++k) planes[(3ull * c + k) * nh + i] = xyz[3 * s + k];
+  }
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(st->d_vec + 6ull * replica * nh, planes.data(), planes.size() * sizeof(float), hipMemcpyHostToDevice));
+  return SCAMPER_OK;
+}
+
+static int vec_download(scamper_state *st, uint32_t replica, float *xyz) {
+  const uint64_t n = st->lat.n_sites, nh = st->geom.nh;
+  std::vector<float> planes(6 * nh);
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(planes.data(), st->d_vec + 6ull * replica * nh, planes.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (uint64_t s = 0; s < n; ++s) {
+    uint32_t c;
+    const uint64_t i = split_index(st->geom, s, &c);
+    for (int k = 0; k < 3; ++k) xyz[3 * s + k] = planes[(3ull * c + k) * nh + i];
+  }
+  return SCAMPER_OK;
+}
+
+// ONModel::validate_spin (models.rs:454-474): finite, |s|^2 within 1e-10 D
+// of one; f32 input is checked to single-precision accuracy.
+static int vec_validate(const double *xyz, uint64_t n, double tol) {
+  for (uint64_t s = 0; s < n; ++s) {
+    const double a = xyz[3 * s], b = xyz[3 * s + 1], c = xyz[3 * s + 2];
+    if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c) || std::fabs(a * a + b * b + c * c - 1.0) > tol)
+      return fail(SCAMPER_EINVAL, "O(3) spin at site %llu is not a finite unit vector", (unsigned long long)s);
+  }
   return SCAMPER_OK;
 }
 
 int scamper_state_upload_i8(scamper_state *st, uint32_t replica, const int8_t *spins) {
   if (int rc = check_state(st, replica)) return rc;
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  if (st->continuous) return fail(SCAMPER_EUNSUPPORTED, "int8 spins belong to Ising and Potts states");
   return upload_i8_impl(st, replica, spins);
 }
 
 int scamper_state_download_i8(scamper_state *st, uint32_t replica, int8_t *spins) {
   if (int rc = check_state(st, replica)) return rc;
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  if (st->continuous) return fail(SCAMPER_EUNSUPPORTED, "int8 spins belong to Ising and Potts states");
   return download_i8_impl(st, replica, spins);
 }
 
@@ -455,6 +520,12 @@ int scamper_state_upload_f64(scamper_state *st, uint32_t replica, const double *
   if (int rc = check_state(st, replica)) return rc;
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
   const uint64_t n = st->lat.n_sites;
+  if (st->continuous) {
+    if (int rc = vec_validate(spins, n, 3e-10)) return rc;
+    std::vector<float> f(3 * n);
+    for (uint64_t k = 0; k < 3 * n; ++k) f[k] = (float)spins[k];
+    return vec_upload(st, replica, f.data());
+  }
   std::vector<int8_t> v(n);
   for (uint64_t s = 0; s < n; ++s) {
     const double x = spins[s];
@@ -470,20 +541,33 @@ int scamper_state_upload_f64(scamper_state *st, uint32_t replica, const double *
 int scamper_state_download_f64(scamper_state *st, uint32_t replica, double *spins) {
   if (int rc = check_state(st, replica)) return rc;
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  if (st->continuous) {
+    std::vector<float> f(3 * st->lat.n_sites);
+    if (int rc = vec_download(st, replica, f.data())) return rc;
+    for (uint64_t k = 0; k < f.size(); ++k) spins[k] = (double)f[k];
+    return SCAMPER_OK;
+  }
   std::vector<int8_t> v(st->lat.n_sites);
   if (int rc = download_i8_impl(st, replica, v.data())) return rc;
   for (uint64_t s = 0; s < st->lat.n_sites; ++s) spins[s] = (double)v[s];
   return SCAMPER_OK;
 }
 
-int scamper_state_upload_f32(scamper_state *st, uint32_t, const float *) {
-  if (!st) return fail(SCAMPER_EINVAL, "null state");
-  return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
+int scamper_state_upload_f32(scamper_state *st, uint32_t replica, const float *spins) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  if (!st->continuous) return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
+  std::vector<double> d(3 * st->lat.n_sites);
+  for (uint64_t k = 0; k < d.size(); ++k) d[k] = spins[k];
+  if (int rc = vec_validate(d.data(), st->lat.n_sites, 1e-5)) return rc;
+  return vec_upload(st, replica, spins);
 }
 
-int scamper_state_download_f32(scamper_state *st, uint32_t, float *) {
-  if (!st) return fail(SCAMPER_EINVAL, "null state");
-  return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
+int scamper_state_download_f32(scamper_state *st, uint32_t replica, float *spins) {
+  if (int rc = check_state(st, replica)) return rc;
+  if (!spins) return fail(SCAMPER_EINVAL, "null spins");
+  if (!st->continuous) return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
+  return vec_download(st, replica, spins);
 }
 
 int scamper_state_sweeps(const scamper_state *st, uint64_t *sweeps) {
@@ -494,10 +578,15 @@ int scamper_state_sweeps(const scamper_state *st, uint64_t *sweeps) {
 
 int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
   if (!st) return fail(SCAMPER_EINVAL, "null state");
-  if (algorithm != kMetropolis && algorithm != kHeatBath)
+  if (st->continuous) {
+    if (algorithm != kContinuousHeatBath && algorithm != kOverRelaxation && algorithm != kHybridHeatBathOverRelaxation)
+      return fail(SCAMPER_EUNSUPPORTED, "O(3) spins take the continuous heat bath, over-relaxation or their hybrid");
+  } else if (algorithm != kMetropolis && algorithm != kHeatBath) {
     return fail(SCAMPER_EUNSUPPORTED, "algorithm %d needs continuous spins", algorithm);
-  if (st->tables_for != algorithm)
-    if (int rc = upload_tables(st, algorithm)) return rc;
+  }
+  const int table_alg = st->continuous ? kContinuousHeatBath : algorithm;
+  if (st->tables_for != table_alg)
+    if (int rc = upload_tables(st, table_alg)) return rc;
   if (measure) {
     if (int rc = ensure_series(st, n_sweeps)) return rc;
     const size_t slot = (size_t)st->n_rep * kEmStride;
@@ -505,6 +594,24 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
   }
   for (uint64_t t = 0; t < n_sweeps; ++t) {
     const uint64_t sweep = st->sweeps + t;
+    if (st->continuous) {
+      // HybridCore: heat bath then over-relaxation each sweep (hybrid.rs:46-59)
+      const int parts[2] = {algorithm == kOverRelaxation ? kOverRelaxation : kContinuousHeatBath,
+                            algorithm == kHybridHeatBathOverRelaxation ? kOverRelaxation : -1};
+      for (int pi = 0; pi < 2; ++pi) {
+        if (parts[pi] < 0) continue;
+        const bool last = pi == 1 || parts[1] < 0;
+        for (uint32_t c = 0; c < 2; ++c) {
+          double *em = (measure && last && c == 1)
+                           ? reinterpret_cast<double *>(st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride)
+                           : nullptr;
+          hipError_t e = launch_on3_sweep(st->geom, parts[pi], st->d_vec, st->d_params, st->n_rep, c, sweep,
+                                          (float)st->model.j, em, st->stream);
+          if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+        }
+      }
+      continue;
+    }
     if (st->csr) {
       hipError_t e = launch_csr_discrete(st->cgeom, st->class_start.data(), st->model.kind, st->model.q, algorithm,
                                          st->d_spins, st->d_params, st->n_rep, sweep, st->d_hbexp, st->stream);
@@ -528,8 +635,12 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
 
 int scamper_sweep_timed(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure, float *ms) {
   if (!st || !ms) return fail(SCAMPER_EINVAL, "null argument");
-  if (st->tables_for != algorithm && (algorithm == kMetropolis || algorithm == kHeatBath))
-    if (int rc = upload_tables(st, algorithm)) return rc;
+  // table uploads and series growth synchronise; do them before the events
+  const int table_alg = st->continuous ? kContinuousHeatBath : algorithm;
+  if (st->tables_for != table_alg && (st->continuous || algorithm == kMetropolis || algorithm == kHeatBath))
+    if (int rc = upload_tables(st, table_alg)) return rc;
+  if (measure)
+    if (int rc = ensure_series(st, n_sweeps)) return rc;
   hipEvent_t a, b;
   SC_HIP(hipEventCreate(&a));
   SC_HIP(hipEventCreate(&b));
@@ -554,8 +665,11 @@ int scamper_synchronize(scamper_state *st) {
 int scamper_state_observables(scamper_state *st, uint32_t replica, double *energy, double *magnetization) {
   if (int rc = check_state(st, replica)) return rc;
   SC_HIP(hipMemsetAsync(st->d_em, 0, sizeof(long long) * kEmStride * st->n_rep, st->stream));
-  hipError_t e = st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream)
-                         : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);
+  hipError_t e = st->continuous
+                     ? launch_on3_measure(st->geom, st->d_vec, st->n_rep, (float)st->model.j,
+                                          reinterpret_cast<double *>(st->d_em), st->stream)
+                 : st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream)
+                           : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure: %s", hipGetErrorString(e));
   long long em[kEmStride];
   SC_HIP(hipMemcpyAsync(em, st->d_em + (size_t)replica * kEmStride, sizeof em, hipMemcpyDeviceToHost, st->stream));
