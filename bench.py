@@ -126,11 +126,13 @@ def main() -> int:
                                                                  "ising2d_L16384_pmc.txt"))},
     }
     if args.calibrate_copy:
+        # an elementwise kernel (not a DMA copy) that reads 1 GiB and writes
+        # 1 GiB once each, so that rocprofv3 counters can be scaled per byte
         import torch
-        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        src = torch.zeros(1 << 30, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.bitwise_not(src, out=dst)
         torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
