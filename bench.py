@@ -72,9 +72,6 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--L", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a 1 GiB device-to-device copy (a copy kernel of known "
-                         "size, used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU "
@@ -125,15 +122,6 @@ def main() -> int:
                      "traffic": traffic_from_profile(os.path.join(ROOT, "profiles",
                                                                  "ising2d_L16384_pmc.txt"))},
     }
-    if args.calibrate_copy:
-        # an elementwise kernel (not a DMA copy) that reads 1 GiB and writes
-        # 1 GiB once each, so that rocprofv3 counters can be scaled per byte
-        import torch
-        src = torch.zeros(1 << 30, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.bitwise_not(src, out=dst)
-        torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -222,6 +222,79 @@ int scamper_run_one(const scamper_lattice *lat, const scamper_model *model, int 
                     double beta, int random_start, const scamper_run_config *cfg,
                     scamper_estimate out[5], double *energy, double *magnetization);
 
+/* ----------------------------------------------------------- job pool
+ * Many independent run_one jobs on one GPU at once: the device counterpart
+ * of Scheduler::run_parallel -> Backend::spawn_tasks, where every task owns
+ * its own MonteCarlo and Context (Carlo.rs/src/scheduler.rs:345-405,
+ * Carlo.rs/src/backend/rayon.rs:36-49).  Jobs are Ising or Potts Metropolis
+ * runs on periodic hypercubic lattices with even sides (1-3 dimensions).
+ *
+ * Jobs are admitted in submission order as device memory (the pool's spin
+ * budget) and job rows free up; admitted jobs advance together, one sweep
+ * of every running job per step, in graph-launched chunks of
+ * `chunk_sweeps` steps.  A job's result depends only on its own spec: its
+ * draws are keyed by (seed, sweep, colour, site), so it is bit-identical to
+ * scamper_run_one on the same spec and to itself in any mix. */
+typedef struct scamper_pool scamper_pool;
+
+typedef struct {
+  uint64_t spin_budget_bytes; /* device bytes for spins (one byte per site) */
+  uint32_t max_jobs;          /* running jobs at once (1..4096) */
+  uint32_t chunk_sweeps;      /* steps per graph launch (1..1024); 0 = 32 */
+  uint32_t grid;              /* blocks per launch, multiple of 8; 0 = autotune */
+  uint32_t use_graphs;        /* 1 = hipGraph per chunk, 0 = plain launches */
+} scamper_pool_config;
+
+typedef struct {
+  uint32_t n_dims;          /* 1, 2 or 3 */
+  uint32_t reserved;
+  uint64_t dims[3];         /* even sides, periodic */
+  scamper_model model;      /* SCAMPER_ISING or SCAMPER_POTTS (q <= 6) */
+  double beta;
+  int32_t algorithm;        /* SCAMPER_METROPOLIS */
+  int32_t random_start;
+  uint64_t thermalization_sweeps;
+  uint64_t measurement_sweeps;
+  uint64_t binsize;
+  uint64_t seed;            /* base seed, as scamper_run_config.base_seed */
+  uint32_t keep_series;     /* keep the per-sweep E and M for scamper_pool_result */
+  uint32_t keep_spins;      /* keep the final spins (site-major int8) */
+} scamper_job_spec;
+
+enum { SCAMPER_JOB_QUEUED = 0, SCAMPER_JOB_RUNNING = 1, SCAMPER_JOB_DONE = 2 };
+
+typedef struct {
+  uint64_t submitted, queued, running, done; /* running = jobs holding a row */
+  uint64_t spin_bytes_in_use;
+  uint64_t site_updates;  /* attempted single-site updates enqueued so far */
+  uint64_t chunks;
+  uint32_t grid;          /* blocks per launch in use (after autotuning) */
+  uint32_t tuned;         /* 1 once the autotuner has settled */
+} scamper_pool_stats;
+
+int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out);
+void scamper_pool_free(scamper_pool *pool);
+/* Queue a job.  SCAMPER_EBUDGET when its spins alone exceed the budget;
+ * otherwise it waits for memory and a row. */
+int scamper_pool_submit(scamper_pool *pool, const scamper_job_spec *spec, uint64_t *job_id);
+/* Enqueue one chunk for the running jobs (after admitting queued ones) and
+ * collect the chunk before the previous one; *n_completed (may be NULL)
+ * gets the number of jobs finished by this call. */
+int scamper_pool_step(scamper_pool *pool, uint32_t *n_completed);
+/* Step until every submitted job is done. */
+int scamper_pool_run(scamper_pool *pool);
+/* Ids of jobs finished since the last call (up to cap; *n = count written). */
+int scamper_pool_completed(scamper_pool *pool, uint64_t *ids, uint64_t cap, uint64_t *n);
+int scamper_pool_job_state(const scamper_pool *pool, uint64_t job_id, int32_t *state, uint64_t *sweeps_done);
+/* Estimates (Energy, E2, Magnetization, M2, M4) of a finished job, its
+ * measured series if kept (each measurement_sweeps long) and its final
+ * spins if kept (n_sites bytes).  NULL outputs are skipped. */
+int scamper_pool_result(const scamper_pool *pool, uint64_t job_id, scamper_estimate out[5], double *energy,
+                        double *magnetization, int8_t *spins);
+/* Forget a finished job's results. */
+int scamper_pool_release(scamper_pool *pool, uint64_t job_id);
+int scamper_pool_stats_get(const scamper_pool *pool, scamper_pool_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,25 @@ class Model(C.Structure):
                 ("reserved", C.c_int32), ("j", C.c_double)]
 
 
+class PoolConfig(C.Structure):
+    _fields_ = [("spin_budget_bytes", C.c_uint64), ("max_jobs", C.c_uint32),
+                ("chunk_sweeps", C.c_uint32), ("grid", C.c_uint32), ("use_graphs", C.c_uint32)]
+
+
+class JobSpec(C.Structure):
+    _fields_ = [("n_dims", C.c_uint32), ("reserved", C.c_uint32), ("dims", C.c_uint64 * 3),
+                ("model", Model), ("beta", C.c_double), ("algorithm", C.c_int32),
+                ("random_start", C.c_int32), ("thermalization_sweeps", C.c_uint64),
+                ("measurement_sweeps", C.c_uint64), ("binsize", C.c_uint64), ("seed", C.c_uint64),
+                ("keep_series", C.c_uint32), ("keep_spins", C.c_uint32)]
+
+
+class PoolStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("queued", C.c_uint64), ("running", C.c_uint64),
+                ("done", C.c_uint64), ("spin_bytes_in_use", C.c_uint64), ("site_updates", C.c_uint64),
+                ("chunks", C.c_uint64), ("grid", C.c_uint32), ("tuned", C.c_uint32)]
+
+
 _lib = None
 u64, u32, i32, dbl, vp, sz = C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_void_p, C.c_size_t
 PP = C.POINTER(C.c_void_p)
@@ -94,6 +113,16 @@ SIGNATURES = {
     "scamper_binned_estimate": (i32, [vp, u64, u64, C.POINTER(Estimate)]),
     "scamper_run_one": (i32, [vp, C.POINTER(Model), i32, dbl, i32, C.POINTER(RunConfig),
                               vp, vp, vp]),
+    "scamper_pool_create": (i32, [C.POINTER(PoolConfig), PP]),
+    "scamper_pool_free": (None, [vp]),
+    "scamper_pool_submit": (i32, [vp, C.POINTER(JobSpec), vp]),
+    "scamper_pool_step": (i32, [vp, vp]),
+    "scamper_pool_run": (i32, [vp]),
+    "scamper_pool_completed": (i32, [vp, vp, u64, vp]),
+    "scamper_pool_job_state": (i32, [vp, u64, vp, vp]),
+    "scamper_pool_result": (i32, [vp, u64, vp, vp, vp, vp]),
+    "scamper_pool_release": (i32, [vp, u64]),
+    "scamper_pool_stats_get": (i32, [vp, C.POINTER(PoolStats)]),
 }
 
 
@@ -365,3 +394,96 @@ class MultiSpin:
         m = np.zeros((n.value, self.n))
         check(lib().scamper_multispin_series(self.h, _p(e), _p(m), n.value, None))
         return e, m
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
+
+
+def job_spec(dims, m: Model, beta: float, thermalization_sweeps: int, measurement_sweeps: int,
+             binsize: int = 100, seed: int = 0, random_start: bool = True, keep_series: bool = False,
+             keep_spins: bool = False, algorithm: int = METROPOLIS) -> JobSpec:
+    dims = [int(d) for d in dims]
+    s = JobSpec()
+    s.n_dims = len(dims)
+    for i, d in enumerate(dims):
+        s.dims[i] = d
+    s.model = m
+    s.beta = beta
+    s.algorithm = algorithm
+    s.random_start = int(random_start)
+    s.thermalization_sweeps = thermalization_sweeps
+    s.measurement_sweeps = measurement_sweeps
+    s.binsize = binsize
+    s.seed = seed
+    s.keep_series = int(keep_series)
+    s.keep_spins = int(keep_spins)
+    return s
+
+
+class Pool:
+    """Continuous-batching job pool: independent run_one jobs (Ising or Potts
+    Metropolis on periodic hypercubic lattices) sharing the GPU
+    (Scheduler::run_parallel, Carlo.rs/src/scheduler.rs:345-405)."""
+
+    def __init__(self, spin_budget_bytes: int = 1 << 30, max_jobs: int = 256, chunk_sweeps: int = 32,
+                 grid: int = 0, use_graphs: bool = True):
+        cfg = PoolConfig(spin_budget_bytes, max_jobs, chunk_sweeps, grid, int(use_graphs))
+        h = C.c_void_p()
+        check(lib().scamper_pool_create(C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        self._specs = {}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().scamper_pool_free(self.h)
+            self.h = None
+
+    def submit(self, spec: JobSpec) -> int:
+        jid = u64()
+        check(lib().scamper_pool_submit(self.h, C.byref(spec), C.byref(jid)))
+        self._specs[jid.value] = spec
+        return jid.value
+
+    def step(self) -> int:
+        n = u32()
+        check(lib().scamper_pool_step(self.h, C.byref(n)))
+        return n.value
+
+    def run(self) -> None:
+        check(lib().scamper_pool_run(self.h))
+
+    def completed(self) -> list[int]:
+        ids = np.zeros(4096, dtype=np.uint64)
+        out = []
+        while True:
+            n = u64()
+            check(lib().scamper_pool_completed(self.h, _p(ids), ids.size, C.byref(n)))
+            out.extend(int(x) for x in ids[: n.value])
+            if n.value < ids.size:
+                return out
+
+    def state(self, job_id: int):
+        st, done = C.c_int32(), u64()
+        check(lib().scamper_pool_job_state(self.h, job_id, C.byref(st), C.byref(done)))
+        return st.value, done.value
+
+    def result(self, job_id: int):
+        """(estimates dict, energy series or None, |M| series or None, spins or None)."""
+        spec = self._specs[job_id]
+        est = (Estimate * 5)()
+        n = spec.measurement_sweeps
+        e = np.zeros(n) if spec.keep_series else None
+        m = np.zeros(n) if spec.keep_series else None
+        n_sites = int(np.prod([spec.dims[i] for i in range(spec.n_dims)]))
+        sp = np.zeros(n_sites, dtype=np.int8) if spec.keep_spins else None
+        check(lib().scamper_pool_result(self.h, job_id, est, _p(e), _p(m), _p(sp)))
+        return {k: est[i] for i, k in enumerate(OBSERVABLES)}, e, m, sp
+
+    def release(self, job_id: int) -> None:
+        check(lib().scamper_pool_release(self.h, job_id))
+        self._specs.pop(job_id, None)
+
+    def stats(self) -> PoolStats:
+        out = PoolStats()
+        check(lib().scamper_pool_stats_get(self.h, C.byref(out)))
+        return out

@@ -28,6 +28,8 @@ COMMON = [
 PER_FILE = {
     "tables.cpp": ["-ffp-contract=off"],
     "capi.cpp": ["-ffp-contract=off"],
+    "carlo.cpp": ["-ffp-contract=off"],
+    "pool.cpp": ["-ffp-contract=off"],
 }
 
 

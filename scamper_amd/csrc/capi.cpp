@@ -84,6 +84,10 @@ struct scamper_state {
   double *d_hbexp = nullptr;
   long long *d_em = nullptr;      // scratch for exact recomputation
   long long *d_series = nullptr;  // [slot][replica][kEmStride]
+  // stencil Metropolis outside the fast Ising path: one tile-table row per
+  // replica (header, then the rows)
+  void *d_tiles = nullptr;
+  uint32_t tile_grid = 0;
   uint64_t series_cap = 0, series_len = 0;
   int tables_for = -1;            // algorithm the uploaded tables serve
   int device = 0;
@@ -96,6 +100,7 @@ struct scamper_state {
     hipFree(d_hbexp);
     hipFree(d_em);
     hipFree(d_series);
+    hipFree(d_tiles);
     hipFree(d_offsets);
     hipFree(d_neighbors);
     hipFree(d_class_sites);
@@ -110,6 +115,14 @@ int check_state(const scamper_state *st, uint32_t replica) {
   if (!st) return fail(SCAMPER_EINVAL, "null state");
   if (replica >= st->n_rep) return fail(SCAMPER_EINVAL, "replica %u out of range (%u replicas)", replica, st->n_rep);
   return SCAMPER_OK;
+}
+
+// Stencil Metropolis runs on the tiled kernel unless the register-blocked
+// Ising kernel takes it.
+bool uses_tiles(const scamper_state *st, int alg) {
+  if (st->csr || st->continuous || alg != kMetropolis) return false;
+  if (st->model.kind == kIsing && ising_fast_path_ok(st->geom)) return false;
+  return st->geom.nh < (1ull << 31);
 }
 
 // Build and upload the per-replica decision tables for `alg`.
@@ -139,6 +152,39 @@ int upload_tables(scamper_state *st, int alg) {
   }
   SC_HIP(hipStreamSynchronize(st->stream));
   SC_HIP(hipMemcpy(st->d_params, p.data(), p.size() * sizeof(ReplicaParams), hipMemcpyHostToDevice));
+  if (uses_tiles(st, alg)) {
+    // one row per replica: every sweep of every replica, measured from 0
+    const uint32_t per = (uint32_t)((st->geom.nh + kTileSites - 1) / kTileSites);
+    std::vector<TileJob> rows(st->n_rep);
+    for (uint32_t r = 0; r < st->n_rep; ++r) {
+      TileJob &t = rows[r];
+      memset(&t, 0, sizeof t);
+      t.key = st->keys[r];
+      t.spins = st->d_spins + 2ull * r * st->geom.nh;
+      t.sweep_base = 0;
+      t.sweep_end = ~0ull;
+      t.therm = 0;
+      t.div_lx2 = div_magic(st->geom.lx2);
+      t.div_ly = div_magic(st->geom.ly);
+      t.nd = (uint32_t)st->geom.nd;
+      t.lx2 = st->geom.lx2;
+      t.ly = st->geom.ly;
+      t.lz = st->geom.lz;
+      t.nh = (uint32_t)st->geom.nh;
+      t.kind = (uint32_t)st->model.kind;
+      t.q = (uint32_t)st->model.q;
+      t.tile_start = r * per;
+      t.n_tiles = per;
+      t.em_row = r;
+      std::memcpy(t.thr, p[r].thr, sizeof t.thr);
+    }
+    TileHeader h{st->n_rep, st->n_rep * per, {0, 0}};
+    if (!st->d_tiles) SC_HIP(hipMalloc(&st->d_tiles, sizeof(TileHeader) + rows.size() * sizeof(TileJob)));
+    SC_HIP(hipMemcpy(st->d_tiles, &h, sizeof h, hipMemcpyHostToDevice));
+    SC_HIP(hipMemcpy(static_cast<char *>(st->d_tiles) + sizeof(TileHeader), rows.data(),
+                     rows.size() * sizeof(TileJob), hipMemcpyHostToDevice));
+    st->tile_grid = std::min<uint32_t>((h.n_tiles + 7u) & ~7u, 4096u);
+  }
   if (!hb.empty()) {
     if (!st->d_hbexp) SC_HIP(hipMalloc(&st->d_hbexp, hb.size() * sizeof(double)));
     SC_HIP(hipMemcpy(st->d_hbexp, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -619,6 +665,16 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
         e = launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep,
                                st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride, st->stream);
       if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+      continue;
+    }
+    if (uses_tiles(st, algorithm)) {
+      const TileHeader *hdr = static_cast<const TileHeader *>(st->d_tiles);
+      const TileJob *rows = reinterpret_cast<const TileJob *>(hdr + 1);
+      const EmLayout em{st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride, kEmStride, 0, 0, 0};
+      for (uint32_t c = 0; c < 2; ++c) {
+        hipError_t e = launch_tile_sweep(hdr, rows, st->tile_grid, sweep, c, measure && c == 1, em, st->stream);
+        if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+      }
       continue;
     }
     for (uint32_t c = 0; c < 2; ++c) {

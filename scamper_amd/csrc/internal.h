@@ -100,6 +100,54 @@ hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const
                               double n_bonds, double n_sites, double j, uint64_t ex_key, uint64_t step, int do_exchange,
                               double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t stream);
 
+// ------------------------------------------------------------ tiled jobs
+// Checkerboard Metropolis of Ising and Potts spins for any number of
+// independent periodic lattices ("jobs") in one launch.  A tile is 4096
+// consecutive sites of one colour class of one job; block b of a launch
+// walks the tiles of its XCD's share of the flattened tile list.
+constexpr uint32_t kTileSites = 4096;
+constexpr int kStencilTable = 13;  // 2z+1 for z = 6
+
+struct alignas(16) TileJob {
+  uint64_t key;          // Philox key of the job's replica 0
+  int8_t *spins;         // colour 0 at spins, colour 1 at spins + nh
+  uint64_t sweep_base;   // sweep number of step 0
+  uint64_t sweep_end;    // steps whose sweep is >= sweep_end do nothing
+  uint64_t therm;        // sweeps >= therm are measured (when asked)
+  uint64_t div_lx2;      // ceil(2^64 / lx2), 0 when lx2 == 1
+  uint64_t div_ly;       // ceil(2^64 / ly), 0 when ly == 1
+  uint32_t nd, lx2, ly, lz;
+  uint32_t nh, kind, q, tile_start;
+  uint32_t n_tiles, em_row, pad0, pad1;
+  Threshold thr[kStencilTable];  // by anti-aligned count (Ising) or d + z (Potts)
+};
+
+struct alignas(16) TileHeader {
+  uint32_t n_jobs;
+  uint32_t n_tiles;
+  uint32_t pad[2];
+};
+
+// Where a launch adds its measured counters: for job row r, step k and
+// block b the record (em[0], em[1], em[2 .. 2+q) as for the stencil) at
+//   base + r * row_stride + k * step_stride + (b & sub_mask) * sub_stride.
+struct EmLayout {
+  long long *base;
+  uint64_t row_stride;
+  uint64_t step_stride;
+  uint32_t sub_mask;
+  uint32_t sub_stride;
+};
+
+uint64_t div_magic(uint32_t d);
+hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint64_t step,
+                             uint32_t colour, int measure, EmLayout em, hipStream_t stream);
+// Zero hdr->n_jobs * per_row counters at p (a kernel: safe inside graphs).
+hipError_t launch_tile_zero(const TileHeader *hdr, long long *p, uint64_t per_row, hipStream_t stream);
+// compact[(r * steps + k) * 8 + c] = sum over s < 8 of ring[((r * steps + k) * 8 + s) * 8 + c].
+hipError_t launch_tile_fold(const TileHeader *hdr, const long long *ring, long long *compact, uint32_t steps,
+                            hipStream_t stream);
+
 // Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
 bool ising_fast_path_ok(const StencilGeom &g);
 hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,

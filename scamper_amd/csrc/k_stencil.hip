@@ -15,6 +15,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Blocks per replica of a grid-stride launch over nh sites.
+inline unsigned capped_blocks(uint64_t nh) {
+  const uint64_t b = (nh + kBlock - 1) / kBlock;
+  return (unsigned)(b < 2048 ? b : 2048);
+}
+
 __device__ __forceinline__ uint32_t wrap_dec(uint32_t v, uint32_t n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ uint32_t wrap_inc(uint32_t v, uint32_t n) { return v + 1 == n ? 0 : v + 1; }
 
@@ -98,7 +104,6 @@ __global__ __launch_bounds__(kBlock) void k_site(int8_t *__restrict__ spins,
                                                  long long *__restrict__ em) {
   constexpr int z = 2 * ND;
   const uint32_t rep = blockIdx.y;
-  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const ReplicaParams &P = reps[rep];
   __shared__ Threshold sthr[kMaxTable];
   if (threadIdx.x < kMaxTable) sthr[threadIdx.x] = P.thr[threadIdx.x];
@@ -106,7 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_site(int8_t *__restrict__ spins,
   long long acc[2 + kMaxQ];
 #pragma unroll
   for (int k = 0; k < 2 + kMaxQ; ++k) acc[k] = 0;
-  if (j < g.nh) {
+  // grid-stride over sites: the grid is capped so that a measured launch
+  // issues a bounded number of same-address atomics
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < g.nh; j += (uint64_t)gridDim.x * kBlock) {
     int8_t *__restrict__ own = spins + (2ull * rep + colour) * g.nh;
     const int8_t *__restrict__ oth = spins + (2ull * rep + 1 - colour) * g.nh;
     const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
@@ -185,14 +192,14 @@ __global__ __launch_bounds__(kBlock) void k_site(int8_t *__restrict__ spins,
         int sum = 0;
 #pragma unroll
         for (int k = 0; k < z; ++k) sum += n.v[k];
-        acc[0] = (long long)ns * sum;
-        acc[1] = (long long)ns + other_site;
+        acc[0] += (long long)ns * sum;
+        acc[1] += (long long)ns + other_site;
       } else {
         int same = 0;
 #pragma unroll
         for (int k = 0; k < z; ++k) same += (n.v[k] == ns);
-        acc[0] = same;
-        for (int v = 0; v < q; ++v) acc[2 + v] = (ns == v) + (other_site == v);
+        acc[0] += same;
+        for (int v = 0; v < q; ++v) acc[2 + v] += (ns == v) + (other_site == v);
       }
     }
   }
@@ -208,11 +215,10 @@ __global__ __launch_bounds__(kBlock) void k_measure(const int8_t *__restrict__ s
                                                     int q, long long *__restrict__ em) {
   constexpr int z = 2 * ND;
   const uint32_t rep = blockIdx.y;
-  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   long long acc[2 + kMaxQ];
 #pragma unroll
   for (int k = 0; k < 2 + kMaxQ; ++k) acc[k] = 0;
-  if (j < g.nh) {
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < g.nh; j += (uint64_t)gridDim.x * kBlock) {
     const int8_t *own = spins + (2ull * rep + 1) * g.nh;
     const int8_t *oth = spins + (2ull * rep) * g.nh;
     const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
@@ -224,14 +230,14 @@ __global__ __launch_bounds__(kBlock) void k_measure(const int8_t *__restrict__ s
       int sum = 0;
 #pragma unroll
       for (int k = 0; k < z; ++k) sum += n.v[k];
-      acc[0] = (long long)s * sum;
-      acc[1] = (long long)s + o;
+      acc[0] += (long long)s * sum;
+      acc[1] += (long long)s + o;
     } else {
       int same = 0;
 #pragma unroll
       for (int k = 0; k < z; ++k) same += (n.v[k] == s);
-      acc[0] = same;
-      for (int v = 0; v < q; ++v) acc[2 + v] = (s == v) + (o == v);
+      acc[0] += same;
+      for (int v = 0; v < q; ++v) acc[2 + v] += (s == v) + (o == v);
     }
   }
   block_accumulate<2 + kMaxQ>(acc, KIND == kIsing ? 2 : 2 + q, em + (uint64_t)rep * kEmStride);
@@ -260,7 +266,7 @@ template <int ND>
 hipError_t dispatch_site(const StencilGeom &g, int kind, int q, int alg, int8_t *spins,
                          const ReplicaParams *reps, uint32_t n_rep, uint32_t colour, uint64_t sweep,
                          const double *hb_exp, long long *em, hipStream_t st) {
-  dim3 grid((unsigned)((g.nh + kBlock - 1) / kBlock), n_rep);
+  dim3 grid(capped_blocks(g.nh), n_rep);
   const bool meas = em != nullptr;
 #define SC_LAUNCH(K, A, M) \
   hipLaunchKernelGGL((k_site<ND, K, A, M>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, q, hb_exp, em)
@@ -299,7 +305,7 @@ hipError_t launch_stencil_discrete(const StencilGeom &g, int kind, int q, int al
 
 hipError_t launch_stencil_measure(const StencilGeom &g, int kind, int q, const int8_t *spins,
                                   uint32_t n_rep, long long *em, hipStream_t st) {
-  dim3 grid((unsigned)((g.nh + kBlock - 1) / kBlock), n_rep);
+  dim3 grid(capped_blocks(g.nh), n_rep);
 #define SC_M(ND)                                                                                   \
   if (kind == kIsing)                                                                              \
     hipLaunchKernelGGL((k_measure<ND, kIsing>), grid, dim3(kBlock), 0, st, spins, g, q, em);       \

@@ -91,7 +91,10 @@ SC_HD U4 low_block(uint64_t key, uint64_t h, uint64_t sweep, uint32_t colour, ui
 }
 
 SC_HD uint32_t word_of(const U4 &b, uint32_t k) {
-  return (k & 2) ? ((k & 1) ? b.w : b.z) : ((k & 1) ? b.y : b.x);
+  // two selects on values (an indexed form gets lowered to a stack array)
+  const uint32_t lo = (k & 1) ? b.y : b.x;
+  const uint32_t hi = (k & 1) ? b.w : b.z;
+  return (k & 2) ? hi : lo;
 }
 
 SC_HD uint32_t site_byte(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream,

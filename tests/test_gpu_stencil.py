@@ -50,6 +50,13 @@ CASES = [
     ((16, 16), 1, 16, 0, 1.5, 3),
     ((16, 16), 1, 7, 1, 1.3, 3),
     ((12,), 0, 2, 0, 0.8, 5),
+    # tiled kernel: several tiles, a partial last tile, rows that straddle
+    # lanes, q with and without proposal straddles, enough draws for ties
+    ((256, 256), 1, 4, 0, 1.098612, 10),
+    ((130, 66), 1, 5, 0, 1.2, 6),
+    ((34, 20, 14), 1, 6, 0, 0.9, 6),
+    ((2, 2, 2), 1, 3, 0, 0.5, 4),
+    ((258, 34), 0, 2, 0, 0.4406868, 6),
 ]
 
 
