@@ -1,13 +1,21 @@
 #!/usr/bin/env python3
-"""Throughput of the 2-D Ising checkerboard Metropolis sweep on one MI355X.
+"""Throughput of the classical lattice sweep on one MI355X.
 
-Workload: 16384 x 16384 periodic square lattice, J = 1, h = 0, int8 +-1
-spins from a seeded uniform random start (seed 42), T = 2.269185.  One step
-is one full sweep (colour 0 then colour 1) of the lattice; spins stay in HBM
-for the whole run.  The line printed reports attempted single-site updates
-per nanosecond, the roofline of the dominant kernel (the half-sweep) and the
-oracle's faithful-layout CPU sweep timed on this host as a reported
-baseline.
+Default line (the headline): the 2-D Ising checkerboard Metropolis sweep on a
+16384 x 16384 periodic square lattice, J = 1, h = 0, int8 +-1 spins from a
+seeded uniform random start (seed 42), T = 2.269185.  One step is one full
+sweep (colour 0 then colour 1); spins stay in HBM for the whole run.  The
+line reports attempted single-site updates per nanosecond, the roofline of
+the dominant kernel (the half-sweep), the oracle's faithful-layout CPU sweep
+timed on this host, and under "serving" the batched job queue: Poisson
+streams of Potts requests served by the continuous-batching pool (lattice
+sweeps/s, spin-flips/ns, p50/p99 completion latency) and the same requests
+as plain per-job launches on the same GPU.
+
+Other workloads, one JSON line each (not the headline):
+    --workload pt3d   64-replica multi-spin parallel tempering, 3-D L=128
+    --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
+    --workload serve  the serving block alone
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
 """
@@ -65,21 +73,7 @@ def cpu_baseline(seconds_hint: float = 15.0):
                       f"{dt:.1f} s; host nproc={os.cpu_count()}"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--L", type=int, default=16384)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one MI355X: this build runs on a single GPU "
-              "(multi-GPU sharding is described in DESIGN.md, not launched).", file=sys.stderr)
-        return 2
-
-    import scamper_amd as sc
-
+def ising2d(sc, args) -> dict:
     L = args.L
     T = 2.269185
     lat = sc.Lattice.square(L, L, True)
@@ -92,15 +86,12 @@ def main() -> int:
     ms = st.sweep_timed(sc.METROPOLIS, args.steps)
     wall = time.perf_counter() - t0
     n = lat.n_sites
-    flips = n * args.steps
-    value = flips / (ms * 1e6)
-    launches = 2 * args.steps
-    launch_s = ms * 1e-3 / launches
+    value = n * args.steps / (ms * 1e6)
+    launch_s = ms * 1e-3 / (2 * args.steps)
     bytes_per_launch = 3 * (n // 2)  # read own + read other + write own colour
     achieved = bytes_per_launch / launch_s / 1e9
     e, m = st.observables()
-
-    line = {
+    return {
         "metric": METRIC,
         "value": round(value, 3),
         "unit": "spin-flips/ns",
@@ -122,8 +113,129 @@ def main() -> int:
                      "traffic": traffic_from_profile(os.path.join(ROOT, "profiles",
                                                                  "ising2d_L16384_pmc.txt"))},
     }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline()
+
+
+def serving(sc, args) -> dict:
+    """Potts request streams on the pool, and the ratio against plain
+    per-job launches on a shared subset (all arriving at once)."""
+    from scamper_amd import serve as sv
+
+    out = {"mix": "Potts q in {2,3,4}; 2-D L in [32,2048] or 3-D L in [16,256] log-uniform (even); "
+                  "T in [0.95,1.05] T_c; sweeps log-uniform in [1e3,1e5], 10% thermalisation; "
+                  "Metropolis, random start, binsize 100",
+           "rates": []}
+    for rate in args.serve_rates:
+        reqs = sv.potts_mix(args.serve_requests, rate, seed=11)
+        res = sv.serve(reqs)
+        row = {"lambda_per_s": rate, **res.summary(reqs), "grid": res.grid}
+        out["rates"].append(row)
+        print(f"# serving lambda={rate}: {json.dumps(row)}", file=sys.stderr, flush=True)
+    sub = sv.potts_mix(args.plain_requests, 1e9, seed=13)
+    batched = sv.serve(sub, keep=len(sub))
+    plain = sv.plain(sub, keep=len(sub))
+    same = all(batched.estimates[k][name].mean == plain.estimates[k][name].mean
+               for k in range(len(sub)) for name in ("Energy", "Magnetization", "M2", "M4"))
+    out["plain_vs_batched"] = {
+        "requests": len(sub),
+        "site_updates": sum(r.updates for r in sub),
+        "batched_spin_flips_per_ns": round(batched.flips_per_ns, 3),
+        "plain_spin_flips_per_ns": round(plain.flips_per_ns, 3),
+        "speedup": round(batched.flips_per_ns / plain.flips_per_ns, 2),
+        "batched_wall_s": round(batched.wall_s, 3),
+        "plain_wall_s": round(plain.wall_s, 3),
+        "estimates_bit_identical": bool(same),
+    }
+    return out
+
+
+def pt3d(sc, args) -> dict:
+    """64 temperatures 4.30..4.72 (geometric) of 3-D Ising, L=128, one bit per
+    replica; each step one checkerboard Metropolis sweep then one even/odd
+    exchange round."""
+    L = 128
+    lat = sc.Lattice.hypercubic([L, L, L], True)
+    ms_ = sc.MultiSpin(lat, 1.0, 1, 42)
+    temps = [4.30 * (4.72 / 4.30) ** (k / 63) for k in range(64)]
+    ms_.set_betas([1.0 / t for t in temps])
+    ms_.init(True)
+    ms_.step(args.warmup, exchange=True)
+    ms = ms_.step_timed(args.pt_steps, exchange=True)
+    n = lat.n_sites
+    flips = 64 * n * args.pt_steps
+    acc, ties, ex = ms_.stats()
+    bytes_per_step = 3 * 8 * n  # u64 words: read own + read other + write own colour, both colours
+    return {"metric": "spin-flips/ns (3-D Ising L=128, 64-replica multi-spin PT)",
+            "value": round(flips / (ms * 1e6), 3), "unit": "spin-flips/ns", "n_gpus": 1,
+            "steps": args.pt_steps, "warmup": args.warmup, "ms_per_step": ms / args.pt_steps,
+            "higher_is_better": True, "dtype": "u64 bit-planes", "data": "synthetic",
+            "config": {"workload": "pt3d_multispin", "L": L, "replicas": 64, "T": [4.30, 4.72],
+                       "ladder": "geometric", "seed": 42, "exchanges_accepted": acc,
+                       "near_tie_flags": ties, "exchange_steps": ex},
+            "roofline": {"bound": "valu (cache-resident state)",
+                         "hbm_equivalent_GBps": round(bytes_per_step * args.pt_steps / (ms * 1e-3) / 1e9, 1)}}
+
+
+def on3(sc, args) -> dict:
+    """32 replicas of the 3-D O(3) model, L=96, T = 1.40 + 0.09 r/31; each
+    step one heat-bath sweep then one over-relaxation sweep."""
+    L, R = 96, 32
+    lat = sc.Lattice.hypercubic([L, L, L], True)
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 3), R, 42)
+    for r in range(R):
+        st.init(True, r)
+        st.set_beta(1.0 / (1.40 + 0.09 * r / 31), r)
+    st.sweep(sc.HYBRID_HB_OR, args.warmup)
+    st.synchronize()
+    ms = st.sweep_timed(sc.HYBRID_HB_OR, args.on3_steps)
+    n = lat.n_sites
+    updates = 2 * R * n * args.on3_steps  # heat bath + over-relaxation
+    # per site-update: read own 12 B + read 6 neighbours' planes (12 B each,
+    # once per site with perfect reuse) + write 12 B = 36 B
+    gbs = 36 * updates / (ms * 1e-3) / 1e9
+    return {"metric": "site-updates/ns (3-D O(3) L=96, 32 replicas, HB+OR)",
+            "value": round(updates / (ms * 1e6), 3), "unit": "site-updates/ns", "n_gpus": 1,
+            "steps": args.on3_steps, "warmup": args.warmup, "ms_per_step": ms / args.on3_steps,
+            "higher_is_better": True, "dtype": "f32 spins, f64 observables", "data": "synthetic",
+            "config": {"workload": "on3_hybrid", "L": L, "replicas": R, "T": [1.40, 1.49], "seed": 42},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--L", type=int, default=16384)
+    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "serve"], default="ising2d")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[10.0, 1000.0])
+    ap.add_argument("--plain-requests", type=int, default=12)
+    ap.add_argument("--pt-steps", type=int, default=10_000)
+    ap.add_argument("--on3-steps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build runs on a single GPU "
+              "(multi-GPU sharding is described in DESIGN.md, not launched).", file=sys.stderr)
+        return 2
+
+    import scamper_amd as sc
+
+    if args.workload == "pt3d":
+        line = pt3d(sc, args)
+    elif args.workload == "on3":
+        line = on3(sc, args)
+    elif args.workload == "serve":
+        line = {"metric": "batched-queue sweeps/s + p99 latency (Potts serving mix)", "n_gpus": 1,
+                "serving": serving(sc, args)}
+    else:
+        line = ising2d(sc, args)
+        if not args.no_serving:
+            line["serving"] = serving(sc, args)
+        if not args.no_cpu_baseline:
+            line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
     return 0
 

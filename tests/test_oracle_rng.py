@@ -88,6 +88,15 @@ def _mix64(v):
     return v ^ (v >> 31)
 
 
+def test_mix64_is_splitmix64():
+    # SplitMix64 (Steele, Lea, Flood 2014) seeded with 0 outputs
+    # e220a8397b1dcdaf, 6e789e6aa1b965f4, 06c45d188009454f: mix64(k * gamma)
+    # is its (k+1)-th output.
+    gamma, M = 0x9E3779B97F4A7C15, 2**64 - 1
+    assert [_mix64((k * gamma) & M) for k in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                                          0x06C45D188009454F]
+
+
 def test_stream_seed_python_restatement(orc):
     M = 2**64 - 1
 
