@@ -88,6 +88,10 @@ struct scamper_state {
   // replica (header, then the rows)
   void *d_tiles = nullptr;
   uint32_t tile_grid = 0;
+  // measured tile sweeps add into 8 per-XCD sub-records per replica in this
+  // ring, folded into the series every ring_slots sweeps
+  long long *d_ring = nullptr;
+  uint32_t ring_slots = 0;
   uint64_t series_cap = 0, series_len = 0;
   int tables_for = -1;            // algorithm the uploaded tables serve
   int device = 0;
@@ -101,6 +105,7 @@ struct scamper_state {
     hipFree(d_em);
     hipFree(d_series);
     hipFree(d_tiles);
+    hipFree(d_ring);
     hipFree(d_offsets);
     hipFree(d_neighbors);
     hipFree(d_class_sites);
@@ -183,13 +188,24 @@ int upload_tables(scamper_state *st, int alg) {
     SC_HIP(hipMemcpy(st->d_tiles, &h, sizeof h, hipMemcpyHostToDevice));
     SC_HIP(hipMemcpy(static_cast<char *>(st->d_tiles) + sizeof(TileHeader), rows.data(),
                      rows.size() * sizeof(TileJob), hipMemcpyHostToDevice));
-    st->tile_grid = std::min<uint32_t>((h.n_tiles + 7u) & ~7u, 4096u);
+    st->tile_grid = std::min<uint32_t>((h.n_tiles + 7u) & ~7u, 2048u);
   }
   if (!hb.empty()) {
     if (!st->d_hbexp) SC_HIP(hipMalloc(&st->d_hbexp, hb.size() * sizeof(double)));
     SC_HIP(hipMemcpy(st->d_hbexp, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   st->tables_for = alg;
+  return SCAMPER_OK;
+}
+
+// The measurement ring of the tiled path: up to 64 sweeps of per-XCD
+// sub-records within 64 MiB.
+int ensure_ring(scamper_state *st, int alg) {
+  if (!uses_tiles(st, alg) || st->d_ring) return SCAMPER_OK;
+  const uint64_t per_sweep = (uint64_t)st->n_rep * 8 * kEmStride * sizeof(long long);
+  const uint64_t slots = std::max<uint64_t>(1, std::min<uint64_t>(64, (64ull << 20) / per_sweep));
+  if (hipMalloc(&st->d_ring, slots * per_sweep) != hipSuccess) return fail(SCAMPER_ENOMEM, "measurement ring");
+  st->ring_slots = (uint32_t)slots;
   return SCAMPER_OK;
 }
 
@@ -637,6 +653,7 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
     if (int rc = ensure_series(st, n_sweeps)) return rc;
     const size_t slot = (size_t)st->n_rep * kEmStride;
     SC_HIP(hipMemsetAsync(st->d_series + st->series_len * slot, 0, n_sweeps * slot * sizeof(long long), st->stream));
+    if (int rc = ensure_ring(st, algorithm)) return rc;
   }
   for (uint64_t t = 0; t < n_sweeps; ++t) {
     const uint64_t sweep = st->sweeps + t;
@@ -670,11 +687,18 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
     if (uses_tiles(st, algorithm)) {
       const TileHeader *hdr = static_cast<const TileHeader *>(st->d_tiles);
       const TileJob *rows = reinterpret_cast<const TileJob *>(hdr + 1);
-      const EmLayout em{st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride, kEmStride, 0, 0, 0};
-      for (uint32_t c = 0; c < 2; ++c) {
-        hipError_t e = launch_tile_sweep(hdr, rows, st->tile_grid, sweep, c, measure && c == 1, em, st->stream);
-        if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
-      }
+      const uint64_t rec = 8ull * kEmStride;  // one replica's sub-records of one sweep
+      const uint32_t slot = measure ? (uint32_t)(t % st->ring_slots) : 0u;
+      hipError_t e = hipSuccess;
+      if (measure && slot == 0)
+        e = launch_em_zero(st->d_ring, std::min<uint64_t>(st->ring_slots, n_sweeps - t) * st->n_rep * rec, st->stream);
+      const EmLayout em{st->d_ring + (uint64_t)slot * st->n_rep * rec, rec, 0, 7, (uint32_t)kEmStride};
+      for (uint32_t c = 0; c < 2 && e == hipSuccess; ++c)
+        e = launch_tile_sweep(hdr, rows, st->tile_grid, sweep, c, measure && c == 1, em, st->stream);
+      if (e == hipSuccess && measure && (slot + 1 == st->ring_slots || t + 1 == n_sweeps))
+        e = launch_em_fold(st->d_ring, st->d_series + (st->series_len + t - slot) * (size_t)st->n_rep * kEmStride,
+                           (uint64_t)(slot + 1) * st->n_rep, (uint32_t)kEmStride, st->stream);
+      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
       continue;
     }
     for (uint32_t c = 0; c < 2; ++c) {
@@ -695,8 +719,10 @@ int scamper_sweep_timed(scamper_state *st, int algorithm, uint64_t n_sweeps, int
   const int table_alg = st->continuous ? kContinuousHeatBath : algorithm;
   if (st->tables_for != table_alg && (st->continuous || algorithm == kMetropolis || algorithm == kHeatBath))
     if (int rc = upload_tables(st, table_alg)) return rc;
-  if (measure)
+  if (measure) {
     if (int rc = ensure_series(st, n_sweeps)) return rc;
+    if (int rc = ensure_ring(st, algorithm)) return rc;
+  }
   hipEvent_t a, b;
   SC_HIP(hipEventCreate(&a));
   SC_HIP(hipEventCreate(&b));

@@ -102,10 +102,10 @@ hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const
 
 // ------------------------------------------------------------ tiled jobs
 // Checkerboard Metropolis of Ising and Potts spins for any number of
-// independent periodic lattices ("jobs") in one launch.  A tile is 4096
+// independent periodic lattices ("jobs") in one launch.  A tile is 1024
 // consecutive sites of one colour class of one job; block b of a launch
 // walks the tiles of its XCD's share of the flattened tile list.
-constexpr uint32_t kTileSites = 4096;
+constexpr uint32_t kTileSites = 1024;
 constexpr int kStencilTable = 13;  // 2z+1 for z = 6
 
 struct alignas(16) TileJob {
@@ -144,6 +144,10 @@ hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_
                              uint32_t colour, int measure, EmLayout em, hipStream_t stream);
 // Zero hdr->n_jobs * per_row counters at p (a kernel: safe inside graphs).
 hipError_t launch_tile_zero(const TileHeader *hdr, long long *p, uint64_t per_row, hipStream_t stream);
+// Zero n counters; out[rec * width + c] = sum over s < 8 of
+// ring[(rec * 8 + s) * width + c] for rec < n_rec (kernels: safe inside graphs).
+hipError_t launch_em_zero(long long *p, uint64_t n, hipStream_t stream);
+hipError_t launch_em_fold(const long long *ring, long long *out, uint64_t n_rec, uint32_t width, hipStream_t stream);
 // compact[(r * steps + k) * 8 + c] = sum over s < 8 of ring[((r * steps + k) * 8 + s) * 8 + c].
 hipError_t launch_tile_fold(const TileHeader *hdr, const long long *ring, long long *compact, uint32_t steps,
                             hipStream_t stream);

@@ -11,12 +11,17 @@
 // accept_log_probability (Carlo.rs/src/acceptance.rs:11-22) folded into the
 // host's integer thresholds.  Visit order is colour 0 then colour 1.
 //
-// Layout of work: a tile is 4096 consecutive ranks j of one colour class of
-// one job; wave w of the 256-thread block owns ranks [1024 w, 1024 w + 1024)
-// of the tile.  Lane t evaluates the Philox blocks of group g = j0/16 + t
-// (16 ranks each: top and mid bytes of the acceptance draw, and of the
-// proposal draw for Potts), parks the bytes in LDS, and then updates ranks
-// j0 + 64 k + t (k < 16) so that spin loads and stores are coalesced.
+// Layout of work: a tile is 1024 consecutive ranks j of one colour class of
+// one job.  Four lanes share one Philox group (16 ranks): each computes one
+// of the group's blocks (acceptance top and mid bytes, proposal top and mid
+// bytes for Potts) and a quad transpose hands every lane dword `lane & 3` of
+// each block, i.e. the draws of its own 4 consecutive sites.  When those 4
+// sites lie in at most two rows, a lane's own spins and its neighbours in
+// the other colour are 4-byte windows (two aligned dword loads and one
+// v_alignbyte each); the left/right windows are the same-row window shifted
+// by one byte with one edge byte, plus the periodic wrap where the lane
+// crosses a row end.  Lanes in rows shorter than 4 sites, or whose windows
+// would leave the colour array, take a per-site path with byte loads.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -25,9 +30,8 @@ namespace scamper {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-constexpr int kWaveSites = 1024;
-static_assert(kWaves * kWaveSites == (int)kTileSites, "tile shape");
+constexpr uint32_t kLaneSites = 4;
+static_assert(kThreads * kLaneSites == kTileSites, "tile shape");
 
 __device__ __forceinline__ uint32_t wrap_dec(uint32_t v, uint32_t n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ uint32_t wrap_inc(uint32_t v, uint32_t n) { return v + 1 == n ? 0 : v + 1; }
@@ -60,31 +64,120 @@ __device__ __forceinline__ uint32_t propose2(uint32_t top, uint32_t mid, uint32_
 }
 
 struct TileLds {
-  uint32_t acc_top[kWaves][kWaveSites / 4];
-  uint32_t acc_mid[kWaves][kWaveSites / 4];
-  uint32_t pro_top[kWaves][kWaveSites / 4];
-  uint32_t pro_mid[kWaves][kWaveSites / 4];
   Threshold thr[kStencilTable];
   unsigned long long red[2 + kMaxQ];
 };
 
-__device__ __forceinline__ uint32_t lds_byte(const uint32_t *a, uint32_t idx) {
-  return reinterpret_cast<const uint8_t *>(a)[idx];
+__device__ __forceinline__ uint32_t byte4(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
+
+// The 4 bytes at p, any alignment: the aligned dword at p, or the two aligned
+// dwords around it (the caller has checked that they lie in the allocation).
+__device__ __forceinline__ uint32_t load4(const int8_t *p) {
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t *A = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+  const uint32_t s = (uint32_t)(a & 3);
+  const uint32_t lo = A[0];
+  if (s == 0) return lo;
+  return __builtin_amdgcn_alignbyte(A[1], lo, s);
 }
 
-__device__ __forceinline__ void put_block(uint32_t *dst, const U4 &b) {
-  *reinterpret_cast<uint4 *>(dst) = make_uint4(b.x, b.y, b.z, b.w);
+// p lies in [begin, end) with its aligned read span ending by `end`; the
+// span may start before `begin` (the preceding array of the same
+// allocation), never before the allocation.
+__device__ __forceinline__ bool window_ok(const int8_t *p, const int8_t *begin, const int8_t *end) {
+  return p >= begin && (((uintptr_t)p + 4 + 3) & ~(uintptr_t)3) <= (uintptr_t)end;
 }
 
-// One tile of one job.  Returns after the block's counters (if any) have been
-// added to `em_rec`.
+__device__ __forceinline__ void store4(int8_t *p, uint32_t w) {
+  if (((uintptr_t)p & 3) == 0) {
+    *reinterpret_cast<uint32_t *>(p) = w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = (int8_t)byte4(w, k);
+  }
+}
+
+// Accumulators of a measured half-sweep (per lane).
+struct Meas {
+  int e = 0, m = 0;
+  uint64_t cnt_lo = 0, cnt_hi = 0;  // byte v of (lo, hi): sites in Potts state v (< 16)
+};
+
+// One site: new spin from its own spin, its z neighbours and its draws.
 template <int ND, int KIND>
-__device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour, bool meas,
-                         long long *em_rec) {
+__device__ __forceinline__ int8_t update_site(int8_t s, const int8_t (&n)[2 * ND], uint32_t top, uint32_t mid,
+                                              uint32_t ptop, uint32_t pmid, const TileLds &L, uint32_t q,
+                                              uint64_t key, uint32_t j, uint64_t sweep, uint32_t colour) {
   constexpr int z = 2 * ND;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (KIND == kIsing) {
+    int anti = 0;
+#pragma unroll
+    for (int c = 0; c < z; ++c) anti += (n[c] != s);
+    return decide2(top, mid, L.thr[anti], key, j, sweep, colour) ? (int8_t)-s : s;
+  }
+  uint32_t r;
+  if (q == 2) {
+    r = 1u - (uint32_t)s;
+  } else {
+    r = propose2(ptop, pmid, q - 1, key, j, sweep, colour);
+    r += (r >= (uint32_t)s);
+  }
+  int d = 0;
+#pragma unroll
+  for (int c = 0; c < z; ++c) d += ((uint32_t)n[c] == r) - (n[c] == s);
+  return decide2(top, mid, L.thr[d + z], key, j, sweep, colour) ? (int8_t)r : s;
+}
+
+template <int ND, int KIND>
+__device__ __forceinline__ void measure_site(Meas &M, int8_t ns, int8_t o, const int8_t (&n)[2 * ND]) {
+  constexpr int z = 2 * ND;
+  if (KIND == kIsing) {
+    int sum = 0;
+#pragma unroll
+    for (int c = 0; c < z; ++c) sum += n[c];
+    M.e += ns * sum;
+    M.m += ns + o;
+  } else {
+    int same = 0;
+#pragma unroll
+    for (int c = 0; c < z; ++c) same += (n[c] == ns);
+    M.e += same;
+    const uint64_t one_ns = 1ull << (8 * (ns & 7)), one_o = 1ull << (8 * (o & 7));
+    if (ns < 8) M.cnt_lo += one_ns; else M.cnt_hi += one_ns;
+    if (o < 8) M.cnt_lo += one_o; else M.cnt_hi += one_o;
+  }
+}
+
+// Draw dwords of this lane's 4 sites: block b of the quad's group lives in
+// lane b of the quad; lane s needs dword s of every block.
+struct LaneDraws {
+  uint32_t top, mid, ptop, pmid;
+};
+
+__device__ __forceinline__ LaneDraws quad_draws(uint64_t key, uint32_t g, uint64_t sweep, uint32_t colour,
+                                                bool need_prop, bool need_pmid) {
+  const uint32_t sub = threadIdx.x & 3u;
+  U4 B{0, 0, 0, 0};
+  if (sub < 2u || (sub == 2u && need_prop) || (sub == 3u && need_pmid)) {
+    const uint32_t stream = sub < 2u ? kStreamAccept : kStreamPropose;
+    B = byte_block(key, g, sweep, colour, stream, sub & 1u);
+  }
+  // 4x4 transpose across the quad in two butterfly stages (lane bit 0, then
+  // bit 1); every exchange picks its operands with lane-constant selects.
+  const bool b0 = sub & 1u, b1 = sub & 2u;
+  const uint32_t r0 = __shfl_xor(b0 ? B.x : B.y, 1), r2 = __shfl_xor(b0 ? B.z : B.w, 1);
+  const uint32_t a0 = b0 ? r0 : B.x, a1 = b0 ? B.y : r0, a2 = b0 ? r2 : B.z, a3 = b0 ? B.w : r2;
+  const uint32_t s0 = __shfl_xor(b1 ? a0 : a2, 2), s1 = __shfl_xor(b1 ? a1 : a3, 2);
+  return LaneDraws{b1 ? s0 : a0, b1 ? s1 : a1, b1 ? a2 : s0, b1 ? a3 : s1};
+}
+
+// One tile of one job; measured counters are added to the lane's M.
+template <int ND, int KIND>
+__device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
+                                         bool meas, Meas &M) {
+  constexpr int z = 2 * ND;
   const uint32_t nh = J.nh;
-  const uint32_t jw0 = tile * kTileSites + wave * kWaveSites;
+  const uint32_t j0 = tile * kTileSites + threadIdx.x * kLaneSites;  // first of the lane's 4 sites
   const uint64_t key = J.key;
   const uint32_t q = J.q;
   const uint32_t m = q - 1;
@@ -92,108 +185,159 @@ __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t 
   const bool need_pmid = need_prop && (m & (m - 1)) != 0;
 
   if (threadIdx.x < kStencilTable) L.thr[threadIdx.x] = J.thr[threadIdx.x];
-  if (threadIdx.x < 2 + kMaxQ) L.red[threadIdx.x] = 0;
-  {
-    const uint32_t g = (jw0 >> 4) + lane;
-    if (16u * g < nh) {
-      put_block(&L.acc_top[wave][4 * lane], byte_block(key, g, sweep, colour, kStreamAccept, 0));
-      put_block(&L.acc_mid[wave][4 * lane], byte_block(key, g, sweep, colour, kStreamAccept, 1));
-      if (need_prop) put_block(&L.pro_top[wave][4 * lane], byte_block(key, g, sweep, colour, kStreamPropose, 0));
-      if (need_pmid) put_block(&L.pro_mid[wave][4 * lane], byte_block(key, g, sweep, colour, kStreamPropose, 1));
-    }
-  }
   __syncthreads();
 
-  int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
-  const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * nh;
-  const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
-  int acc_e = 0, acc_m = 0;
-  uint64_t cnt_lo = 0, cnt_hi = 0;  // byte v of (lo, hi) counts sites in Potts state v (< 16)
-
-#pragma unroll 2
-  for (int k = 0; k < 16; ++k) {
-    const uint32_t idx = 64u * k + lane;
-    const uint32_t j = jw0 + idx;
-    if (j >= nh) break;
-    const uint32_t row = div_by(j, J.div_lx2);
-    const uint32_t i = j - row * lx2;
+  // the quad's group must be drawn by all four lanes even where some of
+  // them are past the end of the colour class
+  const uint32_t g = j0 >> 4;
+  const bool group_active = (g << 4) < nh;
+  if (group_active) {
+    int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
+    const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * nh;
+    const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
+    const uint32_t row = div_by(j0 < nh ? j0 : 0u, J.div_lx2);
+    const uint32_t i0 = (j0 < nh ? j0 : 0u) - row * lx2;
     const uint32_t zz = ND == 3 ? div_by(row, J.div_ly) : 0u;
     const uint32_t y = ND == 1 ? 0u : row - zz * ly;
-    const uint32_t p = (colour + y + zz) & 1u;
-    int8_t n[z];
-    {
-      const uint32_t rb = row * lx2;
-      n[0] = oth[rb + (p ? i : wrap_dec(i, lx2))];
-      n[1] = oth[rb + (p ? wrap_inc(i, lx2) : i)];
+    // sites 0 .. ks-1 of the lane lie in row a = `row`, sites ks .. 3 in
+    // row b = row + 1 (rows of >= 4 sites on the window path)
+    const uint32_t ks = lx2 - i0 < kLaneSites ? lx2 - i0 : kLaneSites;
+    const bool two_rows = ks < kLaneSites;
+    const uint32_t yb = ND == 1 ? 0u : (y + 1 == ly ? 0u : y + 1);
+    const uint32_t zb = ND == 3 ? (y + 1 == ly ? zz + 1 : zz) : 0u;
+    const uint32_t pa = (colour + y + zz) & 1u, pb = (colour + yb + zb) & 1u;
+    const uint32_t rba = row * lx2, rbb = rba + lx2;
+    const int64_t plane = (int64_t)lx2 * ly;
+    const int64_t upa = ND >= 2 ? (y == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
+    const int64_t dna = ND >= 2 ? (y + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
+    const int64_t bka = ND >= 3 ? (zz == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
+    const int64_t fra = ND >= 3 ? (zz + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
+    const int64_t upb = ND >= 2 ? (yb == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
+    const int64_t dnb = ND >= 2 ? (yb + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
+    const int64_t bkb = ND >= 3 ? (zb == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
+    const int64_t frb = ND >= 3 ? (zb + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
+    const int8_t *own_end = own + nh, *oth_end = oth + nh;
+    bool vec = lx2 >= kLaneSites && j0 + kLaneSites <= nh && window_ok(own + j0, own, own_end) &&
+               window_ok(oth + j0, oth, oth_end);
+    if (ND >= 2) {
+      vec = vec && window_ok(oth + j0 + upa, oth, oth_end) && window_ok(oth + j0 + dna, oth, oth_end);
+      if (two_rows) vec = vec && window_ok(oth + j0 + upb, oth, oth_end) && window_ok(oth + j0 + dnb, oth, oth_end);
+    }
+    if (ND >= 3) {
+      vec = vec && window_ok(oth + j0 + bka, oth, oth_end) && window_ok(oth + j0 + fra, oth, oth_end);
+      if (two_rows) vec = vec && window_ok(oth + j0 + bkb, oth, oth_end) && window_ok(oth + j0 + frb, oth, oth_end);
+    }
+
+    // windows first, so that the loads are in flight during the Philox rounds
+    uint32_t O = 0, C = 0, Ua = 0, Da = 0, Ba = 0, Fa = 0, Ub = 0, Db = 0, Bb = 0, Fb = 0, el = 0, er = 0;
+    int8_t wrap_r = 0, wrap_l = 0;
+    if (vec) {
+      O = load4(own + j0);
+      C = load4(oth + j0);
+      // left neighbour of site 0 when pa = 0; right neighbour of site 3 when
+      // its row's parity is 1 (site 3 is never at the end of row b)
+      el = (uint8_t)oth[pa ? j0 : (i0 == 0 ? rba + lx2 - 1u : j0 - 1u)];
+      const uint32_t last_p = two_rows ? pb : pa;
+      const uint32_t er_idx = (!two_rows && i0 + kLaneSites == lx2) ? rba : j0 + kLaneSites;
+      er = (uint8_t)oth[last_p ? er_idx : j0];
+      wrap_r = oth[rba];                             // right of site ks-1 (i = lx2-1) when pa = 1
+      wrap_l = two_rows ? oth[rbb + lx2 - 1u] : 0;  // left of site ks (i = 0) when pb = 0
       if (ND >= 2) {
-        n[2] = oth[(wrap_dec(y, ly) + ly * zz) * lx2 + i];
-        n[3] = oth[(wrap_inc(y, ly) + ly * zz) * lx2 + i];
+        Ua = load4(oth + j0 + upa);
+        Da = load4(oth + j0 + dna);
+        Ub = two_rows && upb != upa ? load4(oth + j0 + upb) : Ua;
+        Db = two_rows && dnb != dna ? load4(oth + j0 + dnb) : Da;
       }
       if (ND >= 3) {
-        n[4] = oth[(y + ly * wrap_dec(zz, lz)) * lx2 + i];
-        n[5] = oth[(y + ly * wrap_inc(zz, lz)) * lx2 + i];
+        Ba = load4(oth + j0 + bka);
+        Fa = load4(oth + j0 + fra);
+        Bb = two_rows && bkb != bka ? load4(oth + j0 + bkb) : Ba;
+        Fb = two_rows && frb != fra ? load4(oth + j0 + frb) : Fa;
       }
     }
-    const int8_t s = own[j];
-    int8_t ns = s;
-    const uint32_t top = lds_byte(L.acc_top[wave], idx);
-    const uint32_t mid = lds_byte(L.acc_mid[wave], idx);
-    if (KIND == kIsing) {
-      int anti = 0;
+    const LaneDraws D = quad_draws(key, g, sweep, colour, need_prop, need_pmid);
+
+    if (vec) {
+      const uint32_t Su = (C << 8) | el;          // Su byte k = C byte k-1
+      const uint32_t Sd = (C >> 8) | (er << 24);  // Sd byte k = C byte k+1
+      uint32_t NS = 0;
 #pragma unroll
-      for (int c = 0; c < z; ++c) anti += (n[c] != s);
-      if (decide2(top, mid, L.thr[anti], key, j, sweep, colour)) ns = (int8_t)-s;
+      for (int k = 0; k < (int)kLaneSites; ++k) {
+        const bool in_a = (uint32_t)k < ks;
+        const uint32_t pk = in_a ? pa : pb;
+        const int8_t s = (int8_t)byte4(O, k);
+        const int8_t c = (int8_t)byte4(C, k);
+        int8_t n[z];
+        n[0] = pk ? c : (int8_t)byte4(Su, k);
+        n[1] = pk ? (int8_t)byte4(Sd, k) : c;
+        if ((uint32_t)k + 1u == ks && two_rows && pa) n[1] = wrap_r;
+        if ((uint32_t)k == ks && !pb) n[0] = wrap_l;
+        if (ND >= 2) {
+          n[2] = (int8_t)byte4(in_a ? Ua : Ub, k);
+          n[3] = (int8_t)byte4(in_a ? Da : Db, k);
+        }
+        if (ND >= 3) {
+          n[4] = (int8_t)byte4(in_a ? Ba : Bb, k);
+          n[5] = (int8_t)byte4(in_a ? Fa : Fb, k);
+        }
+        const int8_t ns = update_site<ND, KIND>(s, n, byte4(D.top, k), byte4(D.mid, k), byte4(D.ptop, k),
+                                                byte4(D.pmid, k), L, q, key, j0 + k, sweep, colour);
+        NS |= (uint32_t)(uint8_t)ns << (8 * k);
+        if (meas) measure_site<ND, KIND>(M, ns, c, n);
+      }
+      if (NS != O) store4(own + j0, NS);
     } else {
-      uint32_t r;
-      if (q == 2) {
-        r = 1u - (uint32_t)s;
-      } else {
-        const uint32_t pt = lds_byte(L.pro_top[wave], idx);
-        const uint32_t pm = need_pmid ? lds_byte(L.pro_mid[wave], idx) : 0u;
-        r = propose2(pt, pm, m, key, j, sweep, colour);
-        r += (r >= (uint32_t)s);
-      }
-      int d = 0;
 #pragma unroll
-      for (int c = 0; c < z; ++c) d += ((uint32_t)n[c] == r) - (n[c] == s);
-      if (decide2(top, mid, L.thr[d + z], key, j, sweep, colour)) ns = (int8_t)r;
-    }
-    if (ns != s) own[j] = ns;
-    if (meas) {
-      const int8_t o = oth[j];
-      if (KIND == kIsing) {
-        int sum = 0;
-#pragma unroll
-        for (int c = 0; c < z; ++c) sum += n[c];
-        acc_e += ns * sum;
-        acc_m += ns + o;
-      } else {
-        int same = 0;
-#pragma unroll
-        for (int c = 0; c < z; ++c) same += (n[c] == ns);
-        acc_e += same;
-        const uint64_t one_ns = 1ull << (8 * (ns & 7)), one_o = 1ull << (8 * (o & 7));
-        if (ns < 8) cnt_lo += one_ns; else cnt_hi += one_ns;
-        if (o < 8) cnt_lo += one_o; else cnt_hi += one_o;
+      for (uint32_t k = 0; k < kLaneSites; ++k) {  // unrolled: draw bytes are picked by constant k
+        const uint32_t j = j0 + k;
+        if (j >= nh) continue;
+        const uint32_t r2 = div_by(j, J.div_lx2);
+        const uint32_t i = j - r2 * lx2;
+        const uint32_t z2 = ND == 3 ? div_by(r2, J.div_ly) : 0u;
+        const uint32_t y2 = ND == 1 ? 0u : r2 - z2 * ly;
+        const uint32_t p2 = (colour + y2 + z2) & 1u;
+        const uint32_t rb2 = r2 * lx2;
+        int8_t n[z];
+        n[0] = oth[rb2 + (p2 ? i : wrap_dec(i, lx2))];
+        n[1] = oth[rb2 + (p2 ? wrap_inc(i, lx2) : i)];
+        if (ND >= 2) {
+          n[2] = oth[(wrap_dec(y2, ly) + ly * z2) * lx2 + i];
+          n[3] = oth[(wrap_inc(y2, ly) + ly * z2) * lx2 + i];
+        }
+        if (ND >= 3) {
+          n[4] = oth[(y2 + ly * wrap_dec(z2, lz)) * lx2 + i];
+          n[5] = oth[(y2 + ly * wrap_inc(z2, lz)) * lx2 + i];
+        }
+        const int8_t s = own[j];
+        const int8_t ns = update_site<ND, KIND>(s, n, byte4(D.top, k), byte4(D.mid, k), byte4(D.ptop, k),
+                                                byte4(D.pmid, k), L, q, key, j, sweep, colour);
+        if (ns != s) own[j] = ns;
+        if (meas) measure_site<ND, KIND>(M, ns, oth[j], n);
       }
     }
   }
 
-  if (meas) {
-    const int nc = KIND == kIsing ? 2 : 2 + (int)q;
-    for (int c = 0; c < nc; ++c) {
-      long long v;
-      if (c == 0) v = acc_e;
-      else if (c == 1) v = KIND == kIsing ? acc_m : 0;
-      else v = (long long)(((c - 2 < 8 ? cnt_lo : cnt_hi) >> (8 * ((c - 2) & 7))) & 0xFFu);
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0 && v != 0) atomicAdd(&L.red[c], (unsigned long long)v);
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nc && L.red[threadIdx.x] != 0)
-      atomicAdd(reinterpret_cast<unsigned long long *>(em_rec) + threadIdx.x, L.red[threadIdx.x]);
+  __syncthreads();  // the block's next tile rewrites L.thr
+}
+
+// Block sum of the lanes' counters, added to the record at `rec` (nc values).
+__device__ __forceinline__ void flush(TileLds &L, Meas &M, int nc, bool ising, long long *rec) {
+  const uint32_t lane = threadIdx.x & 63;
+  if (threadIdx.x < 2 + kMaxQ) L.red[threadIdx.x] = 0;
+  __syncthreads();
+  for (int c = 0; c < nc; ++c) {
+    long long v;
+    if (c == 0) v = M.e;
+    else if (c == 1) v = ising ? M.m : 0;
+    else v = (long long)(((c - 2 < 8 ? M.cnt_lo : M.cnt_hi) >> (8 * ((c - 2) & 7))) & 0xFFu);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0 && v != 0) atomicAdd(&L.red[c], (unsigned long long)v);
   }
-  __syncthreads();  // LDS is reused by the block's next tile
+  __syncthreads();
+  if ((int)threadIdx.x < nc && L.red[threadIdx.x] != 0)
+    atomicAdd(reinterpret_cast<unsigned long long *>(rec) + threadIdx.x, L.red[threadIdx.x]);
+  M = Meas{};
+  __syncthreads();
 }
 
 __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__restrict__ hdr,
@@ -206,6 +350,14 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
   const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3, per = gridDim.x >> 3;
   const uint32_t lo = (uint32_t)(((uint64_t)n_tiles * xcd) >> 3);
   const uint32_t hi = (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
+  // A block adds up its measured counters over consecutive tiles of one job
+  // and flushes them when the job changes, after 31 tiles (a lane's 8-bit
+  // state counters gain at most 8 per tile) and at the end.
+  Meas M;
+  long long *rec = nullptr;
+  int nc = 0;
+  bool ising = false;
+  uint32_t cur = ~0u, pending = 0;
   for (uint32_t t = lo + local; t < hi; t += per) {
     uint32_t a = 0, b = n_jobs;  // last job with tile_start <= t
     while (b - a > 1) {
@@ -216,26 +368,53 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
     const uint64_t sweep = J.sweep_base + step;
     if (sweep >= J.sweep_end) continue;
     const bool meas = measure && colour == 1 && sweep >= J.therm;
-    long long *rec = meas ? em.base + J.em_row * em.row_stride + step * em.step_stride +
-                                (uint64_t)(blockIdx.x & em.sub_mask) * em.sub_stride
-                          : nullptr;
+    if (pending && (a != cur || pending == 31)) {
+      flush(L, M, nc, ising, rec);
+      pending = 0;
+    }
+    if (meas) {
+      cur = a;
+      rec = em.base + J.em_row * em.row_stride + step * em.step_stride +
+            (uint64_t)(blockIdx.x & em.sub_mask) * em.sub_stride;
+      ising = J.kind == kIsing;
+      nc = ising ? 2 : 2 + (int)J.q;
+      ++pending;
+    }
     const uint32_t tile = t - J.tile_start;
     const uint32_t sel = J.nd * 2 + (J.kind == kPotts ? 1 : 0);
     switch (sel) {
-      case 2: run_tile<1, kIsing>(L, J, tile, sweep, colour, meas, rec); break;
-      case 3: run_tile<1, kPotts>(L, J, tile, sweep, colour, meas, rec); break;
-      case 4: run_tile<2, kIsing>(L, J, tile, sweep, colour, meas, rec); break;
-      case 5: run_tile<2, kPotts>(L, J, tile, sweep, colour, meas, rec); break;
-      case 6: run_tile<3, kIsing>(L, J, tile, sweep, colour, meas, rec); break;
-      default: run_tile<3, kPotts>(L, J, tile, sweep, colour, meas, rec); break;
+      case 2: run_tile<1, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+      case 3: run_tile<1, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+      case 4: run_tile<2, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+      case 5: run_tile<2, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+      case 6: run_tile<3, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+      default: run_tile<3, kPotts>(L, J, tile, sweep, colour, meas, M); break;
     }
   }
+  if (pending) flush(L, M, nc, ising, rec);
 }
 
 __global__ void k_tile_zero(const TileHeader *__restrict__ hdr, long long *__restrict__ p, uint64_t per_row) {
   const uint64_t n = (uint64_t)hdr->n_jobs * per_row;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0;
+}
+
+__global__ void k_em_zero(long long *__restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ void k_em_fold(const long long *__restrict__ ring, long long *__restrict__ out, uint64_t n_rec,
+                          uint32_t width) {
+  const uint64_t n = n_rec * width;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t rec = i / width, c = i - rec * width;
+    long long s = 0;
+#pragma unroll
+    for (int sub = 0; sub < 8; ++sub) s += ring[(rec * 8 + sub) * width + c];
+    out[i] = s;
+  }
 }
 
 __global__ void k_tile_fold(const TileHeader *__restrict__ hdr, const long long *__restrict__ ring,
@@ -266,6 +445,16 @@ hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_
 
 hipError_t launch_tile_zero(const TileHeader *hdr, long long *p, uint64_t per_row, hipStream_t stream) {
   hipLaunchKernelGGL(k_tile_zero, dim3(256), dim3(256), 0, stream, hdr, p, per_row);
+  return hipGetLastError();
+}
+
+hipError_t launch_em_zero(long long *p, uint64_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_em_zero, dim3(64), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_em_fold(const long long *ring, long long *out, uint64_t n_rec, uint32_t width, hipStream_t stream) {
+  hipLaunchKernelGGL(k_em_fold, dim3(64), dim3(256), 0, stream, ring, out, n_rec, width);
   return hipGetLastError();
 }
 

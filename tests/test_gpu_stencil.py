@@ -57,6 +57,10 @@ CASES = [
     ((34, 20, 14), 1, 6, 0, 0.9, 6),
     ((2, 2, 2), 1, 3, 0, 0.5, 4),
     ((258, 34), 0, 2, 0, 0.4406868, 6),
+    ((64, 8, 6), 1, 3, 0, 0.55, 5),
+    ((4098,), 0, 2, 0, 0.7, 5),
+    ((96, 40), 1, 5, 0, 1.2, 5),
+    ((40, 40, 40), 1, 4, 0, 0.62863, 4),
 ]
 
 
@@ -121,6 +125,26 @@ def test_replicas_are_independent_streams(sc, orc):
         key = orc.replica_key(seed, r)
         want = _stencil(orc, starts[r], dims, 0, 2, b, False, key, 0, 6)
         assert np.array_equal(st.download_i8(r), want)
+
+
+@pytest.mark.parametrize("dims,kind,q", [((34, 22), 1, 4), ((66, 10, 6), 1, 3), ((36, 14), 0, 2)])
+def test_replicas_with_unaligned_colour_arrays(sc, orc, dims, kind, q):
+    """Several replicas whose colour arrays start at addresses that are not
+    16-byte aligned (N/2 not a multiple of 16), at different temperatures."""
+    seed = 21
+    betas = [0.3, 0.9, 1.4]
+    st = sc.State(_lattice(sc, dims), sc.model(kind, 1.0, q), len(betas), seed)
+    starts = []
+    for r, b in enumerate(betas):
+        st.init(True, r)
+        st.set_beta(b, r)
+        starts.append(st.download_i8(r))
+    st.sweep(0, 5, measure=True)
+    for r, b in enumerate(betas):
+        key = orc.replica_key(seed, r)
+        want = _stencil(orc, starts[r], dims, kind, q, b, False, key, 0, 5)
+        got = st.download_i8(r)
+        assert np.array_equal(got, want), f"replica {r}: {(got != want).sum()} spins differ"
 
 
 def test_upload_roundtrip_and_validation(sc):
