@@ -17,7 +17,7 @@ Other workloads, one JSON line each (not the headline):
     --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
     --workload serve  the serving block alone
 
-    python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 200]
 """
 from __future__ import annotations
 
@@ -130,13 +130,17 @@ def serving(sc, args) -> dict:
         row = {"lambda_per_s": rate, **res.summary(reqs), "grid": res.grid}
         out["rates"].append(row)
         print(f"# serving lambda={rate}: {json.dumps(row)}", file=sys.stderr, flush=True)
-    sub = sv.potts_mix(args.plain_requests, 1e9, seed=13)
+    # Same requests both ways, all arriving at once; budgets a tenth of the
+    # serving mix's so that the one-job-at-a-time leg stays short (both legs
+    # scale linearly with the budget).
+    sub = sv.potts_mix(args.plain_requests, 1e9, seed=13, sweeps=(1_000, 10_000))
     batched = sv.serve(sub, keep=len(sub))
     plain = sv.plain(sub, keep=len(sub))
     same = all(batched.estimates[k][name].mean == plain.estimates[k][name].mean
                for k in range(len(sub)) for name in ("Energy", "Magnetization", "M2", "M4"))
     out["plain_vs_batched"] = {
         "requests": len(sub),
+        "sweeps": "log-uniform in [1e3, 1e4]",
         "site_updates": sum(r.updates for r in sub),
         "batched_spin_flips_per_ns": round(batched.flips_per_ns, 3),
         "plain_spin_flips_per_ns": round(plain.flips_per_ns, 3),
@@ -204,15 +208,15 @@ def on3(sc, args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=16384)
     ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "serve"], default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[10.0, 1000.0])
-    ap.add_argument("--plain-requests", type=int, default=12)
+    ap.add_argument("--plain-requests", type=int, default=48)
     ap.add_argument("--pt-steps", type=int, default=10_000)
     ap.add_argument("--on3-steps", type=int, default=200)
     args = ap.parse_args()

@@ -182,6 +182,7 @@ int upload_tables(scamper_state *st, int alg) {
       t.n_tiles = per;
       t.em_row = r;
       std::memcpy(t.thr, p[r].thr, sizeof t.thr);
+      fill_tile_luts(t);
     }
     TileHeader h{st->n_rep, st->n_rep * per, {0, 0}};
     if (!st->d_tiles) SC_HIP(hipMalloc(&st->d_tiles, sizeof(TileHeader) + rows.size() * sizeof(TileJob)));

@@ -171,6 +171,138 @@ __device__ __forceinline__ LaneDraws quad_draws(uint64_t key, uint32_t g, uint64
   return LaneDraws{b1 ? s0 : a0, b1 ? s1 : a1, b1 ? a2 : s0, b1 ? a3 : s1};
 }
 
+// ---------------------------------------------------------------- SWAR
+// Four sites per 32-bit word, one per byte.
+constexpr uint32_t kMsb = 0x80808080u, kOnes = 0x01010101u;
+
+// Byte-wise unsigned x < y, reported in bit 7 of each byte.  The subtraction
+// is done on 7-bit fields (no borrow crosses a byte), bit 7 is repaired, and
+// the borrow out of bit 7 is the comparison.
+__device__ __forceinline__ uint32_t lt_bytes(uint32_t x, uint32_t y) {
+  const uint32_t same = ~(x ^ y);
+  const uint32_t d = ((x | kMsb) - (y & ~kMsb)) ^ (same & kMsb);
+  return ((~x & y) | (same & d)) & kMsb;
+}
+
+// Byte-wise x == y, reported in bit 7 of each byte.
+__device__ __forceinline__ uint32_t eq_bytes(uint32_t x, uint32_t y) {
+  const uint32_t t = x ^ y;
+  return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & kMsb;
+}
+
+__device__ __forceinline__ uint32_t put_byte(uint32_t w, uint32_t k, uint32_t v) {
+  return (w & ~(0xFFu << (8u * k))) | (v << (8u * k));
+}
+
+// Per-byte look-up in a 16-entry byte table (4 words), index < 16 per byte.
+__device__ __forceinline__ uint32_t lut16(const uint32_t (&t)[4], uint32_t idx) {
+  const uint32_t sel = idx & 0x07070707u;
+  const uint32_t lo = __builtin_amdgcn_perm(t[1], t[0], sel);
+  const uint32_t hi = __builtin_amdgcn_perm(t[3], t[2], sel);
+  const uint32_t mk = ((idx >> 3) & kOnes) * 0xFFu;
+  return (hi & mk) | (lo & ~mk);
+}
+
+// Accept mask (bit 7 per byte) of "word < K[idx]" for the four sites: the
+// top and second bytes decide; 16-bit ties (one decision in 65536) compare
+// the low half of the draw exactly.
+template <bool SMALL_INDEX>
+__device__ __forceinline__ uint32_t accept_bytes(uint32_t idx, const LaneDraws &D, const TileJob &J,
+                                                 const TileLds &L, uint64_t key, uint32_t j0, uint64_t sweep,
+                                                 uint32_t colour) {
+  uint32_t k8, al, m8;
+  if (SMALL_INDEX) {  // every index < 8
+    k8 = __builtin_amdgcn_perm(J.lut_k8[1], J.lut_k8[0], idx);
+    al = __builtin_amdgcn_perm(J.lut_al[1], J.lut_al[0], idx);
+    m8 = __builtin_amdgcn_perm(J.lut_m8[1], J.lut_m8[0], idx);
+  } else {
+    k8 = lut16(J.lut_k8, idx);
+    al = lut16(J.lut_al, idx);
+    m8 = lut16(J.lut_m8, idx);
+  }
+  al &= kMsb;
+  const uint32_t eq_t = eq_bytes(D.top, k8);
+  uint32_t acc = lt_bytes(D.top, k8) | (eq_t & lt_bytes(D.mid, m8)) | al;
+  uint32_t tie = eq_t & eq_bytes(D.mid, m8) & ~al;
+  while (tie) {
+    const uint32_t k = (uint32_t)__builtin_ctz(tie) >> 3;
+    tie &= tie - 1;
+    if (site_low(key, j0 + k, sweep, colour, kStreamAccept) < L.thr[byte4(idx, (int)k)].k16) acc |= 0x80u << (8u * k);
+  }
+  return acc;
+}
+
+template <int ND>
+__device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
+                                               const LaneDraws &D, const TileJob &J, const TileLds &L, uint64_t key,
+                                               uint32_t j0, uint64_t sweep, uint32_t colour, bool meas, Meas &M) {
+  constexpr int z = 2 * ND;
+  uint32_t a2 = 0;  // spins are 0x01 / 0xff: bit 1 of s ^ n is set exactly when they differ
+#pragma unroll
+  for (int c = 0; c < z; ++c) a2 += (S ^ N[c]) & 0x02020202u;
+  const uint32_t anti = a2 >> 1;
+  const uint32_t m = accept_bytes<true>(anti, D, J, L, key, j0, sweep, colour) >> 7;
+  const uint32_t NS = S ^ (((m << 8) - m) & 0xFEFEFEFEu);
+  if (meas) {
+    uint32_t b2 = 0;
+#pragma unroll
+    for (int c = 0; c < z; ++c) b2 += (NS ^ N[c]) & 0x02020202u;
+    M.e += 4 * z - 2 * (int)__builtin_amdgcn_sad_u8(b2 >> 1, 0u, 0u);
+    M.m += 8 - 2 * (__builtin_popcount(NS & kMsb) + __builtin_popcount(C & kMsb));
+  }
+  return NS;
+}
+
+template <int ND>
+__device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
+                                               const LaneDraws &D, const TileJob &J, const TileLds &L, uint32_t q,
+                                               uint64_t key, uint32_t j0, uint64_t sweep, uint32_t colour, bool meas,
+                                               Meas &M) {
+  constexpr int z = 2 * ND;
+  uint32_t R;
+  if (q == 2) {
+    R = S ^ kOnes;
+  } else {
+    // floor(top * m / 256) per byte from 16-bit lane products (top * m <
+    // 4096); a byte whose bucket straddles an index boundary (low byte of
+    // the product + m > 256) is settled with the finer bytes.
+    const uint32_t m = q - 1;
+    const uint32_t pe = (D.ptop & 0x00FF00FFu) * m, po = ((D.ptop >> 8) & 0x00FF00FFu) * m;
+    uint32_t idx = ((pe >> 8) & 0x000F000Fu) | (po & 0x0F000F00u);
+    const uint32_t bias = (m + 255u) * 0x00010001u;
+    uint32_t strad = ((((pe & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) |
+                     (((((po & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) << 8);
+    while (strad) {
+      const uint32_t k = (uint32_t)__builtin_ctz(strad) >> 3;
+      strad &= strad - 1;
+      idx = put_byte(idx, k, propose2(byte4(D.ptop, (int)k), byte4(D.pmid, (int)k), m, key, j0 + k, sweep, colour));
+    }
+    R = idx + ((((idx | kMsb) - S) & kMsb) >> 7);  // skip the current state
+  }
+  uint32_t cr = 0, cs = 0;
+#pragma unroll
+  for (int c = 0; c < z; ++c) {
+    cr += eq_bytes(N[c], R) >> 7;
+    cs += eq_bytes(N[c], S) >> 7;
+  }
+  const uint32_t idx = cr + (uint32_t)z * kOnes - cs;  // d + z per byte, 0 .. 2z
+  const uint32_t am = (accept_bytes<false>(idx, D, J, L, key, j0, sweep, colour) >> 7) * 0xFFu;
+  const uint32_t NS = (R & am) | (S & ~am);
+  if (meas) {
+    uint32_t same = 0;
+#pragma unroll
+    for (int c = 0; c < z; ++c) same += eq_bytes(N[c], NS) >> 7;
+    M.e += (int)__builtin_amdgcn_sad_u8(same, 0u, 0u);
+    for (uint32_t v = 0; v < q; ++v) {
+      const uint32_t vv = v * kOnes;
+      const uint64_t n = __builtin_amdgcn_sad_u8(eq_bytes(NS, vv) >> 7, 0u, 0u) +
+                         __builtin_amdgcn_sad_u8(eq_bytes(C, vv) >> 7, 0u, 0u);
+      if (v < 8) M.cnt_lo += n << (8 * v); else M.cnt_hi += n << (8 * (v - 8));
+    }
+  }
+  return NS;
+}
+
 // One tile of one job; measured counters are added to the lane's M.
 template <int ND, int KIND>
 __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
@@ -260,31 +392,24 @@ __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t 
     if (vec) {
       const uint32_t Su = (C << 8) | el;          // Su byte k = C byte k-1
       const uint32_t Sd = (C >> 8) | (er << 24);  // Sd byte k = C byte k+1
-      uint32_t NS = 0;
-#pragma unroll
-      for (int k = 0; k < (int)kLaneSites; ++k) {
-        const bool in_a = (uint32_t)k < ks;
-        const uint32_t pk = in_a ? pa : pb;
-        const int8_t s = (int8_t)byte4(O, k);
-        const int8_t c = (int8_t)byte4(C, k);
-        int8_t n[z];
-        n[0] = pk ? c : (int8_t)byte4(Su, k);
-        n[1] = pk ? (int8_t)byte4(Sd, k) : c;
-        if ((uint32_t)k + 1u == ks && two_rows && pa) n[1] = wrap_r;
-        if ((uint32_t)k == ks && !pb) n[0] = wrap_l;
-        if (ND >= 2) {
-          n[2] = (int8_t)byte4(in_a ? Ua : Ub, k);
-          n[3] = (int8_t)byte4(in_a ? Da : Db, k);
-        }
-        if (ND >= 3) {
-          n[4] = (int8_t)byte4(in_a ? Ba : Bb, k);
-          n[5] = (int8_t)byte4(in_a ? Fa : Fb, k);
-        }
-        const int8_t ns = update_site<ND, KIND>(s, n, byte4(D.top, k), byte4(D.mid, k), byte4(D.ptop, k),
-                                                byte4(D.pmid, k), L, q, key, j0 + k, sweep, colour);
-        NS |= (uint32_t)(uint8_t)ns << (8 * k);
-        if (meas) measure_site<ND, KIND>(M, ns, c, n);
+      // bytes of row a, and bytes whose row parity is 1
+      const uint32_t amask = two_rows ? ((1u << (8u * ks)) - 1u) : 0xFFFFFFFFu;
+      const uint32_t pmask = (pa ? amask : 0u) | (pb ? ~amask : 0u);
+      uint32_t N[z];
+      N[0] = (C & pmask) | (Su & ~pmask);
+      N[1] = (Sd & pmask) | (C & ~pmask);
+      if (two_rows && pa) N[1] = put_byte(N[1], ks - 1u, (uint8_t)wrap_r);
+      if (two_rows && !pb) N[0] = put_byte(N[0], ks, (uint8_t)wrap_l);
+      if (ND >= 2) {
+        N[2] = (Ua & amask) | (Ub & ~amask);
+        N[3] = (Da & amask) | (Db & ~amask);
       }
+      if (ND >= 3) {
+        N[4] = (Ba & amask) | (Bb & ~amask);
+        N[5] = (Fa & amask) | (Fb & ~amask);
+      }
+      const uint32_t NS = KIND == kIsing ? ising_word<ND>(O, N, C, D, J, L, key, j0, sweep, colour, meas, M)
+                                         : potts_word<ND>(O, N, C, D, J, L, q, key, j0, sweep, colour, meas, M);
       if (NS != O) store4(own + j0, NS);
     } else {
 #pragma unroll

@@ -392,6 +392,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
     t.n_tiles = (uint32_t)((J.geom.nh + kTileSites - 1) / kTileSites);
     t.em_row = (uint32_t)F.rows.size();
     std::memcpy(t.thr, J.thr, sizeof t.thr);
+    fill_tile_luts(t);
     tiles += t.n_tiles;
     F.rows.push_back(RowInfo{id, J.enqueued, steps});
     F.updates += J.n_sites * steps;
