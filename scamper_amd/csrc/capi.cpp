@@ -88,6 +88,7 @@ struct scamper_state {
   // replica (header, then the rows)
   void *d_tiles = nullptr;
   uint32_t tile_grid = 0;
+  uint32_t tile_lane = 4;  // sites per lane of the tiled launches
   // measured tile sweeps add into 8 per-XCD sub-records per replica in this
   // ring, folded into the series every ring_slots sweeps
   long long *d_ring = nullptr;
@@ -159,7 +160,8 @@ int upload_tables(scamper_state *st, int alg) {
   SC_HIP(hipMemcpy(st->d_params, p.data(), p.size() * sizeof(ReplicaParams), hipMemcpyHostToDevice));
   if (uses_tiles(st, alg)) {
     // one row per replica: every sweep of every replica, measured from 0
-    const uint32_t per = (uint32_t)((st->geom.nh + kTileSites - 1) / kTileSites);
+    st->tile_lane = tile_lane_sites((uint64_t)st->n_rep * st->geom.nh);
+    const uint32_t per = (uint32_t)((st->geom.nh + tile_sites(st->tile_lane) - 1) / tile_sites(st->tile_lane));
     std::vector<TileJob> rows(st->n_rep);
     for (uint32_t r = 0; r < st->n_rep; ++r) {
       TileJob &t = rows[r];
@@ -695,7 +697,7 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
         e = launch_em_zero(st->d_ring, std::min<uint64_t>(st->ring_slots, n_sweeps - t) * st->n_rep * rec, st->stream);
       const EmLayout em{st->d_ring + (uint64_t)slot * st->n_rep * rec, rec, 0, 7, (uint32_t)kEmStride};
       for (uint32_t c = 0; c < 2 && e == hipSuccess; ++c)
-        e = launch_tile_sweep(hdr, rows, st->tile_grid, sweep, c, measure && c == 1, em, st->stream);
+        e = launch_tile_sweep(hdr, rows, st->tile_grid, st->tile_lane, sweep, c, measure && c == 1, em, st->stream);
       if (e == hipSuccess && measure && (slot + 1 == st->ring_slots || t + 1 == n_sweeps))
         e = launch_em_fold(st->d_ring, st->d_series + (st->series_len + t - slot) * (size_t)st->n_rep * kEmStride,
                            (uint64_t)(slot + 1) * st->n_rep, (uint32_t)kEmStride, st->stream);

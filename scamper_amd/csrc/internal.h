@@ -102,10 +102,13 @@ hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const
 
 // ------------------------------------------------------------ tiled jobs
 // Checkerboard Metropolis of Ising and Potts spins for any number of
-// independent periodic lattices ("jobs") in one launch.  A tile is 1024
-// consecutive sites of one colour class of one job; block b of a launch
-// walks the tiles of its XCD's share of the flattened tile list.
-constexpr uint32_t kTileSites = 1024;
+// independent periodic lattices ("jobs") in one launch.  A tile is
+// 256 * lane_sites consecutive sites of one colour class of one job (a lane
+// updates 4 or 16 sites); block b of a launch walks the tiles of its XCD's
+// share of the flattened tile list.  Both shapes give identical results;
+// 16 sites per lane amortises more per-lane work, 4 gives more waves.
+uint32_t tile_lane_sites(uint64_t total_sites);  // the shape for a launch over that many sites
+inline uint32_t tile_sites(uint32_t lane_sites) { return 256u * lane_sites; }
 constexpr int kStencilTable = 13;  // 2z+1 for z = 6
 
 struct alignas(16) TileJob {
@@ -156,8 +159,8 @@ struct EmLayout {
 };
 
 uint64_t div_magic(uint32_t d);
-hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint64_t step,
-                             uint32_t colour, int measure, EmLayout em, hipStream_t stream);
+hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint32_t lane_sites,
+                             uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream);
 // Zero hdr->n_jobs * per_row counters at p (a kernel: safe inside graphs).
 hipError_t launch_tile_zero(const TileHeader *hdr, long long *p, uint64_t per_row, hipStream_t stream);
 // Zero n counters; out[rec * width + c] = sum over s < 8 of

@@ -11,17 +11,17 @@
 // accept_log_probability (Carlo.rs/src/acceptance.rs:11-22) folded into the
 // host's integer thresholds.  Visit order is colour 0 then colour 1.
 //
-// Layout of work: a tile is 1024 consecutive ranks j of one colour class of
-// one job.  Four lanes share one Philox group (16 ranks): each computes one
-// of the group's blocks (acceptance top and mid bytes, proposal top and mid
-// bytes for Potts) and a quad transpose hands every lane dword `lane & 3` of
-// each block, i.e. the draws of its own 4 consecutive sites.  When those 4
-// sites lie in at most two rows, a lane's own spins and its neighbours in
-// the other colour are 4-byte windows (two aligned dword loads and one
-// v_alignbyte each); the left/right windows are the same-row window shifted
-// by one byte with one edge byte, plus the periodic wrap where the lane
-// crosses a row end.  Lanes in rows shorter than 4 sites, or whose windows
-// would leave the colour array, take a per-site path with byte loads.
+// Layout of work: a tile is 4096 consecutive ranks j of one colour class of
+// one job, 16 per lane: one Philox group, so a lane's draws are exactly the
+// bytes of its own blocks (acceptance top and mid bytes, and the proposal's
+// for Potts).  Four sites share a 32-bit word and are updated together with
+// byte-wise integer arithmetic (SWAR).  When a lane's 16 sites lie in at most
+// two rows, its own spins and its neighbours in the other colour are
+// 16-byte windows (two or four aligned 8-byte loads and a funnel shift); the
+// left/right windows are the same-row window shifted by one byte with one
+// edge byte, plus the periodic wrap where the lane crosses a row end.  Lanes
+// in rows shorter than 16 sites, or whose windows would leave the colour
+// array, take a per-site path with byte loads.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -30,8 +30,6 @@ namespace scamper {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t kLaneSites = 4;
-static_assert(kThreads * kLaneSites == kTileSites, "tile shape");
 
 __device__ __forceinline__ uint32_t wrap_dec(uint32_t v, uint32_t n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ uint32_t wrap_inc(uint32_t v, uint32_t n) { return v + 1 == n ? 0 : v + 1; }
@@ -70,6 +68,71 @@ struct TileLds {
 
 __device__ __forceinline__ uint32_t byte4(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
 
+// A 16-byte window: byte k is byte k of (lo, hi) in little-endian order.
+struct W16 {
+  uint64_t lo, hi;
+};
+
+__device__ __forceinline__ uint32_t word16(const W16 &w, int k) {
+  return (uint32_t)((k < 2 ? w.lo : w.hi) >> (32 * (k & 1)));
+}
+
+// The 16 bytes at p, any alignment: the aligned 16 bytes at p, or the
+// aligned 32 bytes around it (the caller has checked they are allocated).
+__device__ __forceinline__ W16 load16(const int8_t *p) {
+  const uintptr_t a = (uintptr_t)p;
+  const uint64_t *A = reinterpret_cast<const uint64_t *>(a & ~(uintptr_t)15);
+  const uint32_t s = (uint32_t)(a & 15);
+  const uint64_t a0 = A[0], a1 = A[1];
+  if (s == 0) return W16{a0, a1};
+  const uint64_t a2 = A[2], a3 = A[3];
+  const bool q = s >= 8;
+  const uint32_t r = 8 * (s & 7);
+  const uint64_t e0 = q ? a1 : a0, e1 = q ? a2 : a1, e2 = q ? a3 : a2;
+  if (r == 0) return W16{e0, e1};
+  return W16{(e0 >> r) | (e1 << (64 - r)), (e1 >> r) | (e2 << (64 - r))};
+}
+
+// w moved one byte up (byte 0 = e: the left neighbours) or down (byte 15 = e:
+// the right neighbours).
+__device__ __forceinline__ W16 shift_up(const W16 &w, uint32_t e) {
+  return W16{(w.lo << 8) | e, (w.hi << 8) | (w.lo >> 56)};
+}
+__device__ __forceinline__ W16 shift_down(const W16 &w, uint32_t e) {
+  return W16{(w.lo >> 8) | (w.hi << 56), (w.hi >> 8) | ((uint64_t)e << 56)};
+}
+
+__device__ __forceinline__ W16 put_byte16(const W16 &w, uint32_t k, uint32_t v) {
+  const uint32_t sh = 8u * (k & 7u);
+  const uint64_t m = 0xFFull << sh, b = (uint64_t)v << sh;
+  return k < 8 ? W16{(w.lo & ~m) | b, w.hi} : W16{w.lo, (w.hi & ~m) | b};
+}
+
+__device__ __forceinline__ W16 select16(const W16 &a, const W16 &b, const W16 &mask) {  // a where mask
+  return W16{(a.lo & mask.lo) | (b.lo & ~mask.lo), (a.hi & mask.hi) | (b.hi & ~mask.hi)};
+}
+
+__device__ __forceinline__ void store16(int8_t *p, const W16 &w) {
+  const uintptr_t a = (uintptr_t)p;
+  if ((a & 15) == 0) {
+    *reinterpret_cast<uint4 *>(p) = make_uint4((uint32_t)w.lo, (uint32_t)(w.lo >> 32), (uint32_t)w.hi,
+                                               (uint32_t)(w.hi >> 32));
+  } else if ((a & 7) == 0) {
+    reinterpret_cast<uint64_t *>(p)[0] = w.lo;
+    reinterpret_cast<uint64_t *>(p)[1] = w.hi;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) p[k] = (int8_t)((k < 8 ? w.lo : w.hi) >> (8 * (k & 7)));
+  }
+}
+
+// p lies in [begin, end) and its aligned read span ends by `end` (the span
+// may start before `begin`: the preceding array of the same allocation,
+// never before the allocation).
+__device__ __forceinline__ bool window_ok16(const int8_t *p, const int8_t *begin, const int8_t *end) {
+  return p >= begin && (((uintptr_t)p + 16 + 15) & ~(uintptr_t)15) <= (uintptr_t)end;
+}
+
 // The 4 bytes at p, any alignment: the aligned dword at p, or the two aligned
 // dwords around it (the caller has checked that they lie in the allocation).
 __device__ __forceinline__ uint32_t load4(const int8_t *p) {
@@ -84,7 +147,7 @@ __device__ __forceinline__ uint32_t load4(const int8_t *p) {
 // p lies in [begin, end) with its aligned read span ending by `end`; the
 // span may start before `begin` (the preceding array of the same
 // allocation), never before the allocation.
-__device__ __forceinline__ bool window_ok(const int8_t *p, const int8_t *begin, const int8_t *end) {
+__device__ __forceinline__ bool window_ok4(const int8_t *p, const int8_t *begin, const int8_t *end) {
   return p >= begin && (((uintptr_t)p + 4 + 3) & ~(uintptr_t)3) <= (uintptr_t)end;
 }
 
@@ -148,8 +211,7 @@ __device__ __forceinline__ void measure_site(Meas &M, int8_t ns, int8_t o, const
   }
 }
 
-// Draw dwords of this lane's 4 sites: block b of the quad's group lives in
-// lane b of the quad; lane s needs dword s of every block.
+// Draw bytes of four sites (one word of each block).
 struct LaneDraws {
   uint32_t top, mid, ptop, pmid;
 };
@@ -170,6 +232,7 @@ __device__ __forceinline__ LaneDraws quad_draws(uint64_t key, uint32_t g, uint64
   const uint32_t s0 = __shfl_xor(b1 ? a0 : a2, 2), s1 = __shfl_xor(b1 ? a1 : a3, 2);
   return LaneDraws{b1 ? s0 : a0, b1 ? s1 : a1, b1 ? a2 : s0, b1 ? a3 : s1};
 }
+
 
 // ---------------------------------------------------------------- SWAR
 // Four sites per 32-bit word, one per byte.
@@ -303,13 +366,14 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
   return NS;
 }
 
-// One tile of one job; measured counters are added to the lane's M.
+// One tile of 1024 sites, 4 per lane (quad-shared Philox group); measured
+// counters are added to the lane's M.
 template <int ND, int KIND>
-__device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
+__device__ __forceinline__ void run_tile4(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
                                          bool meas, Meas &M) {
   constexpr int z = 2 * ND;
   const uint32_t nh = J.nh;
-  const uint32_t j0 = tile * kTileSites + threadIdx.x * kLaneSites;  // first of the lane's 4 sites
+  const uint32_t j0 = tile * (4u * kThreads) + threadIdx.x * 4u;  // first of the lane's 4 sites
   const uint64_t key = J.key;
   const uint32_t q = J.q;
   const uint32_t m = q - 1;
@@ -333,8 +397,8 @@ __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t 
     const uint32_t y = ND == 1 ? 0u : row - zz * ly;
     // sites 0 .. ks-1 of the lane lie in row a = `row`, sites ks .. 3 in
     // row b = row + 1 (rows of >= 4 sites on the window path)
-    const uint32_t ks = lx2 - i0 < kLaneSites ? lx2 - i0 : kLaneSites;
-    const bool two_rows = ks < kLaneSites;
+    const uint32_t ks = lx2 - i0 < 4u ? lx2 - i0 : 4u;
+    const bool two_rows = ks < 4u;
     const uint32_t yb = ND == 1 ? 0u : (y + 1 == ly ? 0u : y + 1);
     const uint32_t zb = ND == 3 ? (y + 1 == ly ? zz + 1 : zz) : 0u;
     const uint32_t pa = (colour + y + zz) & 1u, pb = (colour + yb + zb) & 1u;
@@ -349,15 +413,15 @@ __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t 
     const int64_t bkb = ND >= 3 ? (zb == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
     const int64_t frb = ND >= 3 ? (zb + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
     const int8_t *own_end = own + nh, *oth_end = oth + nh;
-    bool vec = lx2 >= kLaneSites && j0 + kLaneSites <= nh && window_ok(own + j0, own, own_end) &&
-               window_ok(oth + j0, oth, oth_end);
+    bool vec = lx2 >= 4u && j0 + 4u <= nh && window_ok4(own + j0, own, own_end) &&
+               window_ok4(oth + j0, oth, oth_end);
     if (ND >= 2) {
-      vec = vec && window_ok(oth + j0 + upa, oth, oth_end) && window_ok(oth + j0 + dna, oth, oth_end);
-      if (two_rows) vec = vec && window_ok(oth + j0 + upb, oth, oth_end) && window_ok(oth + j0 + dnb, oth, oth_end);
+      vec = vec && window_ok4(oth + j0 + upa, oth, oth_end) && window_ok4(oth + j0 + dna, oth, oth_end);
+      if (two_rows) vec = vec && window_ok4(oth + j0 + upb, oth, oth_end) && window_ok4(oth + j0 + dnb, oth, oth_end);
     }
     if (ND >= 3) {
-      vec = vec && window_ok(oth + j0 + bka, oth, oth_end) && window_ok(oth + j0 + fra, oth, oth_end);
-      if (two_rows) vec = vec && window_ok(oth + j0 + bkb, oth, oth_end) && window_ok(oth + j0 + frb, oth, oth_end);
+      vec = vec && window_ok4(oth + j0 + bka, oth, oth_end) && window_ok4(oth + j0 + fra, oth, oth_end);
+      if (two_rows) vec = vec && window_ok4(oth + j0 + bkb, oth, oth_end) && window_ok4(oth + j0 + frb, oth, oth_end);
     }
 
     // windows first, so that the loads are in flight during the Philox rounds
@@ -370,7 +434,7 @@ __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t 
       // its row's parity is 1 (site 3 is never at the end of row b)
       el = (uint8_t)oth[pa ? j0 : (i0 == 0 ? rba + lx2 - 1u : j0 - 1u)];
       const uint32_t last_p = two_rows ? pb : pa;
-      const uint32_t er_idx = (!two_rows && i0 + kLaneSites == lx2) ? rba : j0 + kLaneSites;
+      const uint32_t er_idx = (!two_rows && i0 + 4u == lx2) ? rba : j0 + 4u;
       er = (uint8_t)oth[last_p ? er_idx : j0];
       wrap_r = oth[rba];                             // right of site ks-1 (i = lx2-1) when pa = 1
       wrap_l = two_rows ? oth[rbb + lx2 - 1u] : 0;  // left of site ks (i = 0) when pb = 0
@@ -413,7 +477,7 @@ __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t 
       if (NS != O) store4(own + j0, NS);
     } else {
 #pragma unroll
-      for (uint32_t k = 0; k < kLaneSites; ++k) {  // unrolled: draw bytes are picked by constant k
+      for (uint32_t k = 0; k < 4u; ++k) {  // unrolled: draw bytes are picked by constant k
         const uint32_t j = j0 + k;
         if (j >= nh) continue;
         const uint32_t r2 = div_by(j, J.div_lx2);
@@ -445,6 +509,171 @@ __device__ __forceinline__ void run_tile(TileLds &L, const TileJob &J, uint32_t 
   __syncthreads();  // the block's next tile rewrites L.thr
 }
 
+// One tile of 4096 sites, 16 per lane (one Philox group each); measured
+// counters are added to the lane's M.
+template <int ND, int KIND>
+__device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
+                                         bool meas, Meas &M) {
+  constexpr int z = 2 * ND;
+  const uint32_t nh = J.nh;
+  const uint32_t j0 = tile * (16u * kThreads) + threadIdx.x * 16u;  // first of the lane's 16 sites
+  const uint64_t key = J.key;
+  const uint32_t q = J.q;
+  const uint32_t m = q - 1;
+  const bool need_prop = KIND == kPotts && q > 2;
+  const bool need_pmid = need_prop && (m & (m - 1)) != 0;
+
+  if (threadIdx.x < kStencilTable) L.thr[threadIdx.x] = J.thr[threadIdx.x];
+  __syncthreads();
+
+  if (j0 < nh) {
+    int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
+    const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * nh;
+    const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
+    const uint32_t row = div_by(j0, J.div_lx2);
+    const uint32_t i0 = j0 - row * lx2;
+    const uint32_t zz = ND == 3 ? div_by(row, J.div_ly) : 0u;
+    const uint32_t y = ND == 1 ? 0u : row - zz * ly;
+    // sites 0 .. ks-1 of the lane lie in row a = `row`, sites ks .. 15 in
+    // row b = row + 1 (rows of >= 16 sites on the window path)
+    const uint32_t ks = lx2 - i0 < 16u ? lx2 - i0 : 16u;
+    const bool two_rows = ks < 16u;
+    const uint32_t yb = ND == 1 ? 0u : (y + 1 == ly ? 0u : y + 1);
+    const uint32_t zb = ND == 3 ? (y + 1 == ly ? zz + 1 : zz) : 0u;
+    const uint32_t pa = (colour + y + zz) & 1u, pb = (colour + yb + zb) & 1u;
+    const uint32_t rba = row * lx2, rbb = rba + lx2;
+    const int64_t plane = (int64_t)lx2 * ly;
+    const int64_t upa = ND >= 2 ? (y == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
+    const int64_t dna = ND >= 2 ? (y + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
+    const int64_t bka = ND >= 3 ? (zz == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
+    const int64_t fra = ND >= 3 ? (zz + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
+    const int64_t upb = ND >= 2 ? (yb == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
+    const int64_t dnb = ND >= 2 ? (yb + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
+    const int64_t bkb = ND >= 3 ? (zb == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
+    const int64_t frb = ND >= 3 ? (zb + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
+    const int8_t *own_end = own + nh, *oth_end = oth + nh;
+    bool vec = lx2 >= 16u && j0 + 16u <= nh && window_ok16(own + j0, own, own_end) &&
+               window_ok16(oth + j0, oth, oth_end);
+    if (ND >= 2) {
+      vec = vec && window_ok16(oth + j0 + upa, oth, oth_end) && window_ok16(oth + j0 + dna, oth, oth_end);
+      if (two_rows) vec = vec && window_ok16(oth + j0 + upb, oth, oth_end) && window_ok16(oth + j0 + dnb, oth, oth_end);
+    }
+    if (ND >= 3) {
+      vec = vec && window_ok16(oth + j0 + bka, oth, oth_end) && window_ok16(oth + j0 + fra, oth, oth_end);
+      if (two_rows) vec = vec && window_ok16(oth + j0 + bkb, oth, oth_end) && window_ok16(oth + j0 + frb, oth, oth_end);
+    }
+
+    // windows first, so that the loads are in flight during the Philox rounds
+    W16 O{0, 0}, C{0, 0}, Ua{0, 0}, Da{0, 0}, Ba{0, 0}, Fa{0, 0}, Ub{0, 0}, Db{0, 0}, Bb{0, 0}, Fb{0, 0};
+    uint32_t el = 0, er = 0;
+    int8_t wrap_r = 0, wrap_l = 0;
+    if (vec) {
+      O = load16(own + j0);
+      C = load16(oth + j0);
+      // left neighbour of site 0 when pa = 0; right neighbour of site 15
+      // when its row's parity is 1 (site 15 is never at the end of row b)
+      el = (uint8_t)oth[pa ? j0 : (i0 == 0 ? rba + lx2 - 1u : j0 - 1u)];
+      const uint32_t last_p = two_rows ? pb : pa;
+      const uint32_t er_idx = (!two_rows && i0 + 16u == lx2) ? rba : j0 + 16u;
+      er = (uint8_t)oth[last_p ? er_idx : j0];
+      wrap_r = oth[rba];                             // right of site ks-1 (i = lx2-1) when pa = 1
+      wrap_l = two_rows ? oth[rbb + lx2 - 1u] : 0;  // left of site ks (i = 0) when pb = 0
+      if (ND >= 2) {
+        Ua = load16(oth + j0 + upa);
+        Da = load16(oth + j0 + dna);
+        Ub = two_rows && upb != upa ? load16(oth + j0 + upb) : Ua;
+        Db = two_rows && dnb != dna ? load16(oth + j0 + dnb) : Da;
+      }
+      if (ND >= 3) {
+        Ba = load16(oth + j0 + bka);
+        Fa = load16(oth + j0 + fra);
+        Bb = two_rows && bkb != bka ? load16(oth + j0 + bkb) : Ba;
+        Fb = two_rows && frb != fra ? load16(oth + j0 + frb) : Fa;
+      }
+    }
+    const uint32_t g = j0 >> 4;
+    const U4 T = byte_block(key, g, sweep, colour, kStreamAccept, 0);
+    const U4 Mb = byte_block(key, g, sweep, colour, kStreamAccept, 1);
+    const U4 PT = need_prop ? byte_block(key, g, sweep, colour, kStreamPropose, 0) : U4{0, 0, 0, 0};
+    const U4 PM = need_pmid ? byte_block(key, g, sweep, colour, kStreamPropose, 1) : U4{0, 0, 0, 0};
+
+    if (vec) {
+      // bytes of row a, and bytes whose row parity is 1
+      const W16 amask = two_rows ? W16{ks >= 8 ? ~0ull : ((1ull << (8u * ks)) - 1u),
+                                       ks > 8 ? ((1ull << (8u * (ks - 8))) - 1u) : 0ull}
+                                 : W16{~0ull, ~0ull};
+      const W16 pmask{(pa ? amask.lo : 0ull) | (pb ? ~amask.lo : 0ull), (pa ? amask.hi : 0ull) | (pb ? ~amask.hi : 0ull)};
+      const W16 Su = shift_up(C, el), Sd = shift_down(C, er);
+      W16 Lw = select16(C, Su, pmask), Rw = select16(Sd, C, pmask);
+      if (two_rows && pa) Rw = put_byte16(Rw, ks - 1u, (uint8_t)wrap_r);
+      if (two_rows && !pb) Lw = put_byte16(Lw, ks, (uint8_t)wrap_l);
+      W16 Uw{0, 0}, Dw{0, 0}, Bw{0, 0}, Fw{0, 0};
+      if (ND >= 2) {
+        Uw = select16(Ua, Ub, amask);
+        Dw = select16(Da, Db, amask);
+      }
+      if (ND >= 3) {
+        Bw = select16(Ba, Bb, amask);
+        Fw = select16(Fa, Fb, amask);
+      }
+      uint32_t NSw[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        uint32_t N[z];
+        N[0] = word16(Lw, w);
+        N[1] = word16(Rw, w);
+        if (ND >= 2) {
+          N[2] = word16(Uw, w);
+          N[3] = word16(Dw, w);
+        }
+        if (ND >= 3) {
+          N[4] = word16(Bw, w);
+          N[5] = word16(Fw, w);
+        }
+        const LaneDraws D{word_of(T, (uint32_t)w), word_of(Mb, (uint32_t)w), word_of(PT, (uint32_t)w),
+                          word_of(PM, (uint32_t)w)};
+        const uint32_t S = word16(O, w), Cw = word16(C, w), jw = j0 + 4u * (uint32_t)w;
+        NSw[w] = KIND == kIsing ? ising_word<ND>(S, N, Cw, D, J, L, key, jw, sweep, colour, meas, M)
+                                : potts_word<ND>(S, N, Cw, D, J, L, q, key, jw, sweep, colour, meas, M);
+      }
+      const W16 NS{(uint64_t)NSw[0] | ((uint64_t)NSw[1] << 32), (uint64_t)NSw[2] | ((uint64_t)NSw[3] << 32)};
+      if (NS.lo != O.lo || NS.hi != O.hi) store16(own + j0, NS);
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < 16u; ++k) {  // unrolled: draw bytes are picked by constant k
+        const uint32_t j = j0 + k;
+        if (j >= nh) continue;
+        const uint32_t r2 = div_by(j, J.div_lx2);
+        const uint32_t i = j - r2 * lx2;
+        const uint32_t z2 = ND == 3 ? div_by(r2, J.div_ly) : 0u;
+        const uint32_t y2 = ND == 1 ? 0u : r2 - z2 * ly;
+        const uint32_t p2 = (colour + y2 + z2) & 1u;
+        const uint32_t rb2 = r2 * lx2;
+        int8_t n[z];
+        n[0] = oth[rb2 + (p2 ? i : wrap_dec(i, lx2))];
+        n[1] = oth[rb2 + (p2 ? wrap_inc(i, lx2) : i)];
+        if (ND >= 2) {
+          n[2] = oth[(wrap_dec(y2, ly) + ly * z2) * lx2 + i];
+          n[3] = oth[(wrap_inc(y2, ly) + ly * z2) * lx2 + i];
+        }
+        if (ND >= 3) {
+          n[4] = oth[(y2 + ly * wrap_dec(z2, lz)) * lx2 + i];
+          n[5] = oth[(y2 + ly * wrap_inc(z2, lz)) * lx2 + i];
+        }
+        const int8_t s = own[j];
+        const uint32_t wk = k >> 2;
+        const int8_t ns = update_site<ND, KIND>(s, n, byte4(word_of(T, wk), (int)(k & 3)), byte4(word_of(Mb, wk), (int)(k & 3)),
+                                                byte4(word_of(PT, wk), (int)(k & 3)), byte4(word_of(PM, wk), (int)(k & 3)),
+                                                L, q, key, j, sweep, colour);
+        if (ns != s) own[j] = ns;
+        if (meas) measure_site<ND, KIND>(M, ns, oth[j], n);
+      }
+    }
+  }
+
+  __syncthreads();  // the block's next tile rewrites L.thr
+}
+
 // Block sum of the lanes' counters, added to the record at `rec` (nc values).
 __device__ __forceinline__ void flush(TileLds &L, Meas &M, int nc, bool ising, long long *rec) {
   const uint32_t lane = threadIdx.x & 63;
@@ -465,6 +694,7 @@ __device__ __forceinline__ void flush(TileLds &L, Meas &M, int nc, bool ising, l
   __syncthreads();
 }
 
+template <int LANE>
 __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__restrict__ hdr,
                                                          const TileJob *__restrict__ jobs, uint64_t step,
                                                          uint32_t colour, int measure, EmLayout em) {
@@ -476,8 +706,9 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
   const uint32_t lo = (uint32_t)(((uint64_t)n_tiles * xcd) >> 3);
   const uint32_t hi = (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
   // A block adds up its measured counters over consecutive tiles of one job
-  // and flushes them when the job changes, after 31 tiles (a lane's 8-bit
-  // state counters gain at most 8 per tile) and at the end.
+  // and flushes them when the job changes, before a lane's 8-bit state
+  // counters can overflow (they gain at most 2 * LANE per tile) and at the end.
+  constexpr uint32_t kFlushTiles = 255 / (2 * LANE);
   Meas M;
   long long *rec = nullptr;
   int nc = 0;
@@ -493,7 +724,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
     const uint64_t sweep = J.sweep_base + step;
     if (sweep >= J.sweep_end) continue;
     const bool meas = measure && colour == 1 && sweep >= J.therm;
-    if (pending && (a != cur || pending == 31)) {
+    if (pending && (a != cur || pending == kFlushTiles)) {
       flush(L, M, nc, ising, rec);
       pending = 0;
     }
@@ -507,13 +738,24 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
     }
     const uint32_t tile = t - J.tile_start;
     const uint32_t sel = J.nd * 2 + (J.kind == kPotts ? 1 : 0);
-    switch (sel) {
-      case 2: run_tile<1, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-      case 3: run_tile<1, kPotts>(L, J, tile, sweep, colour, meas, M); break;
-      case 4: run_tile<2, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-      case 5: run_tile<2, kPotts>(L, J, tile, sweep, colour, meas, M); break;
-      case 6: run_tile<3, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-      default: run_tile<3, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+    if (LANE == 16) {
+      switch (sel) {
+        case 2: run_tile16<1, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+        case 3: run_tile16<1, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+        case 4: run_tile16<2, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+        case 5: run_tile16<2, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+        case 6: run_tile16<3, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+        default: run_tile16<3, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+      }
+    } else {
+      switch (sel) {
+        case 2: run_tile4<1, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+        case 3: run_tile4<1, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+        case 4: run_tile4<2, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+        case 5: run_tile4<2, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+        case 6: run_tile4<3, kIsing>(L, J, tile, sweep, colour, meas, M); break;
+        default: run_tile4<3, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+      }
     }
   }
   if (pending) flush(L, M, nc, ising, rec);
@@ -561,10 +803,19 @@ uint64_t div_magic(uint32_t d) {
   return ~0ull / d + 1;  // ceil(2^64 / d) for d >= 2 (exact when d divides 2^64)
 }
 
-hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint64_t step,
-                             uint32_t colour, int measure, EmLayout em, hipStream_t stream) {
+// 16 sites per lane once a launch covers >= 8 Mi colour-class sites (>= 2048
+// waves of 64 lanes); below that, 4 per lane keeps enough waves in flight.
+uint32_t tile_lane_sites(uint64_t total_sites) { return total_sites >= (8ull << 20) ? 16u : 4u; }
+
+hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint32_t lane_sites,
+                             uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream) {
   if (grid == 0 || (grid & 7u)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_tile_sweep, dim3(grid), dim3(kThreads), 0, stream, hdr, jobs, step, colour, measure, em);
+  if (lane_sites == 16)
+    hipLaunchKernelGGL(k_tile_sweep<16>, dim3(grid), dim3(kThreads), 0, stream, hdr, jobs, step, colour, measure, em);
+  else if (lane_sites == 4)
+    hipLaunchKernelGGL(k_tile_sweep<4>, dim3(grid), dim3(kThreads), 0, stream, hdr, jobs, step, colour, measure, em);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -126,6 +126,7 @@ struct scamper_pool {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   uint32_t graph_grid[2] = {0, 0};
+  uint32_t graph_lane[2] = {0, 0};
   hipEvent_t ev_start[2] = {nullptr, nullptr}, ev_graph[2] = {nullptr, nullptr}, ev_end[2] = {nullptr, nullptr};
   InFlight inflight[2];
   uint64_t chunk = 0, next_id = 1;
@@ -167,29 +168,29 @@ namespace {
 uint64_t ring_row_counters(const scamper_pool *P) { return (uint64_t)P->cfg.chunk_sweeps * kSubs * kRec; }
 
 // The chunk's launches: zero the ring, K steps of (colour 0, colour 1), fold.
-int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, hipStream_t st) {
+int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, hipStream_t st) {
   const TileHeader *hdr = reinterpret_cast<const TileHeader *>(P->d_table[p]);
   const TileJob *rows = reinterpret_cast<const TileJob *>(hdr + 1);
   const uint32_t K = P->cfg.chunk_sweeps;
   const EmLayout em{P->d_ring[p], ring_row_counters(P), (uint64_t)kSubs * kRec, kSubs - 1, kRec};
   hipError_t e = launch_tile_zero(hdr, P->d_ring[p], ring_row_counters(P), st);
   for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
-    e = launch_tile_sweep(hdr, rows, grid, k, 0, 0, em, st);
-    if (e == hipSuccess) e = launch_tile_sweep(hdr, rows, grid, k, 1, 1, em, st);
+    e = launch_tile_sweep(hdr, rows, grid, lane, k, 0, 0, em, st);
+    if (e == hipSuccess) e = launch_tile_sweep(hdr, rows, grid, lane, k, 1, 1, em, st);
   }
   if (e == hipSuccess) e = launch_tile_fold(hdr, P->d_ring[p], P->d_compact[p], K, st);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "chunk launch: %s", hipGetErrorString(e));
   return SCAMPER_OK;
 }
 
-int build_graph(scamper_pool *P, int p, uint32_t grid) {
+int build_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane) {
   if (P->exec[p]) hipGraphExecDestroy(P->exec[p]);
   if (P->graph[p]) hipGraphDestroy(P->graph[p]);
   P->exec[p] = nullptr;
   P->graph[p] = nullptr;
   P->graph_grid[p] = 0;
   SC_HIP(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_chunk_body(P, p, grid, P->cap_stream);
+  const int rc = enqueue_chunk_body(P, p, grid, lane, P->cap_stream);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(P->cap_stream, &g);
   if (rc) {
@@ -200,6 +201,7 @@ int build_graph(scamper_pool *P, int p, uint32_t grid) {
   P->graph[p] = g;
   SC_HIP(hipGraphInstantiate(&P->exec[p], g, nullptr, nullptr, 0));
   P->graph_grid[p] = grid;
+  P->graph_lane[p] = lane;
   return SCAMPER_OK;
 }
 
@@ -369,6 +371,9 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   F.rows.clear();
   F.updates = 0;
   uint32_t tiles = 0;
+  uint64_t total_sites = 0;
+  for (uint64_t id : P->running) total_sites += P->jobs.at(id)->geom.nh;
+  const uint32_t lane = tile_lane_sites(total_sites);
   for (uint64_t id : P->running) {
     Job &J = *P->jobs.at(id);
     const uint32_t steps = (uint32_t)std::min<uint64_t>(K, J.total - J.enqueued);
@@ -389,7 +394,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
     t.kind = (uint32_t)J.spec.model.kind;
     t.q = (uint32_t)J.spec.model.q;
     t.tile_start = tiles;
-    t.n_tiles = (uint32_t)((J.geom.nh + kTileSites - 1) / kTileSites);
+    t.n_tiles = (uint32_t)((J.geom.nh + tile_sites(lane) - 1) / tile_sites(lane));
     t.em_row = (uint32_t)F.rows.size();
     std::memcpy(t.thr, J.thr, sizeof t.thr);
     fill_tile_luts(t);
@@ -408,14 +413,14 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   hdr->pad[0] = hdr->pad[1] = 0;
   int candidate;
   const uint32_t grid = choose_grid(P, &candidate);
-  if (P->cfg.use_graphs && P->graph_grid[p] != grid)
-    if (int rc = build_graph(P, p, grid)) return rc;
+  if (P->cfg.use_graphs && (P->graph_grid[p] != grid || P->graph_lane[p] != lane))
+    if (int rc = build_graph(P, p, grid, lane)) return rc;
   SC_HIP(hipMemcpyAsync(P->d_table[p], P->h_table[p], sizeof(TileHeader) + F.rows.size() * sizeof(TileJob),
                         hipMemcpyHostToDevice, P->stream));
   SC_HIP(hipEventRecord(P->ev_start[p], P->stream));
   if (P->cfg.use_graphs) {
     SC_HIP(hipGraphLaunch(P->exec[p], P->stream));
-  } else if (int rc = enqueue_chunk_body(P, p, grid, P->stream)) {
+  } else if (int rc = enqueue_chunk_body(P, p, grid, lane, P->stream)) {
     return rc;
   }
   SC_HIP(hipEventRecord(P->ev_graph[p], P->stream));

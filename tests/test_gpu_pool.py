@@ -97,6 +97,25 @@ def test_pool_final_spins_match_oracle(sc, orc):
     assert res[2][-1] == chain.magnetization()
 
 
+def test_large_mix_matches_small_launches_and_oracle(sc, orc):
+    """With >= 8 Mi sites per colour in the table the pool switches to 16
+    sites per lane; every job still equals its run_one (4 per lane) and the
+    big job's final spins equal the oracle."""
+    specs = [_spec(sc, (4100, 4100), 1, 3, 1.0, 1, 2, 61), _spec(sc, (34, 20, 14), 1, 6, 0.9, 2, 5, 62),
+             _spec(sc, (130, 66), 1, 4, 1.098612, 1, 6, 63)]
+    res, _ = _run_pool(sc, specs, chunk_sweeps=4, max_jobs=8, spin_budget_bytes=1 << 26)
+    for i, (d, k, q, b, t, m) in enumerate([((34, 20, 14), 1, 6, 0.9, 2, 5), ((130, 66), 1, 4, 1.098612, 1, 6)]):
+        est, e, mm = sc.run_one(sc.Lattice.hypercubic(list(d), True), sc.model(k, 1.0, q), sc.METROPOLIS, b,
+                                True, t, m, 10, 62 + i, series=True)
+        assert np.array_equal(res[i + 1][1], e) and np.array_equal(res[i + 1][2], mm)
+    key = orc.replica_key(61, 0)
+    lat = orc.Lattice.hypercubic([4100, 4100], True)
+    mo = orc.model(1, 1.0, 3, 1)
+    start = orc.random_start(lat, mo, key).astype(np.int8)
+    orc.stencil_sweeps(start, (4100, 4100), 1, 3, 1.0, 1.0, False, key, 0, 3)
+    assert np.array_equal(res[0][3], start)
+
+
 def test_alone_equals_in_the_mix(sc):
     specs = [_spec(sc, d, k, q, b, t, m, 500 + i) for i, (d, k, q, b, t, m) in enumerate(MIX)]
     alone = {}
