@@ -124,20 +124,19 @@ struct alignas(16) TileJob {
   uint32_t n_tiles, em_row, pad0, pad1;
   Threshold thr[kStencilTable];  // by anti-aligned count (Ising) or d + z (Potts)
   // byte look-up tables over the same index (v_perm operands, 4 entries per
-  // word): top byte of K (0xff where K = 2^32), 0xff where K = 2^32, second
-  // byte of K
-  uint32_t lut_k8[4], lut_al[4], lut_m8[4];
+  // word): top and second byte of K, both 0xff where K = 2^32 (a draw whose
+  // top 16 bits are all ones then ties, and the tie is settled exactly)
+  uint32_t lut_k8[4], lut_m8[4];
 };
 
 // Fill a row's byte look-up tables from its thresholds.
 inline void fill_tile_luts(TileJob &t) {
-  for (int w = 0; w < 4; ++w) t.lut_k8[w] = t.lut_al[w] = t.lut_m8[w] = 0;
+  for (int w = 0; w < 4; ++w) t.lut_k8[w] = t.lut_m8[w] = 0;
   for (int a = 0; a < kStencilTable; ++a) {
     const uint32_t k8 = t.thr[a].k8;
     const uint32_t sh = 8u * (uint32_t)(a & 3);
     t.lut_k8[a >> 2] |= (k8 > 0xFFu ? 0xFFu : k8) << sh;
-    t.lut_al[a >> 2] |= (k8 > 0xFFu ? 0xFFu : 0u) << sh;
-    t.lut_m8[a >> 2] |= (uint32_t)t.thr[a].m8 << sh;
+    t.lut_m8[a >> 2] |= (k8 > 0xFFu ? 0xFFu : (uint32_t)t.thr[a].m8) << sh;
   }
 }
 

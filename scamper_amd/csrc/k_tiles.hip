@@ -267,33 +267,35 @@ __device__ __forceinline__ uint32_t lut16(const uint32_t (&t)[4], uint32_t idx) 
 }
 
 // Accept mask (bit 7 per byte) of "word < K[idx]" for the four sites: the
-// top and second bytes decide; 16-bit ties (one decision in 65536) compare
-// the low half of the draw exactly.
+// top and second bytes decide; 16-bit ties (one decision in 65536, also
+// every draw with top bytes 0xffff where K = 2^32) are settled exactly.
 template <bool SMALL_INDEX>
 __device__ __forceinline__ uint32_t accept_bytes(uint32_t idx, const LaneDraws &D, const TileJob &J,
                                                  const TileLds &L, uint64_t key, uint32_t j0, uint64_t sweep,
                                                  uint32_t colour) {
-  uint32_t k8, al, m8;
+  uint32_t k8, m8;
   if (SMALL_INDEX) {  // every index < 8
     k8 = __builtin_amdgcn_perm(J.lut_k8[1], J.lut_k8[0], idx);
-    al = __builtin_amdgcn_perm(J.lut_al[1], J.lut_al[0], idx);
     m8 = __builtin_amdgcn_perm(J.lut_m8[1], J.lut_m8[0], idx);
   } else {
     k8 = lut16(J.lut_k8, idx);
-    al = lut16(J.lut_al, idx);
     m8 = lut16(J.lut_m8, idx);
   }
-  al &= kMsb;
   const uint32_t eq_t = eq_bytes(D.top, k8);
-  uint32_t acc = lt_bytes(D.top, k8) | (eq_t & lt_bytes(D.mid, m8)) | al;
-  uint32_t tie = eq_t & eq_bytes(D.mid, m8) & ~al;
+  uint32_t acc = lt_bytes(D.top, k8) | (eq_t & lt_bytes(D.mid, m8));
+  uint32_t tie = eq_t & eq_bytes(D.mid, m8);
   while (tie) {
     const uint32_t k = (uint32_t)__builtin_ctz(tie) >> 3;
     tie &= tie - 1;
-    if (site_low(key, j0 + k, sweep, colour, kStreamAccept) < L.thr[byte4(idx, (int)k)].k16) acc |= 0x80u << (8u * k);
+    const Threshold &t = L.thr[byte4(idx, (int)k)];
+    if (t.k8 > 0xFFu || site_low(key, j0 + k, sweep, colour, kStreamAccept) < t.k16) acc |= 0x80u << (8u * k);
   }
   return acc;
 }
+
+// 1 in every byte where a and b differ, for byte values below 16 (Potts
+// states): (a ^ b) + 0x7f sets bit 7 of a byte exactly when a ^ b != 0.
+__device__ __forceinline__ uint32_t ne01(uint32_t a, uint32_t b) { return (((a ^ b) + 0x7F7F7F7Fu) >> 7) & kOnes; }
 
 template <int ND>
 __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
@@ -342,20 +344,21 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
     }
     R = idx + ((((idx | kMsb) - S) & kMsb) >> 7);  // skip the current state
   }
-  uint32_t cr = 0, cs = 0;
+  // d + z = z + #(n != s) - #(n != r) per byte, 0 .. 2z
+  uint32_t ns_ = (uint32_t)z * kOnes, nr = 0;
 #pragma unroll
   for (int c = 0; c < z; ++c) {
-    cr += eq_bytes(N[c], R) >> 7;
-    cs += eq_bytes(N[c], S) >> 7;
+    ns_ += ne01(N[c], S);
+    nr += ne01(N[c], R);
   }
-  const uint32_t idx = cr + (uint32_t)z * kOnes - cs;  // d + z per byte, 0 .. 2z
+  const uint32_t idx = ns_ - nr;
   const uint32_t am = (accept_bytes<false>(idx, D, J, L, key, j0, sweep, colour) >> 7) * 0xFFu;
   const uint32_t NS = (R & am) | (S & ~am);
   if (meas) {
-    uint32_t same = 0;
+    uint32_t differ = 0;
 #pragma unroll
-    for (int c = 0; c < z; ++c) same += eq_bytes(N[c], NS) >> 7;
-    M.e += (int)__builtin_amdgcn_sad_u8(same, 0u, 0u);
+    for (int c = 0; c < z; ++c) differ += ne01(N[c], NS);
+    M.e += 4 * z - (int)__builtin_amdgcn_sad_u8(differ, 0u, 0u);
     for (uint32_t v = 0; v < q; ++v) {
       const uint32_t vv = v * kOnes;
       const uint64_t n = __builtin_amdgcn_sad_u8(eq_bytes(NS, vv) >> 7, 0u, 0u) +
@@ -803,9 +806,10 @@ uint64_t div_magic(uint32_t d) {
   return ~0ull / d + 1;  // ceil(2^64 / d) for d >= 2 (exact when d divides 2^64)
 }
 
-// 16 sites per lane once a launch covers >= 8 Mi colour-class sites (>= 2048
-// waves of 64 lanes); below that, 4 per lane keeps enough waves in flight.
-uint32_t tile_lane_sites(uint64_t total_sites) { return total_sites >= (8ull << 20) ? 16u : 4u; }
+// 16 sites per lane once a launch covers >= 4 Mi colour-class sites (>= 4096
+// waves of 64 lanes, 4 per SIMD); below that, 4 per lane keeps enough waves
+// in flight.
+uint32_t tile_lane_sites(uint64_t total_sites) { return total_sites >= (4ull << 20) ? 16u : 4u; }
 
 hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint32_t lane_sites,
                              uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream) {

@@ -61,7 +61,7 @@ CASES = [
     ((4098,), 0, 2, 0, 0.7, 5),
     ((96, 40), 1, 5, 0, 1.2, 5),
     ((40, 40, 40), 1, 4, 0, 0.62863, 4),
-    # 16 sites per lane (launches over >= 8 Mi sites per colour)
+    # 16 sites per lane (launches over >= 4 Mi sites per colour)
     ((4100, 4100), 1, 3, 0, 1.0, 3),
     ((204, 206, 400), 1, 4, 0, 0.62863, 3),
     ((4098, 4100), 0, 2, 0, 0.4406868, 3),
