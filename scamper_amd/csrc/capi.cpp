@@ -438,6 +438,7 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
     st->geom.ly = (uint32_t)l.dims[1];
     st->geom.lz = (uint32_t)l.dims[2];
     st->geom.nh = l.n_sites / 2;
+    set_divisors(st->geom);
     st->z = 2 * l.nd;
   } else {
     // Graph lattice: integer weights |w| <= 127, weighted degree <= 32,

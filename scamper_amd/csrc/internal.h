@@ -42,7 +42,31 @@ struct StencilGeom {
   uint32_t ly;
   uint32_t lz;
   uint64_t nh;   // sites per colour = lx2 * ly * lz
+  uint64_t div_lx2 = 0, div_ly = 0;  // division magics (set_divisors)
 };
+
+// ceil(2^64 / d) for d >= 2, 0 for d = 1: floor(n / d) = mulhi64(n, magic)
+// for every 32-bit n (the error term n * (magic * d - 2^64) / 2^64 stays
+// below n * d / 2^64 < 1 / d).
+uint64_t div_magic(uint32_t d);
+inline void set_divisors(StencilGeom &g) {
+  g.div_lx2 = div_magic(g.lx2);
+  g.div_ly = div_magic(g.ly);
+}
+
+__device__ __forceinline__ uint32_t magic_div(uint32_t n, uint64_t magic) {
+  return magic ? (uint32_t)__umul64hi((uint64_t)n, magic) : n;
+}
+
+// Row (y + ly z), column i, y and z of colour-class rank j.
+struct SiteCoords {
+  uint32_t row, i, y, z;
+};
+__device__ __forceinline__ SiteCoords site_coords(uint64_t j, const StencilGeom &g) {
+  const uint32_t row = g.nh < (1ull << 32) ? magic_div((uint32_t)j, g.div_lx2) : (uint32_t)(j / g.lx2);
+  const uint32_t zz = magic_div(row, g.div_ly);
+  return SiteCoords{row, (uint32_t)(j - (uint64_t)row * g.lx2), row - zz * g.ly, zz};
+}
 
 // Graph lattice (any CSR) with a greedy colouring.  Spins are site-major
 // int8 per replica; class_sites lists the sites of every colour class,
@@ -157,7 +181,6 @@ struct EmLayout {
   uint32_t sub_stride;
 };
 
-uint64_t div_magic(uint32_t d);
 hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint32_t lane_sites,
                              uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream);
 // Zero hdr->n_jobs * per_row counters at p (a kernel: safe inside graphs).

@@ -122,8 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
     cen[k] = 0;
     pl[k] = Planes{0, 0, 0};
     if (!valid[k]) continue;
-    const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-    const uint32_t y = row % g.ly, zz = row / g.ly;
+    const SiteCoords sc_ = site_coords(j, g);
+    const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     const uint32_t p = (colour + y + zz) & 1u;
     const uint64_t rb = (uint64_t)row * g.lx2;
     uint64_t x[z];
@@ -238,8 +238,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_init(uint64_t *__restrict__ words
   const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint32_t colour = blockIdx.y;
   if (j >= g.nh) return;
-  const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-  const uint32_t y = row % g.ly, zz = row / g.ly;
+  const SiteCoords sc_ = site_coords(j, g);
+  const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
   const uint64_t site = 2ull * i + ((colour + y + zz) & 1u) + 2ull * g.lx2 * row;
   uint64_t v = ~0ull;
   if (random) {

@@ -101,8 +101,8 @@ __global__ __launch_bounds__(kBlock) void k_on3(float *__restrict__ spins, const
   if (j < g.nh) {
     float *__restrict__ own = spins + (2ull * rep + colour) * 3ull * g.nh;
     const float *__restrict__ oth = spins + (2ull * rep + 1 - colour) * 3ull * g.nh;
-    const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-    const uint32_t y = row % g.ly, zz = row / g.ly;
+    const SiteCoords sc_ = site_coords(j, g);
+    const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     const uint32_t p = (colour + y + zz) & 1u;
     V3 centre{0.f, 0.f, 0.f};
     const V3 h = local_field<ND>(oth, g, j, row, i, y, zz, p, jc, &centre);
@@ -178,8 +178,8 @@ __global__ __launch_bounds__(kBlock) void k_on3_measure(const float *__restrict_
   if (j < g.nh) {
     const float *own = spins + (2ull * rep + 1) * 3ull * g.nh;
     const float *oth = spins + (2ull * rep) * 3ull * g.nh;
-    const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-    const uint32_t y = row % g.ly, zz = row / g.ly;
+    const SiteCoords sc_ = site_coords(j, g);
+    const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     V3 centre{0.f, 0.f, 0.f};
     const V3 h = local_field<ND>(oth, g, j, row, i, y, zz, (1 + y + zz) & 1u, jc, &centre);
     const V3 s = load3(own, g.nh, j);
@@ -206,8 +206,8 @@ __global__ __launch_bounds__(kBlock) void k_on3_init(float *__restrict__ spins, 
   const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint32_t colour = blockIdx.y;
   if (j >= g.nh) return;
-  const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-  const uint32_t y = row % g.ly, zz = row / g.ly;
+  const SiteCoords sc_ = site_coords(j, g);
+  const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
   const uint64_t site = 2ull * i + ((colour + y + zz) & 1u) + 2ull * g.lx2 * row;
   V3 v{1.f, 0.f, 0.f};  // ordered_spin (models.rs:479-484)
   if (random)

@@ -116,8 +116,8 @@ __global__ __launch_bounds__(kBlock) void k_site(int8_t *__restrict__ spins,
   for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < g.nh; j += (uint64_t)gridDim.x * kBlock) {
     int8_t *__restrict__ own = spins + (2ull * rep + colour) * g.nh;
     const int8_t *__restrict__ oth = spins + (2ull * rep + 1 - colour) * g.nh;
-    const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-    const uint32_t y = row % g.ly, zz = row / g.ly;
+    const SiteCoords sc_ = site_coords(j, g);
+    const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     const uint32_t p = (colour + y + zz) & 1u;
     const Neighbours<ND> n = gather<ND>(oth, g, row, i, y, zz, p);
     const int8_t s = own[j];
@@ -221,8 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_measure(const int8_t *__restrict__ s
   for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < g.nh; j += (uint64_t)gridDim.x * kBlock) {
     const int8_t *own = spins + (2ull * rep + 1) * g.nh;
     const int8_t *oth = spins + (2ull * rep) * g.nh;
-    const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-    const uint32_t y = row % g.ly, zz = row / g.ly;
+    const SiteCoords sc_ = site_coords(j, g);
+    const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     const uint32_t p = (1 + y + zz) & 1u;
     const Neighbours<ND> n = gather<ND>(oth, g, row, i, y, zz, p);
     const int8_t s = own[j], o = oth[j];
@@ -249,8 +249,8 @@ __global__ __launch_bounds__(kBlock) void k_init(int8_t *__restrict__ spins, Ste
   const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint32_t colour = blockIdx.y;
   if (j >= g.nh) return;
-  const uint32_t row = (uint32_t)(j / g.lx2), i = (uint32_t)(j - (uint64_t)row * g.lx2);
-  const uint32_t y = row % g.ly, zz = row / g.ly;
+  const SiteCoords sc_ = site_coords(j, g);
+  const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
   const uint64_t site = 2ull * i + ((colour + y + zz) & 1u) + 2ull * g.lx2 * row;
   int8_t v;
   if (!random)

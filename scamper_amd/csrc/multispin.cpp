@@ -101,6 +101,7 @@ int scamper_multispin_create(const scamper_lattice *lh, double j, uint32_t n_bun
   auto ms = std::unique_ptr<scamper_multispin>(new (std::nothrow) scamper_multispin);
   if (!ms) return fail(SCAMPER_ENOMEM, "multispin handle");
   ms->geom = StencilGeom{l.nd, (uint32_t)(l.dims[0] / 2), (uint32_t)l.dims[1], (uint32_t)l.dims[2], l.n_sites / 2};
+  set_divisors(ms->geom);
   ms->z = 2 * l.nd;
   ms->n_sites = l.n_sites;
   ms->n_bundles = n_bundles;

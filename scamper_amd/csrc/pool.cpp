@@ -515,6 +515,7 @@ int scamper_pool_submit(scamper_pool *P, const scamper_job_spec *spec, uint64_t 
   J->geom.ly = s.n_dims >= 2 ? (uint32_t)s.dims[1] : 1u;
   J->geom.lz = s.n_dims >= 3 ? (uint32_t)s.dims[2] : 1u;
   J->geom.nh = n / 2;
+  set_divisors(J->geom);
   J->n_sites = n;
   J->bytes = (n + kAlign - 1) / kAlign * kAlign;
   if (J->bytes > P->cfg.spin_budget_bytes)
