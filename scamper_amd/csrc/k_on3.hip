@@ -29,6 +29,13 @@ struct V3 {
   float x, y, z;
 };
 
+// sin and cos of 2 pi t for t in [0, 1): the hardware's v_sin_f32 /
+// v_cos_f32 take their argument in revolutions, so no range reduction.
+__device__ __forceinline__ void sincos_turns(float t, float *s, float *c) {
+  *s = __builtin_amdgcn_sinf(t);
+  *c = __builtin_amdgcn_cosf(t);
+}
+
 __device__ __forceinline__ V3 load3(const float *base, uint64_t nh, uint64_t j) {
   return V3{base[j], base[nh + j], base[2 * nh + j]};
 }
@@ -54,8 +61,8 @@ __device__ __forceinline__ V3 random_unit(const U4 &w) {
   const float u3 = fmaxf((float)w.z * two_m32, 1.17549435e-38f), u4 = (float)w.w * two_m32;
   const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
   float s1, c1, s2, c2;
-  sincosf(6.283185307179586f * u2, &s1, &c1);
-  sincosf(6.283185307179586f * u4, &s2, &c2);
+  sincos_turns(u2, &s1, &c1);
+  sincos_turns(u4, &s2, &c2);
   return normalised(V3{r1 * c1, r1 * s1, r2 * c2});
 }
 
@@ -125,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_on3(float *__restrict__ spins, const
         c = fminf(fmaxf(c, -1.0f), 1.0f);
         const float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
         float sphi, cphi;
-        sincosf(6.283185307179586f * ((float)w1 * 2.3283064365386963e-10f), &sphi, &cphi);
+        sincos_turns((float)w1 * 2.3283064365386963e-10f, &sphi, &cphi);
         const float lx = sn * cphi, ly = sn * sphi;
         const float inv_n = 1.0f / norm;
         const float ux = h.x * inv_n, uy = h.y * inv_n, uz = h.z * inv_n;
