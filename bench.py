@@ -16,6 +16,7 @@ Other workloads, one JSON line each (not the headline):
     --workload pt3d   64-replica multi-spin parallel tempering, 3-D L=128
     --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
     --workload serve  the serving block alone
+    --workload calibrate  kernels of known traffic for scaling rocprofv3 counters
 
     python bench.py [--gpus 1] [--steps 2000] [--warmup 200]
 """
@@ -205,13 +206,28 @@ def on3(sc, args) -> dict:
                          "frac": round(gbs / HBM_PEAK_GBS, 4)}}
 
 
+def calibrate(sc, args) -> dict:
+    """Kernels of known HBM traffic on a state larger than the Infinity Cache,
+    for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE: a random start of a
+    32768^2 lattice (k_init writes N = 1 GiB) and one exact recount of its
+    observables (k_measure reads N bytes: each colour once, neighbours
+    through the caches)."""
+    L = 32768
+    lat = sc.Lattice.square(L, L, True)
+    st = sc.State(lat, sc.model(sc.ISING, 1.0), 1, 7)
+    st.init(True)
+    e, m = st.observables()
+    return {"workload": "pmc_calibration", "L": L, "bytes_written_by_k_init": lat.n_sites,
+            "bytes_read_by_k_measure": lat.n_sites, "energy": e, "abs_m": m}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=16384)
-    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "serve"], default="ising2d")
+    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "serve", "calibrate"], default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -227,7 +243,9 @@ def main() -> int:
 
     import scamper_amd as sc
 
-    if args.workload == "pt3d":
+    if args.workload == "calibrate":
+        line = calibrate(sc, args)
+    elif args.workload == "pt3d":
         line = pt3d(sc, args)
     elif args.workload == "on3":
         line = on3(sc, args)
