@@ -7,13 +7,13 @@
 // replica; the reference's MultiSpinIsing (multi_spin.rs:107-151) shares one
 // β over the 64 bits, here each bit's β comes from its temperature label.
 //
-// Per thread: 8 consecutive sites of one colour class.  The anti-aligned
-// neighbour count of every bit is formed bit-sliced (three u64 planes from a
-// full-adder tree over the neighbour XORs); then for each bit the Philox
-// bytes of its draw are compared with the bit's threshold table.  Measured
-// sweeps add the new anti-aligned counts into bit-sliced vertical counters,
-// transpose them across the wavefront (64 lanes x 64 bits) and popcount, so
-// each lane ends with the anti-aligned bond count of one bit.
+// Per thread: 16 consecutive sites of one colour class (one Philox group)
+// and one quarter of the 64 bits.  The anti-aligned neighbour count of every
+// bit is formed bit-sliced (three planes from a full-adder tree over the
+// neighbour XORs); then for each of the thread's 16 bits the Philox bytes of
+// its draw are compared with the bit's threshold table.  Measured sweeps add
+// the new anti-aligned counts into bit-sliced vertical counters and popcount
+// them per bit.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -22,7 +22,6 @@ namespace scamper {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSitesPerThread = 8;
 
 __device__ __forceinline__ uint32_t wrap_dec(uint32_t v, uint32_t n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ uint32_t wrap_inc(uint32_t v, uint32_t n) { return v + 1 == n ? 0 : v + 1; }
@@ -58,39 +57,12 @@ __device__ __forceinline__ Planes count_bits(const uint64_t *x) {
   return p;
 }
 
-// z - count, bit-sliced (count <= z <= 6).
-template <int Z>
-__device__ __forceinline__ Planes complement(const Planes &c) {
-  // subtract (b2 b1 b0) from the constant Z with borrow propagation
-  constexpr uint64_t z0 = (Z & 1) ? ~0ull : 0ull, z1 = (Z & 2) ? ~0ull : 0ull, z2 = (Z & 4) ? ~0ull : 0ull;
-  Planes r;
-  r.b0 = z0 ^ c.b0;
-  uint64_t borrow = ~z0 & c.b0;
-  r.b1 = z1 ^ c.b1 ^ borrow;
-  borrow = (~z1 & c.b1) | (~(z1 ^ c.b1) & borrow);
-  r.b2 = z2 ^ c.b2 ^ borrow;
-  return r;
-}
-
-// Wavefront transpose of a 64x64 bit matrix (lane l holds row l); after it
-// lane l holds column l.  Six butterfly stages with lane exchanges.
-__device__ __forceinline__ uint64_t transpose64(uint64_t x) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                             0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-  for (int st = 0; st < 6; ++st) {
-    const int s = 32 >> st;
-    const uint64_t m = masks[st];
-    const uint64_t y = __shfl_xor(x, s);
-    if (lane & s)
-      x = ((y >> s) & m) | (x & ~m);
-    else
-      x = (x & m) | ((y & m) << s);
-  }
-  return x;
-}
-
+// One thread per (group of 16 consecutive sites, quarter q of the 64 bits):
+// the group's Philox blocks of replica 16q + r hold the 16 sites' draws, so
+// no block is computed twice.  The thread keeps the 16-bit slices (bits
+// 16q .. 16q+15) of the site words and of the bit-sliced anti-aligned
+// counts, two sites per register, and writes only its own 16-bit slice of
+// every site word.
 template <int ND, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ words, const Threshold *__restrict__ thr_by_label,
                                                      const int32_t *__restrict__ labels,
@@ -100,96 +72,118 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
   constexpr int z = 2 * ND;
   const uint32_t bundle = blockIdx.y;
   __shared__ Threshold sthr[64 * (z + 1)];
+  __shared__ uint64_t skey[64];
   __shared__ unsigned long long sred[2 * 64];
   for (uint32_t e = threadIdx.x; e < 64 * (z + 1); e += kBlock) {
     const uint32_t r = e / (z + 1), a = e % (z + 1);
     sthr[e] = thr_by_label[(uint32_t)labels[bundle * 64 + r] * (z + 1) + a];
   }
+  if (threadIdx.x < 64) skey[threadIdx.x] = keys[bundle * 64 + threadIdx.x];
   if (MEAS && threadIdx.x < 128) sred[threadIdx.x] = 0;
   __syncthreads();
 
-  const uint64_t j0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kSitesPerThread;
+  const uint64_t tid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t grp = tid >> 2;
+  const uint32_t q = (uint32_t)(tid & 3u), sh0 = 16u * q;
+  const uint64_t j0 = grp * 16;
   uint64_t *__restrict__ own = words + (2ull * bundle + colour) * g.nh;
   const uint64_t *__restrict__ oth = words + (2ull * bundle + 1 - colour) * g.nh;
-  uint64_t w[kSitesPerThread], cen[kSitesPerThread];
-  Planes pl[kSitesPerThread];
-  bool valid[kSitesPerThread];
+
+  // slices of sites 2i (low half) and 2i+1 (high half)
+  uint32_t w2[8], c2[8], b0[8], b1[8], b2[8];
 #pragma unroll
-  for (int k = 0; k < kSitesPerThread; ++k) {
+  for (int k = 0; k < 8; ++k) w2[k] = c2[k] = b0[k] = b1[k] = b2[k] = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
     const uint64_t j = j0 + k;
-    valid[k] = j < g.nh;
-    w[k] = 0;
-    cen[k] = 0;
-    pl[k] = Planes{0, 0, 0};
-    if (!valid[k]) continue;
+    if (j >= g.nh) continue;
     const SiteCoords sc_ = site_coords(j, g);
     const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     const uint32_t p = (colour + y + zz) & 1u;
     const uint64_t rb = (uint64_t)row * g.lx2;
+    const uint64_t w = own[j];
     uint64_t x[z];
-    w[k] = own[j];
-    const uint64_t nl = oth[rb + (p ? i : wrap_dec(i, g.lx2))];
-    const uint64_t nr = oth[rb + (p ? wrap_inc(i, g.lx2) : i)];
-    cen[k] = oth[j];
-    x[0] = w[k] ^ nl;
-    x[1] = w[k] ^ nr;
+    x[0] = w ^ oth[rb + (p ? i : wrap_dec(i, g.lx2))];
+    x[1] = w ^ oth[rb + (p ? wrap_inc(i, g.lx2) : i)];
     if (ND >= 2) {
-      x[2] = w[k] ^ oth[((uint64_t)(wrap_dec(y, g.ly) + g.ly * zz)) * g.lx2 + i];
-      x[3] = w[k] ^ oth[((uint64_t)(wrap_inc(y, g.ly) + g.ly * zz)) * g.lx2 + i];
+      x[2] = w ^ oth[((uint64_t)(wrap_dec(y, g.ly) + g.ly * zz)) * g.lx2 + i];
+      x[3] = w ^ oth[((uint64_t)(wrap_inc(y, g.ly) + g.ly * zz)) * g.lx2 + i];
     }
     if (ND >= 3) {
-      x[4] = w[k] ^ oth[((uint64_t)(y + g.ly * wrap_dec(zz, g.lz))) * g.lx2 + i];
-      x[5] = w[k] ^ oth[((uint64_t)(y + g.ly * wrap_inc(zz, g.lz))) * g.lx2 + i];
+      x[4] = w ^ oth[((uint64_t)(y + g.ly * wrap_dec(zz, g.lz))) * g.lx2 + i];
+      x[5] = w ^ oth[((uint64_t)(y + g.ly * wrap_inc(zz, g.lz))) * g.lx2 + i];
     }
-    pl[k] = count_bits<z>(x);
+    const Planes pl = count_bits<z>(x);
+    const uint32_t hs = 16u * (uint32_t)(k & 1);
+    w2[k >> 1] |= (uint32_t)((w >> sh0) & 0xFFFFu) << hs;
+    b0[k >> 1] |= (uint32_t)((pl.b0 >> sh0) & 0xFFFFu) << hs;
+    b1[k >> 1] |= (uint32_t)((pl.b1 >> sh0) & 0xFFFFu) << hs;
+    b2[k >> 1] |= (uint32_t)((pl.b2 >> sh0) & 0xFFFFu) << hs;
+    if (MEAS) c2[k >> 1] |= (uint32_t)((oth[j] >> sh0) & 0xFFFFu) << hs;
   }
 
-  uint64_t flip[kSitesPerThread];
+  uint32_t f2[8];
 #pragma unroll
-  for (int k = 0; k < kSitesPerThread; ++k) flip[k] = 0;
-  const uint32_t half = (uint32_t)(j0 >> 3) & 1u;  // which 8 of the block's 16 bytes
-  for (uint32_t r = 0; r < 64; ++r) {
-    const uint64_t key = keys[bundle * 64 + r];
-    const U4 T = byte_block(key, j0 >> 4, sweep, colour, kStreamAccept, 0);
-    const U4 M = byte_block(key, j0 >> 4, sweep, colour, kStreamAccept, 1);
-    const uint64_t top8 = half ? ((uint64_t)T.w << 32 | T.z) : ((uint64_t)T.y << 32 | T.x);
-    const uint64_t mid8 = half ? ((uint64_t)M.w << 32 | M.z) : ((uint64_t)M.y << 32 | M.x);
-    const Threshold *tr = sthr + r * (z + 1);
+  for (int k = 0; k < 8; ++k) f2[k] = 0;
+  if (j0 < g.nh) {
+    for (uint32_t r = 0; r < 16; ++r) {
+      const uint32_t bit = sh0 + r;
+      const uint64_t key = skey[bit];
+      const U4 T = byte_block(key, grp, sweep, colour, kStreamAccept, 0);
+      const U4 M = byte_block(key, grp, sweep, colour, kStreamAccept, 1);
+      const Threshold *tr = sthr + bit * (z + 1);
 #pragma unroll
-    for (int k = 0; k < kSitesPerThread; ++k) {
-      const uint32_t a = (uint32_t)((pl[k].b0 >> r) & 1u) | (uint32_t)((pl[k].b1 >> r) & 1u) << 1 |
-                         (uint32_t)((pl[k].b2 >> r) & 1u) << 2;
-      const Threshold t = tr[a];
-      const uint32_t top = (uint32_t)(top8 >> (8 * k)) & 0xFFu, mid = (uint32_t)(mid8 >> (8 * k)) & 0xFFu;
-      bool acc;
-      if (top != t.k8)
-        acc = top < t.k8;
-      else if (mid != t.m8)
-        acc = mid < t.m8;
-      else
-        acc = valid[k] && site_low(key, j0 + k, sweep, colour, kStreamAccept) < t.k16;
-      flip[k] |= (uint64_t)acc << r;
+      for (int k = 0; k < 16; ++k) {
+        const uint32_t s = r + 16u * (uint32_t)(k & 1);
+        const uint32_t a = ((b0[k >> 1] >> s) & 1u) | (((b1[k >> 1] >> s) & 1u) << 1) | (((b2[k >> 1] >> s) & 1u) << 2);
+        const Threshold t = tr[a];
+        const uint32_t top = (word_of(T, (uint32_t)(k >> 2)) >> (8 * (k & 3))) & 0xFFu;
+        const uint32_t mid = (word_of(M, (uint32_t)(k >> 2)) >> (8 * (k & 3))) & 0xFFu;
+        bool acc;
+        if (top != t.k8)
+          acc = top < t.k8;
+        else if (mid != t.m8)
+          acc = mid < t.m8;
+        else
+          acc = j0 + k < g.nh && site_low(key, j0 + k, sweep, colour, kStreamAccept) < t.k16;
+        f2[k >> 1] |= (uint32_t)acc << s;
+      }
     }
   }
 
-  uint64_t v[6] = {0, 0, 0, 0, 0, 0};  // anti-aligned bond counts, bit-sliced
-  uint64_t up[5] = {0, 0, 0, 0, 0};    // number of +1 spins, bit-sliced
+  uint32_t v[6] = {0, 0, 0, 0, 0, 0};  // anti-aligned bond counts with the new spins, bit-sliced (two sites per lane)
+  uint32_t up[5] = {0, 0, 0, 0, 0};    // +1 spins (own new and other colour), bit-sliced
 #pragma unroll
-  for (int k = 0; k < kSitesPerThread; ++k) {
-    if (!valid[k]) continue;
-    const uint64_t f = flip[k];
-    own[j0 + k] = w[k] ^ f;
-    if (MEAS) {
-      const Planes c = complement<z>(pl[k]);
-      const uint64_t n0 = (f & c.b0) | (~f & pl[k].b0), n1 = (f & c.b1) | (~f & pl[k].b1),
-                     n2 = (f & c.b2) | (~f & pl[k].b2);
-      uint64_t cy = v[0] & n0;
+  for (int k = 0; k < 16; ++k) {
+    const uint64_t j = j0 + k;
+    if (j >= g.nh) continue;
+    const uint32_t hs = 16u * (uint32_t)(k & 1);
+    const uint16_t ns = (uint16_t)(((w2[k >> 1] ^ f2[k >> 1]) >> hs) & 0xFFFFu);
+    reinterpret_cast<uint16_t *>(own + j)[q] = ns;
+  }
+  if (MEAS) {
+    constexpr uint32_t z0 = (z & 1) ? ~0u : 0u, z1 = (z & 2) ? ~0u : 0u, z2 = (z & 4) ? ~0u : 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      // sites that fall past the colour class contribute nothing: their
+      // slices are zero and their planes are masked out
+      const uint32_t valid = (j0 + 2 * k < g.nh ? 0xFFFFu : 0u) | (j0 + 2 * k + 1 < g.nh ? 0xFFFF0000u : 0u);
+      const uint32_t f = f2[k];
+      // z - count where flipped (the anti-aligned count of the new spin)
+      const uint32_t c0 = z0 ^ b0[k];
+      uint32_t borrow = ~z0 & b0[k];
+      const uint32_t c1 = z1 ^ b1[k] ^ borrow;
+      borrow = (~z1 & b1[k]) | (~(z1 ^ b1[k]) & borrow);
+      const uint32_t c2b = z2 ^ b2[k] ^ borrow;
+      const uint32_t n0 = ((f & c0) | (~f & b0[k])) & valid, n1 = ((f & c1) | (~f & b1[k])) & valid,
+                     n2 = ((f & c2b) | (~f & b2[k])) & valid;
+      uint32_t cy = v[0] & n0;
       v[0] ^= n0;
-      uint64_t t1 = v[1] ^ n1 ^ cy;
-      cy = maj(v[1], n1, cy);
+      uint32_t t1 = v[1] ^ n1 ^ cy;
+      cy = (v[1] & n1) | (cy & (v[1] ^ n1));
       v[1] = t1;
       t1 = v[2] ^ n2 ^ cy;
-      cy = maj(v[2], n2, cy);
+      cy = (v[2] & n2) | (cy & (v[2] ^ n2));
       v[2] = t1;
       t1 = v[3] ^ cy;
       cy &= v[3];
@@ -198,14 +192,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
       cy &= v[4];
       v[4] = t1;
       v[5] ^= cy;
-      // +1 spins of this site (new) and of the other colour's site at j
-      const uint64_t s0 = w[k] ^ f, s1 = cen[k];
-      uint64_t c2 = up[0] & s0;
+      const uint32_t s0 = (w2[k] ^ f) & valid, s1 = c2[k] & valid;
+      uint32_t d2 = up[0] & s0;
       up[0] ^= s0;
-      uint64_t c3 = up[0] & s1;
+      uint32_t d3 = up[0] & s1;
       up[0] ^= s1;
-      uint64_t t2 = up[1] ^ c2 ^ c3;
-      uint64_t cc = maj(up[1], c2, c3);
+      uint32_t t2 = up[1] ^ d2 ^ d3;
+      uint32_t cc = (up[1] & d2) | (d3 & (up[1] ^ d2));
       up[1] = t2;
       t2 = up[2] ^ cc;
       cc &= up[2];
@@ -215,19 +208,28 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
       up[3] = t2;
       up[4] ^= cc;
     }
-  }
-  if (MEAS) {
-    // lane l ends with bit l's totals over the wavefront
-    unsigned long long anti = 0, plus = 0;
-#pragma unroll
-    for (int pidx = 0; pidx < 6; ++pidx) anti += (unsigned long long)__popcll(transpose64(v[pidx])) << pidx;
-#pragma unroll
-    for (int pidx = 0; pidx < 5; ++pidx) plus += (unsigned long long)__popcll(transpose64(up[pidx])) << pidx;
+    // per replica of the quarter: sum of both halves, then over the lanes of
+    // the wave with the same quarter, then over the block
     const uint32_t lane = threadIdx.x & 63;
-    atomicAdd(&sred[lane], anti);
-    atomicAdd(&sred[64 + lane], plus);
+    for (uint32_t r = 0; r < 16; ++r) {
+      const uint32_t m = (1u << r) | (1u << (16 + r));
+      unsigned long long anti = 0, plus = 0;
+#pragma unroll
+      for (int b = 0; b < 6; ++b) anti += (unsigned long long)__popc(v[b] & m) << b;
+#pragma unroll
+      for (int b = 0; b < 5; ++b) plus += (unsigned long long)__popc(up[b] & m) << b;
+      for (int off = 4; off < 64; off <<= 1) {
+        anti += __shfl_xor(anti, off);
+        plus += __shfl_xor(plus, off);
+      }
+      if (lane < 4) {
+        atomicAdd(&sred[16 * lane + r], anti);
+        atomicAdd(&sred[64 + 16 * lane + r], plus);
+      }
+    }
     __syncthreads();
-    if (threadIdx.x < 128) atomicAdd(&counts[(uint64_t)bundle * 128 + threadIdx.x], sred[threadIdx.x]);
+    if (threadIdx.x < 128 && sred[threadIdx.x] != 0)
+      atomicAdd(&counts[(uint64_t)bundle * 128 + threadIdx.x], sred[threadIdx.x]);
   }
 }
 
@@ -321,7 +323,7 @@ hipError_t launch_ms_init(uint64_t *words, const uint64_t *keys, const StencilGe
 hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const int32_t *labels, const uint64_t *keys,
                            const StencilGeom &g, uint32_t n_bundles, uint32_t colour, uint64_t sweep,
                            unsigned long long *counts, hipStream_t st) {
-  const uint64_t threads = (g.nh + kSitesPerThread - 1) / kSitesPerThread;
+  const uint64_t threads = ((g.nh + 15) / 16) * 4;  // four quarters per group of 16 sites
   dim3 grid((unsigned)((threads + kBlock - 1) / kBlock), n_bundles);
 #define SC_MS(ND)                                                                                                     \
   if (counts)                                                                                                         \
