@@ -77,7 +77,8 @@ struct scamper_state {
   uint32_t *d_offsets = nullptr, *d_neighbors = nullptr, *d_class_sites = nullptr;
   int8_t *d_weights = nullptr;
   hipStream_t stream = nullptr;
-  int8_t *d_spins = nullptr;
+  int8_t *d_spins = nullptr;       // d_spins_alloc + kSpinSlack
+  int8_t *d_spins_alloc = nullptr;
   bool continuous = false;  // O(3): FP32 component planes in d_vec
   float *d_vec = nullptr;
   ReplicaParams *d_params = nullptr;
@@ -99,7 +100,7 @@ struct scamper_state {
 
   ~scamper_state() {
     if (stream) hipStreamSynchronize(stream);
-    hipFree(d_spins);
+    hipFree(d_spins_alloc);
     hipFree(d_vec);
     hipFree(d_params);
     hipFree(d_hbexp);
@@ -490,7 +491,9 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
     if (hipMalloc(&st->d_vec, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
   } else {
     const size_t bytes = (size_t)n_replicas * l.n_sites;
-    if (hipMalloc(&st->d_spins, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
+    if (hipMalloc(&st->d_spins_alloc, bytes + 2 * kSpinSlack) != hipSuccess)
+      return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
+    st->d_spins = st->d_spins_alloc + kSpinSlack;
   }
   SC_HIP(hipMalloc(&st->d_params, sizeof(ReplicaParams) * n_replicas));
   SC_HIP(hipMalloc(&st->d_em, sizeof(long long) * kEmStride * n_replicas));

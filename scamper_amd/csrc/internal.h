@@ -135,6 +135,12 @@ uint32_t tile_lane_sites(uint64_t total_sites);  // the shape for a launch over 
 inline uint32_t tile_sites(uint32_t lane_sites) { return 256u * lane_sites; }
 constexpr int kStencilTable = 13;  // 2z+1 for z = 6
 
+// Allocated bytes kept before and after every int8 spin buffer the tiled
+// kernel reads: its windows are unaligned 16-byte loads that start up to 16
+// bytes before a colour array and end up to 16 bytes after it (the bytes
+// outside the array are loaded and never used).
+constexpr uint32_t kSpinSlack = 64;
+
 struct alignas(16) TileJob {
   uint64_t key;          // Philox key of the job's replica 0
   int8_t *spins;         // colour 0 at spins, colour 1 at spins + nh

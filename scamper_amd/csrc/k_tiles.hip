@@ -68,96 +68,63 @@ struct TileLds {
 
 __device__ __forceinline__ uint32_t byte4(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
 
-// A 16-byte window: byte k is byte k of (lo, hi) in little-endian order.
+// A 16-byte window as four dwords: byte k is byte (k & 3) of d[k >> 2].
 struct W16 {
-  uint64_t lo, hi;
+  uint32_t d[4];
 };
 
-__device__ __forceinline__ uint32_t word16(const W16 &w, int k) {
-  return (uint32_t)((k < 2 ? w.lo : w.hi) >> (32 * (k & 1)));
+// Unaligned loads and stores through global-memory pointers: gfx950 global
+// memory instructions take any byte address, so a window is one dwordx4 (or
+// dword) access, addressed as a 32-bit offset from a uniform base.  Callers
+// keep every window inside the spin allocation (its slack included).
+typedef __attribute__((address_space(1))) int8_t g8;
+struct __attribute__((packed)) Bytes16 {
+  uint32_t d[4];
+};
+struct __attribute__((packed)) Bytes4 {
+  uint32_t d;
+};
+typedef __attribute__((address_space(1))) Bytes16 gBytes16;
+typedef __attribute__((address_space(1))) Bytes4 gBytes4;
+__device__ __forceinline__ W16 load16(const g8 *p) {
+  const gBytes16 *q = reinterpret_cast<const gBytes16 *>(p);
+  return W16{{q->d[0], q->d[1], q->d[2], q->d[3]}};
 }
-
-// The 16 bytes at p, any alignment: the aligned 16 bytes at p, or the
-// aligned 32 bytes around it (the caller has checked they are allocated).
-__device__ __forceinline__ W16 load16(const int8_t *p) {
-  const uintptr_t a = (uintptr_t)p;
-  const uint64_t *A = reinterpret_cast<const uint64_t *>(a & ~(uintptr_t)15);
-  const uint32_t s = (uint32_t)(a & 15);
-  const uint64_t a0 = A[0], a1 = A[1];
-  if (s == 0) return W16{a0, a1};
-  const uint64_t a2 = A[2], a3 = A[3];
-  const bool q = s >= 8;
-  const uint32_t r = 8 * (s & 7);
-  const uint64_t e0 = q ? a1 : a0, e1 = q ? a2 : a1, e2 = q ? a3 : a2;
-  if (r == 0) return W16{e0, e1};
-  return W16{(e0 >> r) | (e1 << (64 - r)), (e1 >> r) | (e2 << (64 - r))};
-}
-
-// w moved one byte up (byte 0 = e: the left neighbours) or down (byte 15 = e:
-// the right neighbours).
-__device__ __forceinline__ W16 shift_up(const W16 &w, uint32_t e) {
-  return W16{(w.lo << 8) | e, (w.hi << 8) | (w.lo >> 56)};
-}
-__device__ __forceinline__ W16 shift_down(const W16 &w, uint32_t e) {
-  return W16{(w.lo >> 8) | (w.hi << 56), (w.hi >> 8) | ((uint64_t)e << 56)};
-}
-
-__device__ __forceinline__ W16 put_byte16(const W16 &w, uint32_t k, uint32_t v) {
-  const uint32_t sh = 8u * (k & 7u);
-  const uint64_t m = 0xFFull << sh, b = (uint64_t)v << sh;
-  return k < 8 ? W16{(w.lo & ~m) | b, w.hi} : W16{w.lo, (w.hi & ~m) | b};
-}
-
-__device__ __forceinline__ W16 select16(const W16 &a, const W16 &b, const W16 &mask) {  // a where mask
-  return W16{(a.lo & mask.lo) | (b.lo & ~mask.lo), (a.hi & mask.hi) | (b.hi & ~mask.hi)};
-}
-
-__device__ __forceinline__ void store16(int8_t *p, const W16 &w) {
-  const uintptr_t a = (uintptr_t)p;
-  if ((a & 15) == 0) {
-    *reinterpret_cast<uint4 *>(p) = make_uint4((uint32_t)w.lo, (uint32_t)(w.lo >> 32), (uint32_t)w.hi,
-                                               (uint32_t)(w.hi >> 32));
-  } else if ((a & 7) == 0) {
-    reinterpret_cast<uint64_t *>(p)[0] = w.lo;
-    reinterpret_cast<uint64_t *>(p)[1] = w.hi;
-  } else {
+__device__ __forceinline__ void store16(g8 *p, const W16 &w) {
+  gBytes16 *q = reinterpret_cast<gBytes16 *>(p);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) p[k] = (int8_t)((k < 8 ? w.lo : w.hi) >> (8 * (k & 7)));
-  }
+  for (int k = 0; k < 4; ++k) q->d[k] = w.d[k];
 }
+__device__ __forceinline__ uint32_t load4(const g8 *p) { return reinterpret_cast<const gBytes4 *>(p)->d; }
+__device__ __forceinline__ void store4(g8 *p, uint32_t w) { reinterpret_cast<gBytes4 *>(p)->d = w; }
 
-// p lies in [begin, end) and its aligned read span ends by `end` (the span
-// may start before `begin`: the preceding array of the same allocation,
-// never before the allocation).
-__device__ __forceinline__ bool window_ok16(const int8_t *p, const int8_t *begin, const int8_t *end) {
-  return p >= begin && (((uintptr_t)p + 16 + 15) & ~(uintptr_t)15) <= (uintptr_t)end;
-}
-
-// The 4 bytes at p, any alignment: the aligned dword at p, or the two aligned
-// dwords around it (the caller has checked that they lie in the allocation).
-__device__ __forceinline__ uint32_t load4(const int8_t *p) {
-  const uintptr_t a = (uintptr_t)p;
-  const uint32_t *A = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
-  const uint32_t s = (uint32_t)(a & 3);
-  const uint32_t lo = A[0];
-  if (s == 0) return lo;
-  return __builtin_amdgcn_alignbyte(A[1], lo, s);
-}
-
-// p lies in [begin, end) with its aligned read span ending by `end`; the
-// span may start before `begin` (the preceding array of the same
-// allocation), never before the allocation.
-__device__ __forceinline__ bool window_ok4(const int8_t *p, const int8_t *begin, const int8_t *end) {
-  return p >= begin && (((uintptr_t)p + 4 + 3) & ~(uintptr_t)3) <= (uintptr_t)end;
-}
-
-__device__ __forceinline__ void store4(int8_t *p, uint32_t w) {
-  if (((uintptr_t)p & 3) == 0) {
-    *reinterpret_cast<uint32_t *>(p) = w;
-  } else {
+// a where mask, b elsewhere (bitwise).
+__device__ __forceinline__ uint32_t pick(uint32_t a, uint32_t b, uint32_t mask) { return (a & mask) | (b & ~mask); }
+__device__ __forceinline__ W16 pick16(const W16 &a, const W16 &b, const W16 &mask) {
+  W16 r;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) p[k] = (int8_t)byte4(w, k);
+  for (int k = 0; k < 4; ++k) r.d[k] = pick(a.d[k], b.d[k], mask.d[k]);
+  return r;
+}
+
+// 0xff in bytes 0 .. n-1 of a window (1 <= n <= 16).
+__device__ __forceinline__ W16 head_mask16(uint32_t n) {
+  W16 r;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = min(max((int)n - 4 * k, 0), 4);
+    r.d[k] = t == 0 ? 0u : ~0u >> (32 - 8 * t);
   }
+  return r;
+}
+
+// 0xff in byte `pos` of a window; no byte when pos > 15.
+__device__ __forceinline__ W16 byte_mask16(uint32_t pos) {
+  const uint32_t b = 0xFFu << (8u * (pos & 3u)), dw = pos >> 2;
+  W16 r;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.d[k] = dw == (uint32_t)k ? b : 0u;
+  return r;
 }
 
 // Accumulators of a measured half-sweep (per lane).
@@ -253,6 +220,11 @@ __device__ __forceinline__ uint32_t eq_bytes(uint32_t x, uint32_t y) {
   return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & kMsb;
 }
 
+// 0xff in every byte of x whose bit 7 is set, 0 in the others: v_perm's
+// selectors 8 .. 11 replicate bits 15, 31, 47 and 63 of {hi, lo}, which with
+// lo = x and hi = x << 8 are bit 7 of bytes 1, 3, 0 and 2 of x.
+__device__ __forceinline__ uint32_t spread_msb(uint32_t x) { return __builtin_amdgcn_perm(x << 8, x, 0x090B080Au); }
+
 __device__ __forceinline__ uint32_t put_byte(uint32_t w, uint32_t k, uint32_t v) {
   return (w & ~(0xFFu << (8u * k))) | (v << (8u * k));
 }
@@ -262,7 +234,7 @@ __device__ __forceinline__ uint32_t lut16(const uint32_t (&t)[4], uint32_t idx) 
   const uint32_t sel = idx & 0x07070707u;
   const uint32_t lo = __builtin_amdgcn_perm(t[1], t[0], sel);
   const uint32_t hi = __builtin_amdgcn_perm(t[3], t[2], sel);
-  const uint32_t mk = ((idx >> 3) & kOnes) * 0xFFu;
+  const uint32_t mk = spread_msb(idx << 4);  // 0xff where index bit 3 is set (indices < 16)
   return (hi & mk) | (lo & ~mk);
 }
 
@@ -293,9 +265,10 @@ __device__ __forceinline__ uint32_t accept_bytes(uint32_t idx, const LaneDraws &
   return acc;
 }
 
-// 1 in every byte where a and b differ, for byte values below 16 (Potts
-// states): (a ^ b) + 0x7f sets bit 7 of a byte exactly when a ^ b != 0.
-__device__ __forceinline__ uint32_t ne01(uint32_t a, uint32_t b) { return (((a ^ b) + 0x7F7F7F7Fu) >> 7) & kOnes; }
+// 16 in every byte where a and b differ, for byte values below 16 (Potts
+// states): (a ^ b) + 15 reaches 16 exactly when a ^ b != 0, and no byte
+// carries into the next.
+__device__ __forceinline__ uint32_t ne16(uint32_t a, uint32_t b) { return ((a ^ b) + 0x0F0F0F0Fu) & 0x10101010u; }
 
 template <int ND>
 __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
@@ -306,8 +279,7 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2
 #pragma unroll
   for (int c = 0; c < z; ++c) a2 += (S ^ N[c]) & 0x02020202u;
   const uint32_t anti = a2 >> 1;
-  const uint32_t m = accept_bytes<true>(anti, D, J, L, key, j0, sweep, colour) >> 7;
-  const uint32_t NS = S ^ (((m << 8) - m) & 0xFEFEFEFEu);
+  const uint32_t NS = S ^ (spread_msb(accept_bytes<true>(anti, D, J, L, key, j0, sweep, colour)) & 0xFEFEFEFEu);
   if (meas) {
     uint32_t b2 = 0;
 #pragma unroll
@@ -324,41 +296,55 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
                                                uint64_t key, uint32_t j0, uint64_t sweep, uint32_t colour, bool meas,
                                                Meas &M) {
   constexpr int z = 2 * ND;
-  uint32_t R;
+  uint32_t R, idx;  // proposed states; table index d + z = z + #(n != s) - #(n != r) per byte, 0 .. 2z
   if (q == 2) {
+    // r is the other state: #(n != r) = z - #(n != s), index 2 #(n != s)
     R = S ^ kOnes;
-  } else {
-    // floor(top * m / 256) per byte from 16-bit lane products (top * m <
-    // 4096); a byte whose bucket straddles an index boundary (low byte of
-    // the product + m > 256) is settled with the finer bytes.
-    const uint32_t m = q - 1;
-    const uint32_t pe = (D.ptop & 0x00FF00FFu) * m, po = ((D.ptop >> 8) & 0x00FF00FFu) * m;
-    uint32_t idx = ((pe >> 8) & 0x000F000Fu) | (po & 0x0F000F00u);
-    const uint32_t bias = (m + 255u) * 0x00010001u;
-    uint32_t strad = ((((pe & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) |
-                     (((((po & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) << 8);
-    while (strad) {
-      const uint32_t k = (uint32_t)__builtin_ctz(strad) >> 3;
-      strad &= strad - 1;
-      idx = put_byte(idx, k, propose2(byte4(D.ptop, (int)k), byte4(D.pmid, (int)k), m, key, j0 + k, sweep, colour));
-    }
-    R = idx + ((((idx | kMsb) - S) & kMsb) >> 7);  // skip the current state
-  }
-  // d + z = z + #(n != s) - #(n != r) per byte, 0 .. 2z
-  uint32_t ns_ = (uint32_t)z * kOnes, nr = 0;
+    uint32_t ds = 0;  // 16 #(n != s) per byte
 #pragma unroll
-  for (int c = 0; c < z; ++c) {
-    ns_ += ne01(N[c], S);
-    nr += ne01(N[c], R);
+    for (int c = 0; c < z; ++c) ds += ne16(N[c], S);
+    idx = ds >> 3;  // bytes of ds are multiples of 16: nothing crosses a byte
+  } else {
+    const uint32_t m = q - 1;
+    uint32_t a;  // floor(u * m) per byte, u the proposal draw
+    if ((m & (m - 1)) == 0) {
+      // m = 2^k: the top k bits of the top byte, never a straddled bucket
+      const uint32_t k = 31u - (uint32_t)__builtin_clz(m);
+      a = (D.ptop >> (8u - k)) & ((m - 1u) * kOnes);
+    } else {
+      // floor(top * m / 256) per byte from 16-bit lane products (top * m <
+      // 4096); a byte whose bucket straddles an index boundary (low byte of
+      // the product + m > 256) is settled with the finer bytes.
+      const uint32_t m8 = m & 0xFFu;  // m < 16: both factors below 2^24 (v_mul_u32_u24)
+      const uint32_t pe = (D.ptop & 0x00FF00FFu) * m8, po = ((D.ptop >> 8) & 0x00FF00FFu) * m8;
+      a = ((pe >> 8) & 0x000F000Fu) | (po & 0x0F000F00u);
+      const uint32_t bias = (m + 255u) * 0x00010001u;
+      uint32_t strad = ((((pe & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) |
+                       (((((po & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) << 8);
+      while (strad) {
+        const uint32_t k = (uint32_t)__builtin_ctz(strad) >> 3;
+        strad &= strad - 1;
+        a = put_byte(a, k, propose2(byte4(D.ptop, (int)k), byte4(D.pmid, (int)k), m, key, j0 + k, sweep, colour));
+      }
+    }
+    R = a + ((((a | kMsb) - S) & kMsb) >> 7);  // skip the current state
+    uint32_t ds = 0, dr = 0;  // 16 #(n != s), 16 #(n != r) per byte
+#pragma unroll
+    for (int c = 0; c < z; ++c) {
+      ds += ne16(N[c], S);
+      dr += ne16(N[c], R);
+    }
+    // 16 z + ds - dr lies in [0, 32 z] per byte, a multiple of 16
+    idx = ((uint32_t)(16 * z) * kOnes + ds - dr) >> 4;
   }
-  const uint32_t idx = ns_ - nr;
-  const uint32_t am = (accept_bytes<false>(idx, D, J, L, key, j0, sweep, colour) >> 7) * 0xFFu;
-  const uint32_t NS = (R & am) | (S & ~am);
+  const uint32_t acc = accept_bytes<ND == 1>(idx, D, J, L, key, j0, sweep, colour);
+  const uint32_t am = spread_msb(acc);  // 0xff in accepted bytes
+  const uint32_t NS = pick(R, S, am);
   if (meas) {
     uint32_t differ = 0;
 #pragma unroll
-    for (int c = 0; c < z; ++c) differ += ne01(N[c], NS);
-    M.e += 4 * z - (int)__builtin_amdgcn_sad_u8(differ, 0u, 0u);
+    for (int c = 0; c < z; ++c) differ += ne16(N[c], NS);
+    M.e += 4 * z - (int)(__builtin_amdgcn_sad_u8(differ, 0u, 0u) >> 4);
     for (uint32_t v = 0; v < q; ++v) {
       const uint32_t vv = v * kOnes;
       const uint64_t n = __builtin_amdgcn_sad_u8(eq_bytes(NS, vv) >> 7, 0u, 0u) +
@@ -368,6 +354,59 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
   }
   return NS;
 }
+
+// Where a lane's W consecutive sites (W = 4 or 16, rows of >= W sites) and
+// their neighbours lie.  Sites 0 .. ks-1 are in row a = (y, zz) from column
+// i0, sites ks .. W-1 (when two_rows) at the start of row b, the next row.
+// Row parities pa, pb = (colour + y + z) & 1 differ unless row b starts a
+// new plane (y wraps: the parity changes by ly - 2).  Neighbour windows are loaded
+// at positions measured from obase = other colour - kSpinSlack in 32-bit
+// arithmetic (colour classes have < 2^31 sites, every window starts at
+// rank >= -16).
+template <int ND, uint32_t W>
+struct LaneSpan {
+  uint32_t i0, ks, pa, pb, left;
+  bool two_rows;
+  uint32_t off_a[2 * ND - 2], off_b[2 * ND - 2];  // U, D (, B, F) offsets of rows a and b, modulo 2^32
+  uint32_t pos_l, pos_r;                          // byte whose left / right neighbour wraps (W = none)
+
+  __device__ __forceinline__ LaneSpan(const TileJob &J, uint32_t j0, uint32_t colour) {
+    const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
+    const uint32_t row = div_by(j0, J.div_lx2);
+    i0 = j0 - row * lx2;
+    const uint32_t zz = ND == 3 ? div_by(row, J.div_ly) : 0u;
+    const uint32_t y = ND == 1 ? 0u : row - zz * ly;
+    left = lx2 - i0;
+    two_rows = left < W;
+    ks = two_rows ? left : W;
+    pa = (colour + y + zz) & 1u;
+    const uint32_t yb = ND == 1 ? 0u : (y + 1u == ly ? 0u : y + 1u);
+    const uint32_t zb = ND == 3 ? (y + 1u == ly ? zz + 1u : zz) : 0u;
+    pb = (colour + yb + zb) & 1u;
+    if (ND >= 2) {
+      const uint32_t wrap = (ly - 1u) * lx2;
+      off_a[0] = y == 0u ? wrap : 0u - lx2;
+      off_a[1] = y + 1u == ly ? 0u - wrap : lx2;
+      off_b[0] = yb == 0u ? wrap : 0u - lx2;
+      off_b[1] = yb + 1u == ly ? 0u - wrap : lx2;
+      if (ND == 3) {
+        const uint32_t plane = lx2 * ly, wrapz = (lz - 1u) * plane;
+        off_a[2] = zz == 0u ? wrapz : 0u - plane;
+        off_a[3] = zz + 1u == lz ? 0u - wrapz : plane;
+        off_b[2] = zb == 0u ? wrapz : 0u - plane;
+        off_b[3] = zb + 1u == lz ? 0u - wrapz : plane;
+      }
+    }
+    // the left neighbour of a parity-0 site at i = 0 is the row's last site
+    // (byte 0 when the lane starts a row, byte ks when row b starts inside);
+    // the right neighbour of a parity-1 site at i = lx2-1 is the row's first
+    pos_l = (i0 == 0u && !pa) ? 0u : (two_rows && !pb) ? ks : W;
+    pos_r = (left <= W && pa) ? ks - 1u : W;
+  }
+  // window start (relative to obase) of neighbour c for rows a and b
+  __device__ __forceinline__ uint32_t at_a(uint32_t js, int c) const { return js + off_a[c]; }
+  __device__ __forceinline__ uint32_t at_b(uint32_t js, int c) const { return js + (two_rows ? off_b[c] : off_a[c]); }
+};
 
 // One tile of 1024 sites, 4 per lane (quad-shared Philox group); measured
 // counters are added to the lane's M.
@@ -394,90 +433,40 @@ __device__ __forceinline__ void run_tile4(TileLds &L, const TileJob &J, uint32_t
     int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
     const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * nh;
     const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
-    const uint32_t row = div_by(j0 < nh ? j0 : 0u, J.div_lx2);
-    const uint32_t i0 = (j0 < nh ? j0 : 0u) - row * lx2;
-    const uint32_t zz = ND == 3 ? div_by(row, J.div_ly) : 0u;
-    const uint32_t y = ND == 1 ? 0u : row - zz * ly;
-    // sites 0 .. ks-1 of the lane lie in row a = `row`, sites ks .. 3 in
-    // row b = row + 1 (rows of >= 4 sites on the window path)
-    const uint32_t ks = lx2 - i0 < 4u ? lx2 - i0 : 4u;
-    const bool two_rows = ks < 4u;
-    const uint32_t yb = ND == 1 ? 0u : (y + 1 == ly ? 0u : y + 1);
-    const uint32_t zb = ND == 3 ? (y + 1 == ly ? zz + 1 : zz) : 0u;
-    const uint32_t pa = (colour + y + zz) & 1u, pb = (colour + yb + zb) & 1u;
-    const uint32_t rba = row * lx2, rbb = rba + lx2;
-    const int64_t plane = (int64_t)lx2 * ly;
-    const int64_t upa = ND >= 2 ? (y == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
-    const int64_t dna = ND >= 2 ? (y + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
-    const int64_t bka = ND >= 3 ? (zz == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
-    const int64_t fra = ND >= 3 ? (zz + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
-    const int64_t upb = ND >= 2 ? (yb == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
-    const int64_t dnb = ND >= 2 ? (yb + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
-    const int64_t bkb = ND >= 3 ? (zb == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
-    const int64_t frb = ND >= 3 ? (zb + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
-    const int8_t *own_end = own + nh, *oth_end = oth + nh;
-    bool vec = lx2 >= 4u && j0 + 4u <= nh && window_ok4(own + j0, own, own_end) &&
-               window_ok4(oth + j0, oth, oth_end);
-    if (ND >= 2) {
-      vec = vec && window_ok4(oth + j0 + upa, oth, oth_end) && window_ok4(oth + j0 + dna, oth, oth_end);
-      if (two_rows) vec = vec && window_ok4(oth + j0 + upb, oth, oth_end) && window_ok4(oth + j0 + dnb, oth, oth_end);
-    }
-    if (ND >= 3) {
-      vec = vec && window_ok4(oth + j0 + bka, oth, oth_end) && window_ok4(oth + j0 + fra, oth, oth_end);
-      if (two_rows) vec = vec && window_ok4(oth + j0 + bkb, oth, oth_end) && window_ok4(oth + j0 + frb, oth, oth_end);
-    }
+    const bool vec = lx2 >= 4u && j0 + 4u <= nh;
+    g8 *const gown = (g8 *)own;
+    const g8 *const obase = (const g8 *)oth - kSpinSlack;
+    const uint32_t js = j0 + kSpinSlack;
+    const LaneSpan<ND, 4> sp(J, vec ? j0 : 0u, colour);
 
     // windows first, so that the loads are in flight during the Philox rounds
-    uint32_t O = 0, C = 0, Ua = 0, Da = 0, Ba = 0, Fa = 0, Ub = 0, Db = 0, Bb = 0, Fb = 0, el = 0, er = 0;
-    int8_t wrap_r = 0, wrap_l = 0;
+    uint32_t O = 0, C = 0, Lm = 0, Rp = 0, Lx = 0, Rx = 0, Na[z - 2 > 0 ? z - 2 : 1], Nb[z - 2 > 0 ? z - 2 : 1];
     if (vec) {
-      O = load4(own + j0);
-      C = load4(oth + j0);
-      // left neighbour of site 0 when pa = 0; right neighbour of site 3 when
-      // its row's parity is 1 (site 3 is never at the end of row b)
-      el = (uint8_t)oth[pa ? j0 : (i0 == 0 ? rba + lx2 - 1u : j0 - 1u)];
-      const uint32_t last_p = two_rows ? pb : pa;
-      const uint32_t er_idx = (!two_rows && i0 + 4u == lx2) ? rba : j0 + 4u;
-      er = (uint8_t)oth[last_p ? er_idx : j0];
-      wrap_r = oth[rba];                             // right of site ks-1 (i = lx2-1) when pa = 1
-      wrap_l = two_rows ? oth[rbb + lx2 - 1u] : 0;  // left of site ks (i = 0) when pb = 0
-      if (ND >= 2) {
-        Ua = load4(oth + j0 + upa);
-        Da = load4(oth + j0 + dna);
-        Ub = two_rows && upb != upa ? load4(oth + j0 + upb) : Ua;
-        Db = two_rows && dnb != dna ? load4(oth + j0 + dnb) : Da;
-      }
-      if (ND >= 3) {
-        Ba = load4(oth + j0 + bka);
-        Fa = load4(oth + j0 + fra);
-        Bb = two_rows && bkb != bka ? load4(oth + j0 + bkb) : Ba;
-        Fb = two_rows && frb != fra ? load4(oth + j0 + frb) : Fa;
+      O = load4(gown + j0);
+      C = load4(obase + js);
+      Lm = load4(obase + (js - 1u));
+      Rp = load4(obase + (js + 1u));
+      Lx = load4(obase + (sp.pos_l < 4u ? js - 1u + lx2 : js));
+      Rx = load4(obase + (sp.pos_r < 4u ? js + 1u - lx2 : js));
+#pragma unroll
+      for (int c = 0; c < z - 2; ++c) {
+        Na[c] = load4(obase + sp.at_a(js, c));
+        Nb[c] = load4(obase + sp.at_b(js, c));
       }
     }
     const LaneDraws D = quad_draws(key, g, sweep, colour, need_prop, need_pmid);
 
     if (vec) {
-      const uint32_t Su = (C << 8) | el;          // Su byte k = C byte k-1
-      const uint32_t Sd = (C >> 8) | (er << 24);  // Sd byte k = C byte k+1
-      // bytes of row a, and bytes whose row parity is 1
-      const uint32_t amask = two_rows ? ((1u << (8u * ks)) - 1u) : 0xFFFFFFFFu;
-      const uint32_t pmask = (pa ? amask : 0u) | (pb ? ~amask : 0u);
+      const uint32_t amask = sp.two_rows ? (1u << (8u * sp.ks)) - 1u : ~0u;  // bytes of row a
+      const uint32_t pmask = (sp.pa ? amask : 0u) | (sp.pb ? ~amask : 0u);  // bytes of parity 1
       uint32_t N[z];
-      N[0] = (C & pmask) | (Su & ~pmask);
-      N[1] = (Sd & pmask) | (C & ~pmask);
-      if (two_rows && pa) N[1] = put_byte(N[1], ks - 1u, (uint8_t)wrap_r);
-      if (two_rows && !pb) N[0] = put_byte(N[0], ks, (uint8_t)wrap_l);
-      if (ND >= 2) {
-        N[2] = (Ua & amask) | (Ub & ~amask);
-        N[3] = (Da & amask) | (Db & ~amask);
-      }
-      if (ND >= 3) {
-        N[4] = (Ba & amask) | (Bb & ~amask);
-        N[5] = (Fa & amask) | (Fb & ~amask);
-      }
+      N[0] = pick(Lx, pick(C, Lm, pmask), sp.pos_l < 4u ? 0xFFu << (8u * sp.pos_l) : 0u);
+      N[1] = pick(Rx, pick(Rp, C, pmask), sp.pos_r < 4u ? 0xFFu << (8u * sp.pos_r) : 0u);
+#pragma unroll
+      for (int c = 0; c < z - 2; ++c) N[2 + c] = pick(Na[c], Nb[c], amask);
       const uint32_t NS = KIND == kIsing ? ising_word<ND>(O, N, C, D, J, L, key, j0, sweep, colour, meas, M)
                                          : potts_word<ND>(O, N, C, D, J, L, q, key, j0, sweep, colour, meas, M);
-      if (NS != O) store4(own + j0, NS);
+      if (NS != O) store4(gown + j0, NS);
     } else {
 #pragma unroll
       for (uint32_t k = 0; k < 4u; ++k) {  // unrolled: draw bytes are picked by constant k
@@ -533,66 +522,23 @@ __device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_
     int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
     const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * nh;
     const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
-    const uint32_t row = div_by(j0, J.div_lx2);
-    const uint32_t i0 = j0 - row * lx2;
-    const uint32_t zz = ND == 3 ? div_by(row, J.div_ly) : 0u;
-    const uint32_t y = ND == 1 ? 0u : row - zz * ly;
-    // sites 0 .. ks-1 of the lane lie in row a = `row`, sites ks .. 15 in
-    // row b = row + 1 (rows of >= 16 sites on the window path)
-    const uint32_t ks = lx2 - i0 < 16u ? lx2 - i0 : 16u;
-    const bool two_rows = ks < 16u;
-    const uint32_t yb = ND == 1 ? 0u : (y + 1 == ly ? 0u : y + 1);
-    const uint32_t zb = ND == 3 ? (y + 1 == ly ? zz + 1 : zz) : 0u;
-    const uint32_t pa = (colour + y + zz) & 1u, pb = (colour + yb + zb) & 1u;
-    const uint32_t rba = row * lx2, rbb = rba + lx2;
-    const int64_t plane = (int64_t)lx2 * ly;
-    const int64_t upa = ND >= 2 ? (y == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
-    const int64_t dna = ND >= 2 ? (y + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
-    const int64_t bka = ND >= 3 ? (zz == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
-    const int64_t fra = ND >= 3 ? (zz + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
-    const int64_t upb = ND >= 2 ? (yb == 0 ? (int64_t)(ly - 1) * lx2 : -(int64_t)lx2) : 0;
-    const int64_t dnb = ND >= 2 ? (yb + 1 == ly ? -(int64_t)(ly - 1) * lx2 : (int64_t)lx2) : 0;
-    const int64_t bkb = ND >= 3 ? (zb == 0 ? (int64_t)(lz - 1) * plane : -plane) : 0;
-    const int64_t frb = ND >= 3 ? (zb + 1 == lz ? -(int64_t)(lz - 1) * plane : plane) : 0;
-    const int8_t *own_end = own + nh, *oth_end = oth + nh;
-    bool vec = lx2 >= 16u && j0 + 16u <= nh && window_ok16(own + j0, own, own_end) &&
-               window_ok16(oth + j0, oth, oth_end);
-    if (ND >= 2) {
-      vec = vec && window_ok16(oth + j0 + upa, oth, oth_end) && window_ok16(oth + j0 + dna, oth, oth_end);
-      if (two_rows) vec = vec && window_ok16(oth + j0 + upb, oth, oth_end) && window_ok16(oth + j0 + dnb, oth, oth_end);
-    }
-    if (ND >= 3) {
-      vec = vec && window_ok16(oth + j0 + bka, oth, oth_end) && window_ok16(oth + j0 + fra, oth, oth_end);
-      if (two_rows) vec = vec && window_ok16(oth + j0 + bkb, oth, oth_end) && window_ok16(oth + j0 + frb, oth, oth_end);
-    }
+    const bool vec = lx2 >= 16u && j0 + 16u <= nh;
+    g8 *const gown = (g8 *)own;
+    const g8 *const obase = (const g8 *)oth - kSpinSlack;
+    const uint32_t js = j0 + kSpinSlack;
+    const LaneSpan<ND, 16> sp(J, vec ? j0 : 0u, colour);
 
     // windows first, so that the loads are in flight during the Philox rounds
-    W16 O{0, 0}, C{0, 0}, Ua{0, 0}, Da{0, 0}, Ba{0, 0}, Fa{0, 0}, Ub{0, 0}, Db{0, 0}, Bb{0, 0}, Fb{0, 0};
-    uint32_t el = 0, er = 0;
-    int8_t wrap_r = 0, wrap_l = 0;
+    W16 O{}, C{}, Lm{}, Rp{}, Lx{}, Rx{}, Na[z - 2 > 0 ? z - 2 : 1];
     if (vec) {
-      O = load16(own + j0);
-      C = load16(oth + j0);
-      // left neighbour of site 0 when pa = 0; right neighbour of site 15
-      // when its row's parity is 1 (site 15 is never at the end of row b)
-      el = (uint8_t)oth[pa ? j0 : (i0 == 0 ? rba + lx2 - 1u : j0 - 1u)];
-      const uint32_t last_p = two_rows ? pb : pa;
-      const uint32_t er_idx = (!two_rows && i0 + 16u == lx2) ? rba : j0 + 16u;
-      er = (uint8_t)oth[last_p ? er_idx : j0];
-      wrap_r = oth[rba];                             // right of site ks-1 (i = lx2-1) when pa = 1
-      wrap_l = two_rows ? oth[rbb + lx2 - 1u] : 0;  // left of site ks (i = 0) when pb = 0
-      if (ND >= 2) {
-        Ua = load16(oth + j0 + upa);
-        Da = load16(oth + j0 + dna);
-        Ub = two_rows && upb != upa ? load16(oth + j0 + upb) : Ua;
-        Db = two_rows && dnb != dna ? load16(oth + j0 + dnb) : Da;
-      }
-      if (ND >= 3) {
-        Ba = load16(oth + j0 + bka);
-        Fa = load16(oth + j0 + fra);
-        Bb = two_rows && bkb != bka ? load16(oth + j0 + bkb) : Ba;
-        Fb = two_rows && frb != fra ? load16(oth + j0 + frb) : Fa;
-      }
+      O = load16(gown + j0);
+      C = load16(obase + js);
+      Lm = load16(obase + (js - 1u));
+      Rp = load16(obase + (js + 1u));
+      Lx = load16(obase + (sp.pos_l < 16u ? js - 1u + lx2 : js));
+      Rx = load16(obase + (sp.pos_r < 16u ? js + 1u - lx2 : js));
+#pragma unroll
+      for (int c = 0; c < z - 2; ++c) Na[c] = load16(obase + sp.at_a(js, c));
     }
     const uint32_t g = j0 >> 4;
     const U4 T = byte_block(key, g, sweep, colour, kStreamAccept, 0);
@@ -601,46 +547,42 @@ __device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_
     const U4 PM = need_pmid ? byte_block(key, g, sweep, colour, kStreamPropose, 1) : U4{0, 0, 0, 0};
 
     if (vec) {
-      // bytes of row a, and bytes whose row parity is 1
-      const W16 amask = two_rows ? W16{ks >= 8 ? ~0ull : ((1ull << (8u * ks)) - 1u),
-                                       ks > 8 ? ((1ull << (8u * (ks - 8))) - 1u) : 0ull}
-                                 : W16{~0ull, ~0ull};
-      const W16 pmask{(pa ? amask.lo : 0ull) | (pb ? ~amask.lo : 0ull), (pa ? amask.hi : 0ull) | (pb ? ~amask.hi : 0ull)};
-      const W16 Su = shift_up(C, el), Sd = shift_down(C, er);
-      W16 Lw = select16(C, Su, pmask), Rw = select16(Sd, C, pmask);
-      if (two_rows && pa) Rw = put_byte16(Rw, ks - 1u, (uint8_t)wrap_r);
-      if (two_rows && !pb) Lw = put_byte16(Lw, ks, (uint8_t)wrap_l);
-      W16 Uw{0, 0}, Dw{0, 0}, Bw{0, 0}, Fw{0, 0};
-      if (ND >= 2) {
-        Uw = select16(Ua, Ub, amask);
-        Dw = select16(Da, Db, amask);
+      const W16 amask = head_mask16(sp.ks);  // bytes of row a
+      W16 pmask;                             // bytes of parity 1
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pmask.d[k] = (sp.pa ? amask.d[k] : 0u) | (sp.pb ? ~amask.d[k] : 0u);
+      const W16 Lw = pick16(Lx, pick16(C, Lm, pmask), byte_mask16(sp.pos_l));
+      const W16 Rw = pick16(Rx, pick16(Rp, C, pmask), byte_mask16(sp.pos_r));
+      // rows a and b take different U/D/B/F offsets only next to a periodic
+      // wrap: only those lanes load a second window
+      W16 Nw[z - 2 > 0 ? z - 2 : 1];
+      bool split = false;
+#pragma unroll
+      for (int c = 0; c < z - 2; ++c) {
+        Nw[c] = Na[c];
+        split |= sp.two_rows && sp.off_b[c] != sp.off_a[c];
       }
-      if (ND >= 3) {
-        Bw = select16(Ba, Bb, amask);
-        Fw = select16(Fa, Fb, amask);
+      if (split) {
+#pragma unroll
+        for (int c = 0; c < z - 2; ++c) Nw[c] = pick16(Na[c], load16(obase + sp.at_b(js, c)), amask);
       }
-      uint32_t NSw[4];
+      W16 NS;
+      bool changed = false;
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
         uint32_t N[z];
-        N[0] = word16(Lw, w);
-        N[1] = word16(Rw, w);
-        if (ND >= 2) {
-          N[2] = word16(Uw, w);
-          N[3] = word16(Dw, w);
-        }
-        if (ND >= 3) {
-          N[4] = word16(Bw, w);
-          N[5] = word16(Fw, w);
-        }
+        N[0] = Lw.d[w];
+        N[1] = Rw.d[w];
+#pragma unroll
+        for (int c = 0; c < z - 2; ++c) N[2 + c] = Nw[c].d[w];
         const LaneDraws D{word_of(T, (uint32_t)w), word_of(Mb, (uint32_t)w), word_of(PT, (uint32_t)w),
                           word_of(PM, (uint32_t)w)};
-        const uint32_t S = word16(O, w), Cw = word16(C, w), jw = j0 + 4u * (uint32_t)w;
-        NSw[w] = KIND == kIsing ? ising_word<ND>(S, N, Cw, D, J, L, key, jw, sweep, colour, meas, M)
-                                : potts_word<ND>(S, N, Cw, D, J, L, q, key, jw, sweep, colour, meas, M);
+        const uint32_t S = O.d[w], jw = j0 + 4u * (uint32_t)w;
+        NS.d[w] = KIND == kIsing ? ising_word<ND>(S, N, C.d[w], D, J, L, key, jw, sweep, colour, meas, M)
+                                 : potts_word<ND>(S, N, C.d[w], D, J, L, q, key, jw, sweep, colour, meas, M);
+        changed |= NS.d[w] != S;
       }
-      const W16 NS{(uint64_t)NSw[0] | ((uint64_t)NSw[1] << 32), (uint64_t)NSw[2] | ((uint64_t)NSw[3] << 32)};
-      if (NS.lo != O.lo || NS.hi != O.hi) store16(own + j0, NS);
+      if (changed) store16(gown + j0, NS);
     } else {
 #pragma unroll
       for (uint32_t k = 0; k < 16u; ++k) {  // unrolled: draw bytes are picked by constant k

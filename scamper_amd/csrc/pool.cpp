@@ -114,7 +114,8 @@ struct scamper_pool {
   scamper_pool_config cfg{};
   hipStream_t stream = nullptr;
   hipStream_t cap_stream = nullptr;  // graph capture only; graphs run on `stream`
-  int8_t *spins = nullptr;
+  int8_t *spins = nullptr;        // spins_alloc + kSpinSlack; jobs at spins + offset
+  int8_t *spins_alloc = nullptr;
   Arena arena;
   uint64_t in_use = 0;
   size_t table_bytes = 0;
@@ -157,7 +158,7 @@ struct scamper_pool {
       if (ev_graph[p]) hipEventDestroy(ev_graph[p]);
       if (ev_end[p]) hipEventDestroy(ev_end[p]);
     }
-    hipFree(spins);
+    hipFree(spins_alloc);
     if (cap_stream) hipStreamDestroy(cap_stream);
     if (stream) hipStreamDestroy(stream);
   }
@@ -466,8 +467,9 @@ int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out) {
   P->grid = c.grid;
   SC_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
   SC_HIP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
-  if (hipMalloc(&P->spins, c.spin_budget_bytes) != hipSuccess)
+  if (hipMalloc(&P->spins_alloc, c.spin_budget_bytes + 2 * kSpinSlack) != hipSuccess)
     return fail(SCAMPER_ENOMEM, "spin budget of %llu bytes", (unsigned long long)c.spin_budget_bytes);
+  P->spins = P->spins_alloc + kSpinSlack;
   P->arena.reset(c.spin_budget_bytes);
   P->table_bytes = sizeof(TileHeader) + (size_t)c.max_jobs * sizeof(TileJob);
   const size_t ring = (size_t)c.max_jobs * c.chunk_sweeps * kSubs * kRec * sizeof(long long);

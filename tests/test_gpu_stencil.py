@@ -61,6 +61,11 @@ CASES = [
     ((4098,), 0, 2, 0, 0.7, 5),
     ((96, 40), 1, 5, 0, 1.2, 5),
     ((40, 40, 40), 1, 4, 0, 0.62863, 4),
+    # lanes that cross from row (ly-1, z) into row (0, z+1), where both rows
+    # have the same parity (Lx/2 * Ly not a multiple of 4)
+    ((10, 6, 4), 1, 3, 0, 0.55, 5),
+    ((18, 10, 6), 0, 2, 0, 0.25, 4),
+    ((26, 14, 6), 1, 4, 0, 0.62863, 4),
     # 16 sites per lane (launches over >= 4 Mi sites per colour)
     ((4100, 4100), 1, 3, 0, 1.0, 3),
     ((204, 206, 400), 1, 4, 0, 0.62863, 3),
