@@ -133,6 +133,23 @@ struct Meas {
   uint64_t cnt_lo = 0, cnt_hi = 0;  // byte v of (lo, hi): sites in Potts state v (< 16)
 };
 
+// A tile's counts of Potts states 0 .. 3 (q <= 4), per byte position: byte b
+// of n01 is (#state 0) + 16 (#state 1), of n23 (#state 2) + 16 (#state 3);
+// a word adds at most 2 per byte, a tile at most 8.
+struct Nibbles {
+  uint32_t n01 = 0, n23 = 0;
+};
+
+// Add a tile's nibble counts to the lane's byte counters of states 0 .. 3.
+__device__ __forceinline__ void fold_nibbles(Meas &M, const Nibbles &B) {
+  const uint32_t lo = 0x0F0F0F0Fu;
+  const uint32_t packed = __builtin_amdgcn_sad_u8(B.n01 & lo, 0u, 0u) |
+                          (__builtin_amdgcn_sad_u8((B.n01 >> 4) & lo, 0u, 0u) << 8) |
+                          (__builtin_amdgcn_sad_u8(B.n23 & lo, 0u, 0u) << 16) |
+                          (__builtin_amdgcn_sad_u8((B.n23 >> 4) & lo, 0u, 0u) << 24);
+  M.cnt_lo += packed;  // every byte stays below 256 between flushes: no carries
+}
+
 // One site: new spin from its own spin, its z neighbours and its draws.
 template <int ND, int KIND>
 __device__ __forceinline__ int8_t update_site(int8_t s, const int8_t (&n)[2 * ND], uint32_t top, uint32_t mid,
@@ -279,11 +296,11 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2
 #pragma unroll
   for (int c = 0; c < z; ++c) a2 += (S ^ N[c]) & 0x02020202u;
   const uint32_t anti = a2 >> 1;
-  const uint32_t NS = S ^ (spread_msb(accept_bytes<true>(anti, D, J, L, key, j0, sweep, colour)) & 0xFEFEFEFEu);
+  const uint32_t flip = spread_msb(accept_bytes<true>(anti, D, J, L, key, j0, sweep, colour));
+  const uint32_t NS = S ^ (flip & 0xFEFEFEFEu);
   if (meas) {
-    uint32_t b2 = 0;
-#pragma unroll
-    for (int c = 0; c < z; ++c) b2 += (NS ^ N[c]) & 0x02020202u;
+    // a flipped site has z - anti anti-aligned neighbours
+    const uint32_t b2 = pick((uint32_t)(2 * z) * kOnes - a2, a2, flip);
     M.e += 4 * z - 2 * (int)__builtin_amdgcn_sad_u8(b2 >> 1, 0u, 0u);
     M.m += 8 - 2 * (__builtin_popcount(NS & kMsb) + __builtin_popcount(C & kMsb));
   }
@@ -294,15 +311,16 @@ template <int ND>
 __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
                                                const LaneDraws &D, const TileJob &J, const TileLds &L, uint32_t q,
                                                uint64_t key, uint32_t j0, uint64_t sweep, uint32_t colour, bool meas,
-                                               Meas &M) {
+                                               Meas &M, Nibbles &B) {
   constexpr int z = 2 * ND;
   uint32_t R, idx;  // proposed states; table index d + z = z + #(n != s) - #(n != r) per byte, 0 .. 2z
+  uint32_t ds = 0, dr;  // 16 #(n != s), 16 #(n != r) per byte
   if (q == 2) {
     // r is the other state: #(n != r) = z - #(n != s), index 2 #(n != s)
     R = S ^ kOnes;
-    uint32_t ds = 0;  // 16 #(n != s) per byte
 #pragma unroll
     for (int c = 0; c < z; ++c) ds += ne16(N[c], S);
+    dr = (uint32_t)(16 * z) * kOnes - ds;
     idx = ds >> 3;  // bytes of ds are multiples of 16: nothing crosses a byte
   } else {
     const uint32_t m = q - 1;
@@ -328,7 +346,7 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
       }
     }
     R = a + ((((a | kMsb) - S) & kMsb) >> 7);  // skip the current state
-    uint32_t ds = 0, dr = 0;  // 16 #(n != s), 16 #(n != r) per byte
+    dr = 0;
 #pragma unroll
     for (int c = 0; c < z; ++c) {
       ds += ne16(N[c], S);
@@ -337,19 +355,33 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
     // 16 z + ds - dr lies in [0, 32 z] per byte, a multiple of 16
     idx = ((uint32_t)(16 * z) * kOnes + ds - dr) >> 4;
   }
-  const uint32_t acc = accept_bytes<ND == 1>(idx, D, J, L, key, j0, sweep, colour);
-  const uint32_t am = spread_msb(acc);  // 0xff in accepted bytes
+  const uint32_t am = spread_msb(accept_bytes<ND == 1>(idx, D, J, L, key, j0, sweep, colour));  // 0xff where accepted
   const uint32_t NS = pick(R, S, am);
   if (meas) {
-    uint32_t differ = 0;
+    // bonds to unlike neighbours after the update: those of r where accepted
+    M.e += 4 * z - (int)(__builtin_amdgcn_sad_u8(pick(dr, ds, am), 0u, 0u) >> 4);
+    // sites in each state among the four updated and the four other-colour
+    // sites: v_perm look-ups give the nibble counts of states 0 .. 3 (q <= 4)
+    // or 1 in every byte in state v (q <= 8)
+    if (q <= 4u) {
+      constexpr uint32_t k01 = 0x00001001u, k23 = 0x10010000u;  // state -> nibble
+      B.n01 += __builtin_amdgcn_perm(0u, k01, NS) + __builtin_amdgcn_perm(0u, k01, C);
+      if (q > 2u) B.n23 += __builtin_amdgcn_perm(0u, k23, NS) + __builtin_amdgcn_perm(0u, k23, C);
+      return NS;
+    }
 #pragma unroll
-    for (int c = 0; c < z; ++c) differ += ne16(N[c], NS);
-    M.e += 4 * z - (int)(__builtin_amdgcn_sad_u8(differ, 0u, 0u) >> 4);
-    for (uint32_t v = 0; v < q; ++v) {
-      const uint32_t vv = v * kOnes;
-      const uint64_t n = __builtin_amdgcn_sad_u8(eq_bytes(NS, vv) >> 7, 0u, 0u) +
-                         __builtin_amdgcn_sad_u8(eq_bytes(C, vv) >> 7, 0u, 0u);
-      if (v < 8) M.cnt_lo += n << (8 * v); else M.cnt_hi += n << (8 * (v - 8));
+    for (uint32_t v = 0; v < (uint32_t)kMaxQ; ++v) {
+      if (v >= q) break;
+      uint32_t n;
+      if (q <= 8u) {
+        const uint32_t lo = v < 4u ? 1u << (8u * v) : 0u, hi = v >= 4u ? 1u << (8u * (v - 4u)) : 0u;
+        n = __builtin_amdgcn_sad_u8(__builtin_amdgcn_perm(hi, lo, NS), 0u,
+                                    __builtin_amdgcn_sad_u8(__builtin_amdgcn_perm(hi, lo, C), 0u, 0u));
+      } else {
+        const uint32_t vv = v * kOnes;
+        n = __builtin_amdgcn_sad_u8(eq_bytes(NS, vv) >> 7, 0u, __builtin_amdgcn_sad_u8(eq_bytes(C, vv) >> 7, 0u, 0u));
+      }
+      if (v < 8u) M.cnt_lo += (uint64_t)n << (8u * v); else M.cnt_hi += (uint64_t)n << (8u * (v - 8u));
     }
   }
   return NS;
@@ -464,8 +496,10 @@ __device__ __forceinline__ void run_tile4(TileLds &L, const TileJob &J, uint32_t
       N[1] = pick(Rx, pick(Rp, C, pmask), sp.pos_r < 4u ? 0xFFu << (8u * sp.pos_r) : 0u);
 #pragma unroll
       for (int c = 0; c < z - 2; ++c) N[2 + c] = pick(Na[c], Nb[c], amask);
+      Nibbles B;
       const uint32_t NS = KIND == kIsing ? ising_word<ND>(O, N, C, D, J, L, key, j0, sweep, colour, meas, M)
-                                         : potts_word<ND>(O, N, C, D, J, L, q, key, j0, sweep, colour, meas, M);
+                                         : potts_word<ND>(O, N, C, D, J, L, q, key, j0, sweep, colour, meas, M, B);
+      if (KIND == kPotts && meas && q <= 4u) fold_nibbles(M, B);
       if (NS != O) store4(gown + j0, NS);
     } else {
 #pragma unroll
@@ -567,6 +601,7 @@ __device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_
         for (int c = 0; c < z - 2; ++c) Nw[c] = pick16(Na[c], load16(obase + sp.at_b(js, c)), amask);
       }
       W16 NS;
+      Nibbles B;
       bool changed = false;
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
@@ -579,9 +614,10 @@ __device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_
                           word_of(PM, (uint32_t)w)};
         const uint32_t S = O.d[w], jw = j0 + 4u * (uint32_t)w;
         NS.d[w] = KIND == kIsing ? ising_word<ND>(S, N, C.d[w], D, J, L, key, jw, sweep, colour, meas, M)
-                                 : potts_word<ND>(S, N, C.d[w], D, J, L, q, key, jw, sweep, colour, meas, M);
+                                 : potts_word<ND>(S, N, C.d[w], D, J, L, q, key, jw, sweep, colour, meas, M, B);
         changed |= NS.d[w] != S;
       }
+      if (KIND == kPotts && meas && q <= 4u) fold_nibbles(M, B);
       if (changed) store16(gown + j0, NS);
     } else {
 #pragma unroll
@@ -748,10 +784,10 @@ uint64_t div_magic(uint32_t d) {
   return ~0ull / d + 1;  // ceil(2^64 / d) for d >= 2 (exact when d divides 2^64)
 }
 
-// 16 sites per lane once a launch covers >= 4 Mi colour-class sites (>= 4096
-// waves of 64 lanes, 4 per SIMD); below that, 4 per lane keeps enough waves
-// in flight.
-uint32_t tile_lane_sites(uint64_t total_sites) { return total_sites >= (4ull << 20) ? 16u : 4u; }
+// 16 sites per lane once a launch covers >= 2 Mi colour-class sites (>= 2048
+// waves of 64 lanes); below that, 4 per lane keeps enough waves in flight
+// (measured crossover on MI355X: 1.5 M sites favour 4, 2 M sites favour 16).
+uint32_t tile_lane_sites(uint64_t total_sites) { return total_sites >= (2ull << 20) ? 16u : 4u; }
 
 hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint32_t lane_sites,
                              uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream) {

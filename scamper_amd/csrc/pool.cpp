@@ -124,10 +124,8 @@ struct scamper_pool {
   long long *d_ring[2] = {nullptr, nullptr};
   long long *d_compact[2] = {nullptr, nullptr};
   long long *h_compact[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  uint32_t graph_grid[2] = {0, 0};
-  uint32_t graph_lane[2] = {0, 0};
+  // captured chunks per table buffer, keyed by launch shape (grid << 8 | lane)
+  std::map<uint64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs[2];
   hipEvent_t ev_start[2] = {nullptr, nullptr}, ev_graph[2] = {nullptr, nullptr}, ev_end[2] = {nullptr, nullptr};
   InFlight inflight[2];
   uint64_t chunk = 0, next_id = 1;
@@ -147,8 +145,10 @@ struct scamper_pool {
   ~scamper_pool() {
     if (stream) hipStreamSynchronize(stream);
     for (int p = 0; p < 2; ++p) {
-      if (exec[p]) hipGraphExecDestroy(exec[p]);
-      if (graph[p]) hipGraphDestroy(graph[p]);
+      for (auto &kv : graphs[p]) {
+        hipGraphExecDestroy(kv.second.second);
+        hipGraphDestroy(kv.second.first);
+      }
       hipFree(d_table[p]);
       hipFree(d_ring[p]);
       hipFree(d_compact[p]);
@@ -184,12 +184,15 @@ int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, hip
   return SCAMPER_OK;
 }
 
-int build_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane) {
-  if (P->exec[p]) hipGraphExecDestroy(P->exec[p]);
-  if (P->graph[p]) hipGraphDestroy(P->graph[p]);
-  P->exec[p] = nullptr;
-  P->graph[p] = nullptr;
-  P->graph_grid[p] = 0;
+// The captured chunk for launch shape (grid, lane) on table buffer p,
+// captured on first use.
+int chunk_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane, hipGraphExec_t *exec) {
+  const uint64_t key = ((uint64_t)grid << 8) | lane;
+  auto it = P->graphs[p].find(key);
+  if (it != P->graphs[p].end()) {
+    *exec = it->second.second;
+    return SCAMPER_OK;
+  }
   SC_HIP(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
   const int rc = enqueue_chunk_body(P, p, grid, lane, P->cap_stream);
   hipGraph_t g = nullptr;
@@ -199,10 +202,14 @@ int build_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane) {
     return rc;
   }
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "graph capture: %s", hipGetErrorString(e));
-  P->graph[p] = g;
-  SC_HIP(hipGraphInstantiate(&P->exec[p], g, nullptr, nullptr, 0));
-  P->graph_grid[p] = grid;
-  P->graph_lane[p] = lane;
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    hipGraphDestroy(g);
+    return fail(SCAMPER_EDEVICE, "graph instantiation: %s", hipGetErrorString(ei));
+  }
+  P->graphs[p][key] = {g, x};
+  *exec = x;
   return SCAMPER_OK;
 }
 
@@ -413,14 +420,20 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   hdr->n_tiles = tiles;
   hdr->pad[0] = hdr->pad[1] = 0;
   int candidate;
-  const uint32_t grid = choose_grid(P, &candidate);
-  if (P->cfg.use_graphs && (P->graph_grid[p] != grid || P->graph_lane[p] != lane))
-    if (int rc = build_graph(P, p, grid, lane)) return rc;
+  uint32_t grid = choose_grid(P, &candidate);
+  // no more blocks than tiles (rounded up to a power of two, so that few
+  // launch shapes and captured graphs arise as the table shrinks and grows)
+  uint32_t fit = 8;
+  while (fit < tiles && fit < grid) fit <<= 1;
+  grid = std::min(grid, fit);
+  hipGraphExec_t exec = nullptr;
+  if (P->cfg.use_graphs)
+    if (int rc = chunk_graph(P, p, grid, lane, &exec)) return rc;
   SC_HIP(hipMemcpyAsync(P->d_table[p], P->h_table[p], sizeof(TileHeader) + F.rows.size() * sizeof(TileJob),
                         hipMemcpyHostToDevice, P->stream));
   SC_HIP(hipEventRecord(P->ev_start[p], P->stream));
   if (P->cfg.use_graphs) {
-    SC_HIP(hipGraphLaunch(P->exec[p], P->stream));
+    SC_HIP(hipGraphLaunch(exec, P->stream));
   } else if (int rc = enqueue_chunk_body(P, p, grid, lane, P->stream)) {
     return rc;
   }

@@ -98,7 +98,7 @@ def test_pool_final_spins_match_oracle(sc, orc):
 
 
 def test_large_mix_matches_small_launches_and_oracle(sc, orc):
-    """With >= 4 Mi sites per colour in the table the pool switches to 16
+    """With >= 2 Mi sites per colour in the table the pool switches to 16
     sites per lane; every job still equals its run_one (4 per lane) and the
     big job's final spins equal the oracle."""
     specs = [_spec(sc, (4100, 4100), 1, 3, 1.0, 1, 2, 61), _spec(sc, (34, 20, 14), 1, 6, 0.9, 2, 5, 62),

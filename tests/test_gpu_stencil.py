@@ -66,7 +66,7 @@ CASES = [
     ((10, 6, 4), 1, 3, 0, 0.55, 5),
     ((18, 10, 6), 0, 2, 0, 0.25, 4),
     ((26, 14, 6), 1, 4, 0, 0.62863, 4),
-    # 16 sites per lane (launches over >= 4 Mi sites per colour)
+    # 16 sites per lane (launches over >= 2 Mi sites per colour)
     ((4100, 4100), 1, 3, 0, 1.0, 3),
     ((204, 206, 400), 1, 4, 0, 0.62863, 3),
     ((4098, 4100), 0, 2, 0, 0.4406868, 3),
