@@ -15,6 +15,7 @@ as plain per-job launches on the same GPU.
 Other workloads, one JSON line each (not the headline):
     --workload pt3d   64-replica multi-spin parallel tempering, 3-D L=128
     --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
+    --workload potts3d  the tiled Potts kernel alone: q=3, 3-D L=256, measured sweeps
     --workload serve  the serving block alone
     --workload calibrate  kernels of known traffic for scaling rocprofv3 counters
 
@@ -206,6 +207,33 @@ def on3(sc, args) -> dict:
                          "frac": round(gbs / HBM_PEAK_GBS, 4)}}
 
 
+def potts3d(sc, args) -> dict:
+    """The serving pool's kernel on one large lattice through the plain state
+    path (no graphs, so a profiler sees every launch): 3-state Potts, 3-D
+    L=256, J=1, beta=1.0 (T_c is 1/0.550565), Metropolis with the fused
+    per-sweep measurement on every sweep."""
+    L, q = 256, 3
+    lat = sc.Lattice.hypercubic([L, L, L], True)
+    st = sc.State(lat, sc.model(sc.POTTS, 1.0, q), 1, 42)
+    st.init(True)
+    st.set_beta(1.0)
+    st.sweep(sc.METROPOLIS, args.warmup, measure=True)
+    st.synchronize()
+    ms = st.sweep_timed(sc.METROPOLIS, args.potts_steps, measure=True)
+    n = lat.n_sites
+    flips = n * args.potts_steps
+    # per site-update: read own 1 B + the other colour 1 B (neighbours through
+    # the caches) + write own 1 B
+    gbs = 3 * flips / (ms * 1e-3) / 1e9
+    return {"metric": "spin-flips/ns (3-state Potts 3-D L=256, tiled kernel, measured sweeps)",
+            "value": round(flips / (ms * 1e6), 3), "unit": "spin-flips/ns", "n_gpus": 1,
+            "steps": args.potts_steps, "warmup": args.warmup, "ms_per_step": ms / args.potts_steps,
+            "higher_is_better": True, "dtype": "int8 states", "data": "synthetic",
+            "config": {"workload": "potts3d_tiles", "L": L, "q": q, "beta": 1.0, "seed": 42},
+            "roofline": {"bound": "valu", "hbm_equivalent_GBps": round(gbs, 1),
+                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}}
+
+
 def calibrate(sc, args) -> dict:
     """Kernels of known HBM traffic on a state larger than the Infinity Cache,
     for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE: a random start of a
@@ -227,7 +255,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=16384)
-    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "serve", "calibrate"], default="ising2d")
+    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "potts3d", "serve", "calibrate"],
+                    default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -235,6 +264,7 @@ def main() -> int:
     ap.add_argument("--plain-requests", type=int, default=48)
     ap.add_argument("--pt-steps", type=int, default=10_000)
     ap.add_argument("--on3-steps", type=int, default=200)
+    ap.add_argument("--potts-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU "
@@ -249,6 +279,8 @@ def main() -> int:
         line = pt3d(sc, args)
     elif args.workload == "on3":
         line = on3(sc, args)
+    elif args.workload == "potts3d":
+        line = potts3d(sc, args)
     elif args.workload == "serve":
         line = {"metric": "batched-queue sweeps/s + p99 latency (Potts serving mix)", "n_gpus": 1,
                 "serving": serving(sc, args)}
