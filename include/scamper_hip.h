@@ -64,6 +64,16 @@ int scamper_lattice_info(const scamper_lattice *lat, uint64_t *n_sites, uint64_t
 /* CSR arrays in the reference's order (offsets[n+1], neighbors/edge_ids[n_bonds]). */
 int scamper_lattice_csr(const scamper_lattice *lat, uint64_t *offsets, uint64_t *neighbors,
                         uint64_t *edge_ids);
+/* Physical edges in CsrLattice::edges order (graph.rs:112-184): source,
+ * target, BondType code and weight, n_edges of each; any output may be NULL.
+ * Codes follow BondType's declaration order (graph.rs:12-28): 0 generic,
+ * 1 chain_x, 2 square_x, 3 square_y, 4 square_z, 5 cubic_x, 6 cubic_y,
+ * 7 cubic_z, 8 tri_x, 9 tri_y, 10 tri_diag, 11 honey_x, 12 honey_y,
+ * 13 kagome; the builders label as build_chain/build_square, the 3-D
+ * lattices of build_lattice_from_params (classical_mc.rs:361-383),
+ * build_triangular/honeycomb/kagome and try_from_edges (generic) do. */
+int scamper_lattice_edges(const scamper_lattice *lat, uint64_t *source, uint64_t *target, uint8_t *kind,
+                          double *weight);
 /* Colour of each site and its rank inside its colour class. */
 int scamper_lattice_colours(const scamper_lattice *lat, uint32_t *colour, uint64_t *class_index);
 void scamper_lattice_free(scamper_lattice *lat);
@@ -110,6 +120,11 @@ int scamper_state_upload_f32(scamper_state *st, uint32_t replica, const float *s
 int scamper_state_download_f32(scamper_state *st, uint32_t replica, float *spins);
 /* Sweeps done so far (the counter in every Philox draw). */
 int scamper_state_sweeps(const scamper_state *st, uint64_t *sweeps);
+/* Set that counter: the whole random-number state of a replica is (seed,
+ * replica, sweeps), so a state restored from a snapshot continues the
+ * original trajectory exactly (the role of the Context/RNG checkpoint,
+ * Carlo.rs/src/context.rs:348-420, rng_checkpoint.rs:15-60). */
+int scamper_state_set_sweeps(scamper_state *st, uint64_t sweeps);
 
 /* ------------------------------------------------------------- sweeping */
 enum {

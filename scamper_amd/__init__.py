@@ -78,6 +78,7 @@ SIGNATURES = {
     "scamper_lattice_from_edges": (i32, [u64, vp, vp, vp, u64, PP]),
     "scamper_lattice_info": (i32, [vp, vp, vp, vp, vp]),
     "scamper_lattice_csr": (i32, [vp, vp, vp, vp]),
+    "scamper_lattice_edges": (i32, [vp, vp, vp, vp, vp]),
     "scamper_lattice_colours": (i32, [vp, vp, vp]),
     "scamper_lattice_free": (None, [vp]),
     "scamper_state_create": (i32, [vp, C.POINTER(Model), u32, u64, PP]),
@@ -92,6 +93,7 @@ SIGNATURES = {
     "scamper_state_upload_f32": (i32, [vp, u32, vp]),
     "scamper_state_download_f32": (i32, [vp, u32, vp]),
     "scamper_state_sweeps": (i32, [vp, vp]),
+    "scamper_state_set_sweeps": (i32, [vp, u64]),
     "scamper_sweep": (i32, [vp, i32, u64, i32]),
     "scamper_synchronize": (i32, [vp]),
     "scamper_sweep_timed": (i32, [vp, i32, u64, i32, vp]),
@@ -153,6 +155,11 @@ def check(rc: int) -> None:
 
 def _p(a: np.ndarray | None):
     return None if a is None else a.ctypes.data
+
+
+# BondType::as_label in declaration order (CMC.rs/src/lattice/graph.rs:12-50).
+BOND_LABELS = ("generic", "chain_x", "square_x", "square_y", "square_z", "cubic_x", "cubic_y",
+               "cubic_z", "tri_x", "tri_y", "tri_diag", "honey_x", "honey_y", "kagome")
 
 
 class Lattice:
@@ -222,6 +229,16 @@ class Lattice:
         check(lib().scamper_lattice_csr(self.h, _p(off), _p(nb), _p(eid)))
         return off, nb[: self.n_bonds], eid[: self.n_bonds]
 
+    def edges(self):
+        """Physical edges (CsrLattice::edges, graph.rs:112-184): source, target,
+        BondType label (graph.rs:33-50) and weight per edge."""
+        n = max(1, self.n_edges)
+        src, dst = np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+        kind, w = np.zeros(n, np.uint8), np.zeros(n, np.float64)
+        check(lib().scamper_lattice_edges(self.h, _p(src), _p(dst), _p(kind), _p(w)))
+        k = self.n_edges
+        return src[:k], dst[:k], [BOND_LABELS[int(c)] for c in kind[:k]], w[:k]
+
     def colours(self):
         col = np.zeros(self.n_sites, np.uint32)
         idx = np.zeros(self.n_sites, np.uint64)
@@ -273,11 +290,41 @@ class State:
         check(lib().scamper_state_download_f64(self.h, replica, _p(a)))
         return a
 
+    def upload_f32(self, spins, replica: int = 0):
+        """O(3) states: 3 f32 components per site, site-major."""
+        a = np.ascontiguousarray(spins, dtype=np.float32)
+        check(lib().scamper_state_upload_f32(self.h, replica, _p(a)))
+
+    def download_f32(self, replica: int = 0) -> np.ndarray:
+        a = np.zeros(self.lattice.n_sites * 3, np.float32)
+        check(lib().scamper_state_download_f32(self.h, replica, _p(a)))
+        return a
+
     @property
     def sweeps(self) -> int:
         v = u64()
         check(lib().scamper_state_sweeps(self.h, C.byref(v)))
         return v.value
+
+    def set_sweeps(self, sweeps: int):
+        check(lib().scamper_state_set_sweeps(self.h, int(sweeps)))
+
+    def get_beta(self, replica: int = 0) -> float:
+        b = dbl()
+        check(lib().scamper_state_get_beta(self.h, replica, C.byref(b)))
+        return b.value
+
+    def save_snapshot(self, replica: int = 0, topology: bool = True) -> dict:
+        """ClassicalMC::save_snapshot (CMC.rs/src/classical_mc.rs:65-92) of one
+        replica; see scamper_amd.snapshot."""
+        from .snapshot import save_snapshot
+        return save_snapshot(self, replica, topology)
+
+    def load_snapshot(self, snapshot: dict, replica: int = 0):
+        """ClassicalMC::load_snapshot (classical_mc.rs:94-165) into one replica;
+        see scamper_amd.snapshot."""
+        from .snapshot import load_snapshot
+        load_snapshot(self, snapshot, replica)
 
     def sweep(self, algorithm: int = METROPOLIS, n_sweeps: int = 1, measure: bool = False):
         check(lib().scamper_sweep(self.h, algorithm, n_sweeps, int(measure)))
@@ -487,3 +534,6 @@ class Pool:
         out = PoolStats()
         check(lib().scamper_pool_stats_get(self.h, C.byref(out)))
         return out
+
+
+from . import snapshot  # noqa: E402  (cmc-rs-snapshot-v2 export/import)

@@ -367,7 +367,7 @@ int scamper_lattice_from_edges(uint64_t n_sites, const uint64_t *src, const uint
                                const double *weight, uint64_t n_edges, scamper_lattice **out) {
   if (!out || (n_edges && (!src || !dst))) return fail(SCAMPER_EINVAL, "null argument");
   std::vector<Bond> e(n_edges);
-  for (uint64_t k = 0; k < n_edges; ++k) e[k] = Bond{src[k], dst[k], 0xFF, weight ? weight[k] : 1.0};
+  for (uint64_t k = 0; k < n_edges; ++k) e[k] = Bond{src[k], dst[k], kBondGeneric, weight ? weight[k] : 1.0};
   Lattice l;
   std::string err;
   if (!lattice_from_edges(n_sites, std::move(e), l, err)) return fail(SCAMPER_EINVAL, "%s", err.c_str());
@@ -392,6 +392,21 @@ int scamper_lattice_csr(const scamper_lattice *h, uint64_t *offsets, uint64_t *n
   if (offsets) std::copy(l.offsets.begin(), l.offsets.end(), offsets);
   if (neighbors) std::copy(l.neighbors.begin(), l.neighbors.end(), neighbors);
   if (edge_ids) std::copy(l.edge_ids.begin(), l.edge_ids.end(), edge_ids);
+  return SCAMPER_OK;
+}
+
+int scamper_lattice_edges(const scamper_lattice *h, uint64_t *source, uint64_t *target, uint8_t *kind,
+                          double *weight) {
+  if (!h) return fail(SCAMPER_EINVAL, "null lattice");
+  std::string err;
+  if (!materialize(const_cast<scamper_lattice *>(h)->lat, err)) return fail(SCAMPER_ENOMEM, "%s", err.c_str());
+  const std::vector<Bond> &e = h->lat.edges;
+  for (size_t k = 0; k < e.size(); ++k) {
+    if (source) source[k] = e[k].source;
+    if (target) target[k] = e[k].target;
+    if (kind) kind[k] = e[k].kind;
+    if (weight) weight[k] = e[k].weight;
+  }
   return SCAMPER_OK;
 }
 
@@ -642,6 +657,12 @@ int scamper_state_download_f32(scamper_state *st, uint32_t replica, float *spins
 int scamper_state_sweeps(const scamper_state *st, uint64_t *sweeps) {
   if (!st || !sweeps) return fail(SCAMPER_EINVAL, "null argument");
   *sweeps = st->sweeps;
+  return SCAMPER_OK;
+}
+
+int scamper_state_set_sweeps(scamper_state *st, uint64_t sweeps) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  st->sweeps = sweeps;
   return SCAMPER_OK;
 }
 

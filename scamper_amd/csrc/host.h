@@ -11,10 +11,29 @@
 namespace scamper {
 
 // One physical undirected bond (CMC.rs/src/lattice/graph.rs:76-82).
+// Bond labels in the order of the reference's BondType (CMC.rs/src/lattice/
+// graph.rs:12-28); the serialised labels are BondType::as_label (graph.rs:33-50).
+enum BondKind : uint8_t {
+  kBondGeneric = 0,
+  kBondChainX,
+  kBondSquareX,
+  kBondSquareY,
+  kBondSquareZ,
+  kBondCubicX,
+  kBondCubicY,
+  kBondCubicZ,
+  kBondTriX,
+  kBondTriY,
+  kBondTriDiag,
+  kBondHoneyX,
+  kBondHoneyY,
+  kBondKagome,
+};
+
 struct Bond {
   uint64_t source;
   uint64_t target;
-  uint8_t kind;
+  uint8_t kind;  // BondKind
   double weight;
 };
 

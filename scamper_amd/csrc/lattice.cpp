@@ -125,7 +125,13 @@ bool lattice_from_edges(uint64_t n_sites, std::vector<Bond> edges, Lattice &out,
 namespace {
 // build_hypercubic (graph.rs:417-451): site = x + Lx*(y + Ly*z), one bond
 // to +1 along every axis, wrapped when periodic and the side exceeds one.
+// Bond labels as build_chain / build_square (graph.rs:402-414) and the 3-D
+// lattices of build_lattice_from_params (classical_mc.rs:361-383) give them;
+// generic beyond three dimensions.
 bool build_hypercubic_csr(const uint64_t *dims, int n_dims, bool pbc, Lattice &out, std::string &err) {
+  static const uint8_t kAxisKind[3][3] = {{kBondChainX, 0, 0},
+                                          {kBondSquareX, kBondSquareY, 0},
+                                          {kBondCubicX, kBondCubicY, kBondCubicZ}};
   uint64_t strides[8], n = 1;
   for (int a = 0; a < n_dims; ++a) {
     strides[a] = n;
@@ -144,7 +150,8 @@ bool build_hypercubic_csr(const uint64_t *dims, int n_dims, bool pbc, Lattice &o
       if (coords[a] + 1 < dims[a]) next = coords[a] + 1;
       else if (pbc && dims[a] > 1) next = 0;
       else continue;
-      edges.push_back(Bond{site, site - coords[a] * strides[a] + next * strides[a], (uint8_t)a, 1.0});
+      const uint8_t kind = n_dims <= 3 ? kAxisKind[n_dims - 1][a] : (uint8_t)kBondGeneric;
+      edges.push_back(Bond{site, site - coords[a] * strides[a] + next * strides[a], kind, 1.0});
     }
   }
   return lattice_from_edges(n, std::move(edges), out, err);
@@ -204,9 +211,9 @@ bool lattice_triangular(uint64_t lx, uint64_t ly, Lattice &out, std::string &err
   for (uint64_t y = 0; y < ly; ++y)
     for (uint64_t x = 0; x < lx; ++x) {
       const uint64_t s = y * lx + x;
-      e.push_back(Bond{s, y * lx + (x + 1) % lx, 0, 1.0});
-      e.push_back(Bond{s, ((y + 1) % ly) * lx + x, 1, 1.0});
-      e.push_back(Bond{s, ((y + 1) % ly) * lx + (x + 1) % lx, 2, 1.0});
+      e.push_back(Bond{s, y * lx + (x + 1) % lx, kBondTriX, 1.0});
+      e.push_back(Bond{s, ((y + 1) % ly) * lx + x, kBondTriY, 1.0});
+      e.push_back(Bond{s, ((y + 1) % ly) * lx + (x + 1) % lx, kBondTriDiag, 1.0});
     }
   return lattice_from_edges(lx * ly, std::move(e), out, err);
 }
@@ -230,7 +237,8 @@ bool from_adjacency(const std::vector<std::vector<uint64_t>> &rows, Lattice &out
       err = "adjacency is not symmetric";
       return false;
     }
-    for (uint64_t k = 0; k < kv.second.first; ++k) e.push_back(Bond{kv.first.first, kv.first.second, 0, 1.0});
+    for (uint64_t k = 0; k < kv.second.first; ++k)
+      e.push_back(Bond{kv.first.first, kv.first.second, kBondGeneric, 1.0});
   }
   return lattice_from_edges(rows.size(), std::move(e), out, err);
 }
@@ -249,7 +257,10 @@ bool lattice_honeycomb(uint64_t lx, uint64_t ly, Lattice &out, std::string &err)
                                           : ((y + ly - 1) % ly) * lx + (x + lx - 1) % lx;
       adj[y * lx + x] = {y * lx + (x + 1) % lx, y * lx + (x + lx - 1) % lx, matched};
     }
-  return from_adjacency(adj, out, err);
+  if (!from_adjacency(adj, out, err)) return false;
+  // labels (graph.rs:497-504): honey_x within a row, honey_y across rows
+  for (Bond &b : out.edges) b.kind = b.source / lx == b.target / lx ? kBondHoneyX : kBondHoneyY;
+  return true;
 }
 
 // build_kagome (graph.rs:508-543).
@@ -266,7 +277,9 @@ bool lattice_kagome(uint64_t lx, uint64_t ly, Lattice &out, std::string &err) {
       adj[idx(1, x, y)] = {idx(0, x, y), idx(2, x, y), idx(0, (x + 1) % lx, y), idx(2, (x + lx - 1) % lx, y)};
       adj[idx(2, x, y)] = {idx(0, x, y), idx(1, x, y), idx(0, x, (y + 1) % ly), idx(1, (x + 1) % lx, y)};
     }
-  return from_adjacency(adj, out, err);
+  if (!from_adjacency(adj, out, err)) return false;
+  for (Bond &b : out.edges) b.kind = kBondKagome;  // graph.rs:539-541
+  return true;
 }
 
 }  // namespace scamper
