@@ -92,6 +92,9 @@ def _declare(L: C.CDLL) -> None:
         "orc_run_one_ising": (u64, [vp, dbl, dbl, i32, u64, u64, u64, u64, vp, vp, vp]),
         "orc_plain_sweeps": (u64, [vp, dbl, dbl, u64, u64]),
         "orc_xoshiro_seq": (None, [u64, u64, vp]),
+        "orc_cluster_word": (u32, [u64, u64, u64, u32]),
+        "orc_cluster_threshold": (u64, [i32, dbl, dbl, dbl]),
+        "orc_cluster_steps": (u64, [vp, vp, vp, dbl, u64, u64, u64, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -263,6 +266,13 @@ def random_start(lat: Lattice, m: Model, key: int) -> np.ndarray:
     out = np.zeros(lat.n_sites * m.spin_dim, np.float64)
     lib().orc_random_start(C.byref(m), lat.ptr, key, _p(out))
     return out
+
+
+def cluster_steps(lat: Lattice, m: Model, spins: np.ndarray, beta: float, key: int, step0: int,
+                  n: int, wolff: bool) -> int:
+    """orc_cluster_steps: n Swendsen-Wang or Wolff updates in place (f64 spins)."""
+    assert spins.dtype == np.float64 and spins.flags.c_contiguous
+    return lib().orc_cluster_steps(C.byref(m), lat.ptr, _p(spins), beta, key, step0, n, int(wolff))
 
 
 def stencil_sweeps(spins_i8: np.ndarray, dims, kind, q, beta, j, heat_bath, key, sweep0,

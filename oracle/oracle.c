@@ -41,6 +41,9 @@
 #define ORC_STREAM_CONT_FALLBACK 8u
 #define ORC_STREAM_EXCHANGE 9u
 #define ORC_STREAM_CONT_INIT 10u
+#define ORC_STREAM_CLUSTER_BOND 11u
+#define ORC_STREAM_CLUSTER_TARGET 12u
+#define ORC_STREAM_CLUSTER_SEED 13u
 
 static const double TWO_M32 = 2.3283064365386963e-10; /* 2^-32 */
 static const double TAU = 6.283185307179586;          /* std::f64::consts::TAU */
@@ -1270,4 +1273,99 @@ uint64_t orc_plain_sweeps(const orc_lattice *lat, double j, double beta, uint64_
   free(order);
   free(spins);
   return n * n_sweeps;
+}
+
+
+/* ----------------------------------------------------- cluster updates */
+
+uint32_t orc_cluster_word(uint64_t key, uint64_t idx, uint64_t step, uint32_t stream) {
+  uint32_t out[4];
+  block(key, (uint32_t)(idx >> 2), step, 0, stream, (uint32_t)(idx >> 34), out);
+  return out[idx & 3];
+}
+
+/* clamp_probability (models.rs:16-24). */
+static double clamp_probability(double v) {
+  if (isfinite(v)) return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+  return v > 0.0 ? 1.0 : 0.0;
+}
+
+uint64_t orc_cluster_threshold(int kind, double j, double weight, double beta) {
+  const double coupling = j * weight;
+  if (coupling == 0.0) return 0;
+  /* Ising 1 - exp(-2 beta c) (models.rs:146), Potts 1 - exp(-beta c) (models.rs:328) */
+  const double arg = kind == ORC_ISING ? -2.0 * beta * coupling : -beta * coupling;
+  const double p = clamp_probability(1.0 - exp(arg));
+  return (uint64_t)ceil(p * 4294967296.0); /* p 2^32 is exact; k < K <=> k 2^-32 < p */
+}
+
+/* Union-find with the smaller site as the surviving root, so every root is
+ * its component's smallest site (SWCore's union by rank picks another root;
+ * the partition, the only thing the update depends on, is the same). */
+static uint64_t uf_find(uint64_t *parent, uint64_t x) {
+  uint64_t r = x;
+  while (parent[r] != r) r = parent[r];
+  while (parent[x] != r) {
+    const uint64_t next = parent[x];
+    parent[x] = r;
+    x = next;
+  }
+  return r;
+}
+
+static void uf_union(uint64_t *parent, uint64_t a, uint64_t b) {
+  a = uf_find(parent, a);
+  b = uf_find(parent, b);
+  if (a == b) return;
+  if (a < b) parent[b] = a;
+  else parent[a] = b;
+}
+
+uint64_t orc_cluster_steps(const orc_model *m, const orc_lattice *lat, double *spins, double beta,
+                           uint64_t key, uint64_t step0, uint64_t n, int wolff) {
+  const uint64_t ns = lat->n_sites, ne = lat->n_edges;
+  uint64_t *parent = (uint64_t *)malloc((ns ? ns : 1) * sizeof(uint64_t));
+  uint64_t result = 0;
+  for (uint64_t k = 0; k < n; ++k) {
+    const uint64_t step = step0 + k;
+    for (uint64_t s = 0; s < ns; ++s) parent[s] = s;
+    /* every physical edge once, in CsrLattice::edges order (swendsen_wang.rs:85-103) */
+    for (uint64_t e = 0; e < ne; ++e) {
+      const orc_bond *b = &lat->edges[e];
+      if (spins[b->source] != spins[b->target]) continue;
+      const uint64_t kk = orc_cluster_threshold(m->kind, m->j, b->weight, beta);
+      if ((uint64_t)orc_cluster_word(key, e, step, ORC_STREAM_CLUSTER_BOND) < kk)
+        uf_union(parent, b->source, b->target);
+    }
+    if (!wolff) {
+      result = 0;
+      for (uint64_t s = 0; s < ns; ++s) {
+        const uint64_t r = uf_find(parent, s);
+        if (r == s) ++result;
+        const uint32_t w = orc_cluster_word(key, r, step, ORC_STREAM_CLUSTER_TARGET);
+        spins[s] = m->kind == ORC_ISING ? (w < 0x80000000u ? 1.0 : -1.0)
+                                        : (double)(((uint64_t)w * (uint64_t)m->q) >> 32);
+      }
+    } else if (ns) {
+      const uint64_t seed = ((uint64_t)orc_cluster_word(key, 0, step, ORC_STREAM_CLUSTER_SEED) * ns) >> 32;
+      double target;
+      if (m->kind == ORC_ISING) {
+        target = -spins[seed];
+      } else {
+        const uint64_t s0 = (uint64_t)spins[seed];
+        uint64_t r = ((uint64_t)orc_cluster_word(key, 1, step, ORC_STREAM_CLUSTER_SEED) * (uint64_t)(m->q - 1)) >> 32;
+        r += r >= s0;
+        target = (double)r;
+      }
+      const uint64_t root = uf_find(parent, seed);
+      result = 0;
+      for (uint64_t s = 0; s < ns; ++s)
+        if (uf_find(parent, s) == root) {
+          spins[s] = target;
+          ++result;
+        }
+    }
+  }
+  free(parent);
+  return result;
 }

@@ -199,6 +199,35 @@ uint64_t orc_plain_sweeps(const orc_lattice *lat, double j, double beta, uint64_
 /* xoshiro256++ seeded like rand_xoshiro's seed_from_u64 (for tests). */
 void orc_xoshiro_seq(uint64_t seed, uint64_t n, uint64_t *out);
 
+/* ----------------------------------------------------- cluster updates */
+
+/* Counter-based uniform of cluster update `step` for index idx in stream s
+ * (11 bond activation by physical edge index, 12 Swendsen-Wang cluster
+ * target by the cluster's smallest site, 13 Wolff seed (idx 0) and Potts
+ * target (idx 1)): word idx & 3 of Philox({idx >> 2, step, x(s), idx >> 34}).
+ * u = word * 2^-32. */
+uint32_t orc_cluster_word(uint64_t key, uint64_t idx, uint64_t step, uint32_t stream);
+/* K with (u < p) <=> (word < K) for the bond probability of
+ * cluster_bond_probability (models.rs:130-147 Ising: 1 - exp(-2 beta J w);
+ * 312-329 Potts: 1 - exp(-beta J w)), after clamp_probability (models.rs:
+ * 16-24): K = ceil(p 2^32) in [0, 2^32]. */
+uint64_t orc_cluster_threshold(int kind, double j, double weight, double beta);
+/* n Swendsen-Wang (wolff = 0; SWCore::sweep_with_phase, algorithms/
+ * swendsen_wang.rs:73-135) or Wolff (wolff = 1; WolffCore, wolff.rs:72-134)
+ * updates of Ising (+-1) or Potts (0..q-1) f64 site-major spins, steps step0
+ * .. step0+n-1, with the draws above:
+ *   bond e (source s, target t) is active iff s and t hold the same state
+ *     and word(11, e) < K(weight e);
+ *   SW: every cluster takes the state its smallest site draws: Ising +1 if
+ *     word(12, root) < 2^31 else -1; Potts floor(u q);
+ *   Wolff: the seed is floor(u N) of word(13, 0); its cluster (connected by
+ *     active bonds) takes -seed (Ising) or, for Potts, the other state
+ *     r = floor(u' (q-1)), r += (r >= seed state), u' from word(13, 1).
+ * Returns the number of SW clusters, or the size of the last Wolff cluster,
+ * of the last step. */
+uint64_t orc_cluster_steps(const orc_model *m, const orc_lattice *lat, double *spins, double beta,
+                           uint64_t key, uint64_t step0, uint64_t n, int wolff);
+
 #ifdef __cplusplus
 }
 #endif
