@@ -16,6 +16,8 @@ Other workloads, one JSON line each (not the headline):
     --workload pt3d   64-replica multi-spin parallel tempering, 3-D L=128
     --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
     --workload potts3d  the tiled Potts kernel alone: q=3, 3-D L=256, measured sweeps
+    --workload cluster  Swendsen-Wang and Wolff updates of the 2-D Ising model at
+                        beta_c, L=4096
     --workload serve  the serving block alone
     --workload calibrate  kernels of known traffic for scaling rocprofv3 counters
 
@@ -234,6 +236,33 @@ def potts3d(sc, args) -> dict:
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}}
 
 
+def cluster(sc, args) -> dict:
+    """Swendsen-Wang and Wolff updates of the 2-D Ising model at beta_c on a
+    4096 x 4096 torus: bond activation, lock-free union-find labelling and the
+    cluster transform, per update over every site."""
+    L, beta = 4096, 0.4406868
+    lat = sc.Lattice.square(L, L, True)
+    n = lat.n_sites
+    out = {"metric": "site-updates/ns (2-D Ising L=4096 at beta_c, cluster updates)", "unit": "site-updates/ns",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "int8 spins, uint32 labels", "data": "synthetic",
+           "config": {"workload": "cluster_ising2d", "L": L, "beta": beta, "seed": 42}}
+    for name, alg, steps in (("swendsen_wang", sc.SWENDSEN_WANG, args.cluster_steps),
+                             ("wolff", sc.WOLFF, args.cluster_steps)):
+        st = sc.State(lat, sc.model(sc.ISING, 1.0), 1, 42)
+        st.init(True)
+        st.set_beta(beta)
+        st.sweep(alg, 20)
+        st.synchronize()
+        ms = st.sweep_timed(alg, steps)
+        e, m = st.observables()
+        out[name] = {"updates": steps, "ms_per_update": round(ms / steps, 4),
+                     "site_updates_per_ns": round(n * steps / (ms * 1e6), 3),
+                     "energy_per_site": e / n, "abs_m": m}
+        st.close()
+    out["value"] = out["swendsen_wang"]["site_updates_per_ns"]
+    return out
+
+
 def calibrate(sc, args) -> dict:
     """Kernels of known HBM traffic on a state larger than the Infinity Cache,
     for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE: a random start of a
@@ -255,7 +284,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=16384)
-    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "potts3d", "serve", "calibrate"],
+    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "potts3d", "cluster", "serve", "calibrate"],
                     default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -265,6 +294,7 @@ def main() -> int:
     ap.add_argument("--pt-steps", type=int, default=10_000)
     ap.add_argument("--on3-steps", type=int, default=200)
     ap.add_argument("--potts-steps", type=int, default=1000)
+    ap.add_argument("--cluster-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU "
@@ -281,6 +311,8 @@ def main() -> int:
         line = on3(sc, args)
     elif args.workload == "potts3d":
         line = potts3d(sc, args)
+    elif args.workload == "cluster":
+        line = cluster(sc, args)
     elif args.workload == "serve":
         line = {"metric": "batched-queue sweeps/s + p99 latency (Potts serving mix)", "n_gpus": 1,
                 "serving": serving(sc, args)}

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libscamper_hip.so")
 
 ISING, POTTS, ON = 0, 1, 2
 METROPOLIS, HEAT_BATH, CONT_HEAT_BATH, OVER_RELAXATION, HYBRID_HB_OR = 0, 1, 2, 3, 4
+SWENDSEN_WANG, WOLFF = 5, 6
 
 STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EINVARIANT",
           5: "EUNSUPPORTED", 6: "EBUDGET"}

@@ -96,11 +96,25 @@ struct scamper_state {
   uint32_t ring_slots = 0;
   uint64_t series_cap = 0, series_len = 0;
   int tables_for = -1;            // algorithm the uploaded tables serve
+  // cluster updates: union-find parents (one replica at a time), the Wolff
+  // seed's root and target, bond thresholds per replica and weight, and a
+  // graph lattice's edges with their integer weights
+  uint32_t *d_parent = nullptr, *d_cscratch = nullptr;
+  uint64_t *d_cthr = nullptr;
+  ClusterEdge *d_cedges = nullptr;
+  int8_t *d_cweights = nullptr;
+  std::vector<uint64_t> cthr;  // host copy of d_cthr, valid while cthr_ready
+  bool cthr_ready = false;
   int device = 0;
 
   ~scamper_state() {
     if (stream) hipStreamSynchronize(stream);
     hipFree(d_spins_alloc);
+    hipFree(d_parent);
+    hipFree(d_cscratch);
+    hipFree(d_cthr);
+    hipFree(d_cedges);
+    hipFree(d_cweights);
     hipFree(d_vec);
     hipFree(d_params);
     hipFree(d_hbexp);
@@ -530,6 +544,7 @@ int scamper_state_set_beta(scamper_state *st, uint32_t replica, double beta) {
   if (!std::isfinite(beta) || beta < 0.0) return fail(SCAMPER_EINVAL, "beta must be finite and non-negative");
   st->beta[replica] = beta;
   st->tables_for = -1;
+  st->cthr_ready = false;
   return SCAMPER_OK;
 }
 
@@ -666,8 +681,111 @@ int scamper_state_set_sweeps(scamper_state *st, uint64_t sweeps) {
   return SCAMPER_OK;
 }
 
+namespace {
+
+// Buffers of the cluster updates, allocated on first use.
+int ensure_cluster(scamper_state *st) {
+  if (st->d_parent) return SCAMPER_OK;
+  const uint64_t n = st->lat.n_sites;
+  if (hipMalloc(&st->d_parent, std::max<uint64_t>(n, 1) * sizeof(uint32_t)) != hipSuccess)
+    return fail(SCAMPER_ENOMEM, "cluster labels: %llu sites", (unsigned long long)n);
+  SC_HIP(hipMalloc(&st->d_cscratch, 2 * sizeof(uint32_t)));
+  SC_HIP(hipMalloc(&st->d_cthr, (size_t)st->n_rep * kClusterWeights * sizeof(uint64_t)));
+  if (st->csr) {
+    const std::vector<Bond> &e = st->lat.edges;
+    std::vector<ClusterEdge> ce(e.size());
+    std::vector<int8_t> cw(e.size());
+    for (size_t k = 0; k < e.size(); ++k) {
+      ce[k] = ClusterEdge{(uint32_t)e[k].source, (uint32_t)e[k].target};
+      cw[k] = (int8_t)e[k].weight;  // integer weights in [-127, 127] on the graph path
+    }
+    SC_HIP(hipMalloc(&st->d_cedges, std::max<size_t>(ce.size(), 1) * sizeof(ClusterEdge)));
+    SC_HIP(hipMalloc(&st->d_cweights, std::max<size_t>(cw.size(), 1)));
+    if (!ce.empty()) {
+      SC_HIP(hipMemcpy(st->d_cedges, ce.data(), ce.size() * sizeof(ClusterEdge), hipMemcpyHostToDevice));
+      SC_HIP(hipMemcpy(st->d_cweights, cw.data(), cw.size(), hipMemcpyHostToDevice));
+    }
+  }
+  return SCAMPER_OK;
+}
+
+// Buffers and bond thresholds (per replica and integer weight) of the
+// cluster updates; the thresholds are rebuilt after a beta changes.
+int prepare_cluster(scamper_state *st) {
+  if (int rc = ensure_cluster(st)) return rc;
+  if (st->cthr_ready) return SCAMPER_OK;
+  st->cthr.assign((size_t)st->n_rep * kClusterWeights, 0);
+  for (uint32_t r = 0; r < st->n_rep; ++r)
+    for (int w = 0; w < kClusterWeights; ++w)
+      st->cthr[(size_t)r * kClusterWeights + w] =
+          cluster_bond_threshold(st->model.kind, st->beta[r], st->model.j, (double)w);
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(st->d_cthr, st->cthr.data(), st->cthr.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  st->cthr_ready = true;
+  return SCAMPER_OK;
+}
+
+// n Swendsen-Wang or Wolff updates of every replica (SWCore / WolffCore,
+// algorithms/swendsen_wang.rs:73-135, wolff.rs:72-134); each counts as one
+// sweep of the draw counter.
+int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
+  if (st->continuous) return fail(SCAMPER_EUNSUPPORTED, "cluster updates take Ising or Potts spins");
+  const uint64_t n = st->lat.n_sites;
+  if (n >= (1ull << 32)) return fail(SCAMPER_EUNSUPPORTED, "cluster updates need fewer than 2^32 sites");
+  // the reference asserts J * w >= 0 (models.rs:139-142, 321-324)
+  if (st->csr) {
+    for (const Bond &b : st->lat.edges)
+      if (st->model.j * b.weight < 0.0)
+        return fail(SCAMPER_EUNSUPPORTED, "cluster updates require J * bond.weight >= 0");
+  } else if (st->model.j < 0.0) {
+    return fail(SCAMPER_EUNSUPPORTED, "cluster updates require J * bond.weight >= 0");
+  }
+  if (int rc = prepare_cluster(st)) return rc;
+  const std::vector<uint64_t> &thr = st->cthr;
+  if (measure) {
+    if (int rc = ensure_series(st, n_sweeps)) return rc;
+    const size_t slot = (size_t)st->n_rep * kEmStride;
+    SC_HIP(hipMemsetAsync(st->d_series + st->series_len * slot, 0, n_sweeps * slot * sizeof(long long), st->stream));
+  }
+  const int wolff = algorithm == kWolff;
+  for (uint64_t t = 0; t < n_sweeps; ++t) {
+    const uint64_t step = st->sweeps + t;
+    for (uint32_t r = 0; r < st->n_rep; ++r) {
+      hipError_t e;
+      if (st->csr) {
+        int8_t *spins = st->d_spins + (uint64_t)r * n;
+        e = launch_cluster_bonds_csr(n, st->lat.edges.size(), st->d_cedges, st->d_cweights, spins, st->d_parent,
+                                     st->keys[r], step, st->d_cthr + (size_t)r * kClusterWeights, st->stream);
+        if (e == hipSuccess)
+          e = launch_cluster_transform(nullptr, n, spins, st->d_parent, st->model.kind, st->model.q, wolff,
+                                       st->keys[r], step, st->d_cscratch, st->stream);
+      } else {
+        int8_t *spins = st->d_spins + 2ull * r * st->geom.nh;
+        e = launch_cluster_bonds_stencil(st->geom, spins, st->d_parent, st->keys[r], step,
+                                         thr[(size_t)r * kClusterWeights + 1], st->stream);
+        if (e == hipSuccess)
+          e = launch_cluster_transform(&st->geom, n, spins, st->d_parent, st->model.kind, st->model.q, wolff,
+                                       st->keys[r], step, st->d_cscratch, st->stream);
+      }
+      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "cluster update launch: %s", hipGetErrorString(e));
+    }
+    if (measure) {
+      long long *em = st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride;
+      const hipError_t e = st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream)
+                                   : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream);
+      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure launch: %s", hipGetErrorString(e));
+    }
+  }
+  st->sweeps += n_sweeps;
+  if (measure) st->series_len += n_sweeps;
+  return SCAMPER_OK;
+}
+
+}  // namespace
+
 int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
   if (!st) return fail(SCAMPER_EINVAL, "null state");
+  if (algorithm == kSwendsenWang || algorithm == kWolff) return cluster_sweeps(st, algorithm, n_sweeps, measure);
   if (st->continuous) {
     if (algorithm != kContinuousHeatBath && algorithm != kOverRelaxation && algorithm != kHybridHeatBathOverRelaxation)
       return fail(SCAMPER_EUNSUPPORTED, "O(3) spins take the continuous heat bath, over-relaxation or their hybrid");
@@ -747,6 +865,8 @@ int scamper_sweep_timed(scamper_state *st, int algorithm, uint64_t n_sweeps, int
   const int table_alg = st->continuous ? kContinuousHeatBath : algorithm;
   if (st->tables_for != table_alg && (st->continuous || algorithm == kMetropolis || algorithm == kHeatBath))
     if (int rc = upload_tables(st, table_alg)) return rc;
+  if ((algorithm == kSwendsenWang || algorithm == kWolff) && !st->continuous && st->lat.n_sites < (1ull << 32))
+    if (int rc = prepare_cluster(st)) return rc;
   if (measure) {
     if (int rc = ensure_series(st, n_sweeps)) return rc;
     if (int rc = ensure_ring(st, algorithm)) return rc;

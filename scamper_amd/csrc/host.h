@@ -88,6 +88,10 @@ bool materialize(Lattice &l, std::string &err);
 uint64_t accept_threshold(double log_acceptance);
 uint64_t below_threshold(double p);
 double log_acceptance(double beta, double delta_energy);
+// Bond threshold of a cluster update: an aligned bond of weight w is active
+// when its draw is below it (cluster_bond_probability, models.rs:130-147,
+// 312-329, clamped as clamp_probability, models.rs:16-24).
+uint64_t cluster_bond_threshold(int kind, double beta, double j, double weight);
 
 // Tables for one replica; `z` is the coordination of the stencil.
 void ising_metropolis_table(double beta, double j, int z, Threshold *thr);

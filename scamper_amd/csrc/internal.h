@@ -18,6 +18,8 @@ enum Algorithm : int32_t {
   kContinuousHeatBath = 2,
   kOverRelaxation = 3,
   kHybridHeatBathOverRelaxation = 4,
+  kSwendsenWang = 5,
+  kWolff = 6,
 };
 
 constexpr int kMaxZ = 32;          // largest coordination the discrete kernels accept
@@ -198,6 +200,30 @@ hipError_t launch_em_fold(const long long *ring, long long *out, uint64_t n_rec,
 // compact[(r * steps + k) * 8 + c] = sum over s < 8 of ring[((r * steps + k) * 8 + s) * 8 + c].
 hipError_t launch_tile_fold(const TileHeader *hdr, const long long *ring, long long *compact, uint32_t steps,
                             hipStream_t stream);
+
+// ------------------------------------------------------------ cluster updates
+// Swendsen-Wang and Wolff updates of one replica (Ising +-1 or Potts states
+// in int8).  Sites are numbered site-major (x + Lx (y + Ly z) on stencils,
+// the lattice's numbering on graphs); a stencil replica's spins are its two
+// colour arrays at `spins` (site s at colour (x+y+z)&1, rank s >> 1), a
+// graph replica's are site-major.  `parent` holds one uint32 per site.
+struct ClusterEdge {
+  uint32_t source, target;
+};
+// Bond thresholds: an aligned bond of integer weight w is active when its
+// draw is below thr[w] (thr[1] for stencils).
+constexpr int kClusterWeights = 128;
+hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spins, uint32_t *parent,
+                                        uint64_t key, uint64_t step, uint64_t thr, hipStream_t stream);
+hipError_t launch_cluster_bonds_csr(uint64_t n_sites, uint64_t n_edges, const ClusterEdge *edges,
+                                    const int8_t *weights, const int8_t *spins, uint32_t *parent, uint64_t key,
+                                    uint64_t step, const uint64_t *thr, hipStream_t stream);
+// SW: every cluster takes its smallest site's target; Wolff: the seed's
+// cluster takes -seed (Ising) or another state (Potts).  `scratch` holds 2
+// uint32 for the Wolff seed's root and target.
+hipError_t launch_cluster_transform(const StencilGeom *g, uint64_t n_sites, int8_t *spins, uint32_t *parent,
+                                    int kind, int q, int wolff, uint64_t key, uint64_t step, uint32_t *scratch,
+                                    hipStream_t stream);
 
 // Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
 bool ising_fast_path_ok(const StencilGeom &g);

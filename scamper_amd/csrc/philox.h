@@ -39,6 +39,9 @@ enum Stream : uint32_t {
   kStreamContFallback = 8,  // continuous isotropic redraw (zero field)
   kStreamExchange = 9,      // replica exchange
   kStreamContInit = 10,     // continuous random start
+  kStreamClusterBond = 11,   // cluster updates: bond activation by physical edge index
+  kStreamClusterTarget = 12, // Swendsen-Wang cluster target by the cluster's smallest site
+  kStreamClusterSeed = 13,   // Wolff seed site (index 0) and Potts target (index 1)
 };
 
 struct U4 {
@@ -95,6 +98,14 @@ SC_HD uint32_t word_of(const U4 &b, uint32_t k) {
   const uint32_t lo = (k & 1) ? b.y : b.x;
   const uint32_t hi = (k & 1) ? b.w : b.z;
   return (k & 2) ? hi : lo;
+}
+
+// Cluster-update draw for index idx of update `step` in stream s: word
+// idx & 3 of the block {idx >> 2, step, x(s), idx >> 34}.
+SC_HD uint32_t cluster_word(uint64_t key, uint64_t idx, uint64_t step, uint32_t stream) {
+  const U4 b = philox4x32_10((uint32_t)(idx >> 2), (uint32_t)step, ctr2(step, 0, stream), (uint32_t)(idx >> 34),
+                             (uint32_t)key, (uint32_t)(key >> 32));
+  return word_of(b, (uint32_t)idx & 3u);
 }
 
 SC_HD uint32_t site_byte(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t stream,

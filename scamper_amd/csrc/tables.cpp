@@ -50,6 +50,19 @@ uint64_t accept_threshold(double a) {
   return hi;
 }
 
+uint64_t cluster_bond_threshold(int kind, double beta, double j, double weight) {
+  // cluster_bond_probability: coupling = J w; Ising 1 - exp(-2 beta c)
+  // (models.rs:130-147), Potts 1 - exp(-beta c) (models.rs:312-329), then
+  // clamp_probability (models.rs:16-24)
+  const double coupling = j * weight;
+  if (coupling == 0.0) return 0;
+  const double arg = kind == kIsing ? -2.0 * beta * coupling : -beta * coupling;
+  double p = 1.0 - std::exp(arg);
+  if (std::isfinite(p)) p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
+  else p = p > 0.0 ? 1.0 : 0.0;
+  return below_threshold(p);
+}
+
 uint64_t below_threshold(double p) {
   if (!(p > 0.0)) return 0;
   if ((double)(kTwo32 - 1) * kTwoM32 < p) return kTwo32;

@@ -1,0 +1,122 @@
+"""Cluster updates on the GPU (SCAMPER_SWENDSEN_WANG, SCAMPER_WOLFF).
+
+Parity: from the same spins and the same counter-based draws, k device
+updates leave exactly the spins of k oracle updates (orc_cluster_steps), on
+stencil lattices (1-D, 2-D, 3-D, colour-split int8) and graph lattices (open
+boundaries, triangular, a weighted graph with a self-loop and a parallel
+bond), for Ising and Potts, with several replicas at different
+temperatures, and the measured per-update energy and magnetisation equal the
+oracle's.  Statistics: Swendsen-Wang and Wolff chains of the 2-D Ising model
+at beta_c on a 32 x 32 torus reproduce Kaufman's exact finite-torus energy.
+Unsupported inputs (O(3) spins, J w < 0) are refused with EUNSUPPORTED, as
+the reference asserts (models.rs:139-142, 321-324).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import exact  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+LATTICES = {
+    "square16": (lambda m: m.Lattice.square(16, 16), None),
+    "cube": (lambda m: m.Lattice.hypercubic([8, 6, 4]), None),
+    "chain": (lambda m: m.Lattice.chain(14), None),
+    "open": (lambda m: m.Lattice.square(7, 5, pbc=False), None),
+    "tri": (lambda m: m.Lattice.triangular(6, 5), None),
+    "graph": (lambda m: m.Lattice.from_edges(6, [0, 0, 1, 2, 3, 4, 5, 2], [1, 1, 2, 2, 4, 5, 0, 4],
+                                             [1.0, 2.0, 1.0, 1.0, 3.0, 1.0, 2.0, 1.0]), None),
+}
+
+
+def _models(sc, orc, kind):
+    if kind == "ising":
+        return sc.model(sc.ISING, 1.0), orc.model(orc.ISING)
+    return sc.model(sc.POTTS, 1.0, 3), orc.model(orc.POTTS, 1.0, 3)
+
+
+@pytest.mark.parametrize("lat_name", list(LATTICES))
+@pytest.mark.parametrize("kind", ["ising", "potts"])
+@pytest.mark.parametrize("wolff", [False, True])
+def test_bit_exact_against_oracle(sc, orc, lat_name, kind, wolff):
+    make, _ = LATTICES[lat_name]
+    lat, olat = make(sc), make(orc)
+    m, om = _models(sc, orc, kind)
+    seed, n_rep, steps = 31, 3, 7
+    betas = [0.2, 0.44, 0.9] if kind == "ising" else [0.5, 1.0, 1.6]
+    print(f"case {lat_name} {kind} wolff={wolff}", flush=True)
+    st = sc.State(lat, m, n_rep, seed)
+    for r in range(n_rep):
+        st.init(True, r)
+        st.set_beta(betas[r], r)
+    start = [st.download_f64(r) for r in range(n_rep)]
+    st.sweep(sc.WOLFF if wolff else sc.SWENDSEN_WANG, steps, measure=True)
+    e_dev, m_dev = zip(*[st.series(r) for r in range(n_rep)])
+    for r in range(n_rep):
+        key = orc.replica_key(seed, r)
+        s = start[r].copy()
+        assert np.array_equal(s, orc.random_start(olat, om, key)), "random start differs"
+        es, ms = [], []
+        for t in range(steps):
+            orc.cluster_steps(olat, om, s, betas[r], key, t, 1, wolff)
+            es.append(orc.lib().orc_total_energy(orc.C.byref(om), olat.ptr, orc._p(s)))
+            ms.append(orc.lib().orc_magnetization(orc.C.byref(om), orc._p(s), olat.n_sites))
+        assert np.array_equal(st.download_f64(r), s), f"replica {r}: spins differ"
+        assert np.allclose(e_dev[r], es, rtol=0, atol=1e-9), (r, e_dev[r], es)
+        assert np.allclose(m_dev[r], ms, rtol=0, atol=1e-12), (r, m_dev[r], ms)
+    assert st.sweeps == steps
+
+
+def test_continues_after_metropolis(sc, orc):
+    """Cluster draws have their own streams: Metropolis sweeps, then cluster
+    updates numbered by the same counter, equal the oracle's sequence."""
+    lat, olat = sc.Lattice.square(16, 16), orc.Lattice.square(16, 16)
+    m, om = sc.model(sc.ISING, 1.0), orc.model(orc.ISING)
+    st = sc.State(lat, m, 1, 5)
+    st.init(True)
+    st.set_beta(0.44)
+    st.sweep(sc.METROPOLIS, 3)
+    s = st.download_f64()
+    st.sweep(sc.SWENDSEN_WANG, 4)
+    orc.cluster_steps(olat, om, s, 0.44, orc.replica_key(5, 0), 3, 4, False)
+    assert np.array_equal(st.download_f64(), s)
+
+
+@pytest.mark.parametrize("alg", ["sw", "wolff"])
+def test_energy_matches_kaufman_at_criticality(sc, alg):
+    L, beta = 32, 0.4406868
+    want = exact.kaufman_energy(L, L, beta)
+    lat = sc.Lattice.square(L, L)
+    st = sc.State(lat, sc.model(sc.ISING, 1.0), 4, 17)
+    for r in range(4):
+        st.init(True, r)
+        st.set_beta(beta, r)
+    a = sc.SWENDSEN_WANG if alg == "sw" else sc.WOLFF
+    st.sweep(a, 500 if alg == "sw" else 3000)
+    st.sweep(a, 6000 if alg == "sw" else 30000, measure=True)
+    zs = []
+    for r in range(4):
+        e, _ = st.series(r)
+        est = sc.binned_estimate(e, 200 if alg == "sw" else 1000)
+        zs.append((est.mean - want) / est.stderr)
+    print(alg, want, zs)
+    assert all(abs(z) < 4.5 for z in zs), zs
+
+
+def test_unsupported_inputs(sc):
+    on3 = sc.State(sc.Lattice.cubic(4), sc.model(sc.ON, 1.0, 2, 3), 1, 1)
+    with pytest.raises(sc.ScamperError) as err:
+        on3.sweep(sc.SWENDSEN_WANG, 1)
+    assert err.value.code == 5  # SCAMPER_EUNSUPPORTED
+    anti = sc.State(sc.Lattice.square(8, 8), sc.model(sc.ISING, -1.0), 1, 1)
+    with pytest.raises(sc.ScamperError) as err:
+        anti.sweep(sc.WOLFF, 1)
+    assert err.value.code == 5
+    neg = sc.State(sc.Lattice.from_edges(3, [0, 1], [1, 2], [1.0, -1.0]), sc.model(sc.ISING, 1.0), 1, 1)
+    with pytest.raises(sc.ScamperError) as err:
+        neg.sweep(sc.SWENDSEN_WANG, 1)
+    assert err.value.code == 5
