@@ -122,6 +122,135 @@ __global__ void k_bonds_csr(uint64_t n_edges, const ClusterEdge *__restrict__ ed
   }
 }
 
+// ------------------------------------------------ tile-local labelling
+// Stencils whose sides are multiples of the tile (2-D 32 x 32, 3-D 16 x 8 x 8
+// sites, 1024 per block): each block unites the bonds inside its tile in LDS
+// and writes every site's tile-local root (the tile's smallest member of its
+// local cluster, which in site-major order is also the smallest global
+// index); a second kernel unites the bonds that cross tile faces in global
+// memory.  The draws and the final partition are those of the global path.
+template <int ND>
+struct Tile {
+  static constexpr uint32_t X = ND == 2 ? 32 : 16, Y = ND == 2 ? 32 : 8, Z = ND == 2 ? 1 : 8;
+  static constexpr uint32_t N = X * Y * Z;  // 1024
+};
+
+__device__ __forceinline__ uint32_t load_lds(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ uint32_t find_lds(uint32_t *lp, uint32_t x) {
+  while (true) {
+    const uint32_t p = load_lds(lp + x);
+    if (p == x) return x;
+    const uint32_t gp = load_lds(lp + p);
+    if (gp == p) return p;
+    __hip_atomic_store(lp + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = gp;
+  }
+}
+
+__device__ void unite_lds(uint32_t *lp, uint32_t a, uint32_t b) {
+  while (true) {
+    a = find_lds(lp, a);
+    b = find_lds(lp, b);
+    if (a == b) return;
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint32_t old = atomicCAS(lp + a, a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+// Each thread takes G = 4 / gcd(ND, 4) consecutive sites along x (2 in 2-D,
+// 4 in 3-D), whose ND G bonds are the words of ND G / 4 whole Philox blocks
+// (edge e = ND s + a, G | s), so one block serves four bonds.
+template <int ND>
+__global__ __launch_bounds__(kThreads) void k_bonds_tile(const int8_t *__restrict__ spins, uint32_t *__restrict__ parent,
+                                                         uint32_t lx, uint32_t ly, uint64_t nh, uint64_t key,
+                                                         uint64_t step, uint64_t thr) {
+  using T = Tile<ND>;
+  constexpr uint32_t G = ND == 2 ? 2 : 4, NB = ND * G / 4;
+  __shared__ uint32_t lp[T::N];
+  __shared__ int8_t sp[T::N];
+  const uint32_t tiles_x = lx / T::X, tiles_y = ly / T::Y;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t ox = (tile % tiles_x) * T::X, oy = ((tile / tiles_x) % tiles_y) * T::Y,
+                 oz = (tile / (tiles_x * tiles_y)) * T::Z;
+  // tile-local index i = x + X (y + Y z)
+  auto global_of = [&](uint32_t i) -> uint32_t {
+    const uint32_t x = i % T::X, y = (i / T::X) % T::Y, z = i / (T::X * T::Y);
+    return ox + x + lx * (oy + y + ly * (oz + z));
+  };
+  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads) {
+    const uint32_t x = i % T::X, y = (i / T::X) % T::Y, z = i / (T::X * T::Y);
+    const uint32_t s = global_of(i);
+    sp[i] = spins[(uint64_t)((ox + x + oy + y + oz + z) & 1u) * nh + (s >> 1)];
+    lp[i] = i;
+  }
+  __syncthreads();
+  for (uint32_t g = threadIdx.x; g < T::N / G; g += kThreads) {
+    const uint32_t i0 = g * G, x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
+    const uint64_t s0 = global_of(i0);
+    uint32_t w[4 * NB];
+#pragma unroll
+    for (uint32_t k = 0; k < NB; ++k) {
+      const U4 b = cluster_block(key, (ND * s0) / 4 + k, step, kStreamClusterBond);
+      w[4 * k] = b.x, w[4 * k + 1] = b.y, w[4 * k + 2] = b.z, w[4 * k + 3] = b.w;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) {
+      const uint32_t i = i0 + u;
+#pragma unroll
+      for (int a = 0; a < ND; ++a) {
+        const bool inside = a == 0 ? x0 + u + 1 < T::X : (a == 1 ? y + 1 < T::Y : z + 1 < T::Z);
+        if (!inside) continue;
+        const uint32_t j = i + (a == 0 ? 1u : (a == 1 ? T::X : T::X * T::Y));
+        if (sp[j] == sp[i] && (uint64_t)w[u * ND + a] < thr) unite_lds(lp, i, j);
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads) parent[global_of(i)] = global_of(find_lds(lp, i));
+}
+
+// Bonds from each tile's last layer along every axis to the next tile
+// (periodic): X-face, Y-face (and Z-face) per tile, one thread each.
+template <int ND>
+__global__ void k_bonds_faces(const int8_t *__restrict__ spins, uint32_t *parent, uint32_t lx, uint32_t ly, uint32_t lz,
+                              uint64_t nh, uint64_t n_tiles, uint64_t key, uint64_t step, uint64_t thr) {
+  using T = Tile<ND>;
+  constexpr uint32_t fx = T::Y * T::Z, fy = T::X * T::Z, fz = ND == 3 ? T::X * T::Y : 0u;
+  constexpr uint32_t per_tile = fx + fy + fz;
+  const uint32_t tiles_x = lx / T::X, tiles_y = ly / T::Y;
+  for (uint64_t k = thread_id(); k < n_tiles * per_tile; k += grid_threads()) {
+    const uint32_t tile = (uint32_t)(k / per_tile), f = (uint32_t)(k - (uint64_t)tile * per_tile);
+    const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
+    uint32_t x, y, z, a;
+    if (f < fx) {  // +x bond of the tile's last column
+      a = 0, x = tx * T::X + T::X - 1, y = ty * T::Y + f % T::Y, z = tz * T::Z + f / T::Y;
+    } else if (f < fx + fy) {  // +y bond of its last row
+      const uint32_t g = f - fx;
+      a = 1, x = tx * T::X + g % T::X, y = ty * T::Y + T::Y - 1, z = tz * T::Z + g / T::X;
+    } else {  // +z bond of its last plane
+      const uint32_t g = f - fx - fy;
+      a = 2, x = tx * T::X + g % T::X, y = ty * T::Y + g / T::X, z = tz * T::Z + T::Z - 1;
+    }
+    const uint32_t x2 = a == 0 ? (x + 1 == lx ? 0u : x + 1) : x;
+    const uint32_t y2 = a == 1 ? (y + 1 == ly ? 0u : y + 1) : y;
+    const uint32_t z2 = a == 2 ? (z + 1 == lz ? 0u : z + 1) : z;
+    const uint64_t s = (uint64_t)x + (uint64_t)lx * ((uint64_t)y + (uint64_t)ly * z);
+    const uint64_t t = (uint64_t)x2 + (uint64_t)lx * ((uint64_t)y2 + (uint64_t)ly * z2);
+    if (spins[stencil_byte(s, StencilSite{x, y, z}, nh)] != spins[stencil_byte(t, StencilSite{x2, y2, z2}, nh)]) continue;
+    if ((uint64_t)cluster_word(key, (uint64_t)ND * s + a, step, kStreamClusterBond) < thr)
+      unite(parent, (uint32_t)s, (uint32_t)t);
+  }
+}
+
 __device__ __forceinline__ int8_t sw_target(uint32_t root, int kind, int q, uint64_t key, uint64_t step) {
   const uint32_t w = cluster_word(key, root, step, kStreamClusterTarget);
   if (kind == kIsing) return w < 0x80000000u ? (int8_t)1 : (int8_t)-1;
@@ -166,6 +295,46 @@ __global__ void k_transform(int8_t *__restrict__ spins, uint32_t *parent, int st
   }
 }
 
+// Stencils with 4 | lx: one thread per four consecutive sites along x.  Sites
+// s0 and s0 + 2 sit at bytes s0/2 and s0/2 + 1 of colour c = (y + z) & 1,
+// s0 + 1 and s0 + 3 at the same bytes of the other colour, so each colour
+// takes one 16-bit store.
+__global__ void k_transform_stencil(int8_t *__restrict__ spins, uint32_t *parent, uint32_t lx, uint32_t ly,
+                                    uint64_t nh, uint32_t n_sites, int kind, int q, int wolff, uint64_t key,
+                                    uint64_t step, const uint32_t *__restrict__ seed) {
+  const uint32_t seed_root = wolff ? seed[0] : 0u;
+  const int8_t wolff_target = wolff ? (int8_t)(uint8_t)seed[1] : (int8_t)0;
+  for (uint64_t g = thread_id(); g < n_sites / 4; g += grid_threads()) {
+    const uint32_t s0 = (uint32_t)g * 4, r = s0 / lx, c = (r % ly + r / ly) & 1u;
+    const uint4 p4 = *reinterpret_cast<const uint4 *>(parent + s0);
+    const uint32_t first[4] = {p4.x, p4.y, p4.z, p4.w};
+    int16_t *at_c = reinterpret_cast<int16_t *>(spins + (uint64_t)c * nh + s0 / 2);
+    int16_t *at_o = reinterpret_cast<int16_t *>(spins + (uint64_t)(c ^ 1u) * nh + s0 / 2);
+    uint32_t v[4];
+    bool any = !wolff;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t root = first[k] == s0 + k ? first[k] : find_root(parent, first[k]);
+      if (wolff) {
+        v[k] = root == seed_root ? (uint32_t)(uint8_t)wolff_target : 0x100u;  // 0x100: keep
+        any |= root == seed_root;
+      } else {
+        v[k] = (uint8_t)sw_target(root, kind, q, key, step);
+      }
+    }
+    if (!any) continue;
+    if (wolff) {
+      const uint32_t oc = (uint16_t)*at_c, oo = (uint16_t)*at_o;
+      v[0] = v[0] & 0x100u ? oc & 0xFFu : v[0];
+      v[2] = v[2] & 0x100u ? oc >> 8 : v[2];
+      v[1] = v[1] & 0x100u ? oo & 0xFFu : v[1];
+      v[3] = v[3] & 0x100u ? oo >> 8 : v[3];
+    }
+    *at_c = (int16_t)(v[0] | v[2] << 8);
+    *at_o = (int16_t)(v[1] | v[3] << 8);
+  }
+}
+
 uint32_t blocks_for(uint64_t n) {
   const uint64_t b = (n + kThreads - 1) / kThreads;
   return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
@@ -176,6 +345,23 @@ uint32_t blocks_for(uint64_t n) {
 hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spins, uint32_t *parent, uint64_t key,
                                         uint64_t step, uint64_t thr, hipStream_t stream) {
   const uint64_t n = 2 * g.nh;
+  const uint32_t lxs = 2 * g.lx2;
+  if (g.nd == 2 && lxs % Tile<2>::X == 0 && g.ly % Tile<2>::Y == 0) {
+    const uint64_t tiles = n / Tile<2>::N;
+    hipLaunchKernelGGL(k_bonds_tile<2>, dim3((uint32_t)tiles), dim3(kThreads), 0, stream, spins, parent, lxs, g.ly,
+                       g.nh, key, step, thr);
+    hipLaunchKernelGGL(k_bonds_faces<2>, dim3(blocks_for(tiles * 64)), dim3(kThreads), 0, stream, spins, parent, lxs,
+                       g.ly, g.lz, g.nh, tiles, key, step, thr);
+    return hipGetLastError();
+  }
+  if (g.nd == 3 && lxs % Tile<3>::X == 0 && g.ly % Tile<3>::Y == 0 && g.lz % Tile<3>::Z == 0) {
+    const uint64_t tiles = n / Tile<3>::N;
+    hipLaunchKernelGGL(k_bonds_tile<3>, dim3((uint32_t)tiles), dim3(kThreads), 0, stream, spins, parent, lxs, g.ly,
+                       g.nh, key, step, thr);
+    hipLaunchKernelGGL(k_bonds_faces<3>, dim3(blocks_for(tiles * 320)), dim3(kThreads), 0, stream, spins, parent, lxs,
+                       g.ly, g.lz, g.nh, tiles, key, step, thr);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n)), dim3(kThreads), 0, stream, parent, n);
   const uint32_t lx = 2 * g.lx2;
   if (g.nd == 1)
@@ -209,6 +395,11 @@ hipError_t launch_cluster_transform(const StencilGeom *g, uint64_t n_sites, int8
   if (wolff)
     hipLaunchKernelGGL(k_wolff_seed, dim3(1), dim3(64), 0, stream, spins, parent, stencil, lx, ly, nh, n_sites, kind,
                        q, key, step, scratch);
+  if (stencil && g->nd >= 2 && lx % 4 == 0) {
+    hipLaunchKernelGGL(k_transform_stencil, dim3(blocks_for(n_sites / 4)), dim3(kThreads), 0, stream, spins, parent,
+                       lx, ly, nh, (uint32_t)n_sites, kind, q, wolff, key, step, scratch);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_transform, dim3(blocks_for(n_sites)), dim3(kThreads), 0, stream, spins, parent, stencil, lx, ly,
                      nh, n_sites, kind, q, wolff, key, step, scratch);
   return hipGetLastError();

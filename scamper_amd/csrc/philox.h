@@ -102,6 +102,12 @@ SC_HD uint32_t word_of(const U4 &b, uint32_t k) {
 
 // Cluster-update draw for index idx of update `step` in stream s: word
 // idx & 3 of the block {idx >> 2, step, x(s), idx >> 34}.
+// The block holding the cluster words of indices 4 blk .. 4 blk + 3.
+SC_HD U4 cluster_block(uint64_t key, uint64_t blk, uint64_t step, uint32_t stream) {
+  return philox4x32_10((uint32_t)blk, (uint32_t)step, ctr2(step, 0, stream), (uint32_t)(blk >> 32), (uint32_t)key,
+                       (uint32_t)(key >> 32));
+}
+
 SC_HD uint32_t cluster_word(uint64_t key, uint64_t idx, uint64_t step, uint32_t stream) {
   const U4 b = philox4x32_10((uint32_t)(idx >> 2), (uint32_t)step, ctr2(step, 0, stream), (uint32_t)(idx >> 34),
                              (uint32_t)key, (uint32_t)(key >> 32));

@@ -2,7 +2,9 @@
 
 Parity: from the same spins and the same counter-based draws, k device
 updates leave exactly the spins of k oracle updates (orc_cluster_steps), on
-stencil lattices (1-D, 2-D, 3-D, colour-split int8) and graph lattices (open
+stencil lattices (1-D, 2-D, 3-D, colour-split int8; sides that are
+multiples of the tile labelling's 32 x 32 and 16 x 8 x 8 tiles, so that tile
+faces and periodic wraps inside one tile are crossed) and graph lattices (open
 boundaries, triangular, a weighted graph with a self-loop and a parallel
 bond), for Ising and Potts, with several replicas at different
 temperatures, and the measured per-update energy and magnetisation equal the
@@ -24,6 +26,10 @@ pytestmark = pytest.mark.gpu
 
 LATTICES = {
     "square16": (lambda m: m.Lattice.square(16, 16), None),
+    # sides that are multiples of the 32 x 32 and 16 x 8 x 8 labelling tiles
+    "square_tiles": (lambda m: m.Lattice.square(64, 96), None),
+    "cube_tiles": (lambda m: m.Lattice.hypercubic([16, 16, 8]), None),
+    "cube_tiles_wrap": (lambda m: m.Lattice.hypercubic([32, 8, 16]), None),
     "cube": (lambda m: m.Lattice.hypercubic([8, 6, 4]), None),
     "chain": (lambda m: m.Lattice.chain(14), None),
     "open": (lambda m: m.Lattice.square(7, 5, pbc=False), None),
