@@ -166,56 +166,81 @@ __device__ void unite_lds(uint32_t *lp, uint32_t a, uint32_t b) {
   }
 }
 
-// Each thread takes G = 4 / gcd(ND, 4) consecutive sites along x (2 in 2-D,
-// 4 in 3-D), whose ND G bonds are the words of ND G / 4 whole Philox blocks
-// (edge e = ND s + a, G | s), so one block serves four bonds.
+// One thread per quad of four consecutive sites along x (a tile row is X / 4
+// quads).  The quad's 4 ND bonds (edge e = ND s + a, 4 | s) are the words of
+// ND whole Philox blocks, and its spins are two 16-bit loads: sites s0 and
+// s0 + 2 sit at bytes s0/2 and s0/2 + 1 of one colour, s0 + 1 and s0 + 3 at the
+// same bytes of the other.  Bonds along x need no union-find: each site's
+// parent is the first site of its run of active x bonds, found from the row's
+// bond mask with one count of leading zeros, which is already a forest of
+// depth 1 rooted at each run's smallest site.  Bonds along y (and z) then
+// unite runs in LDS.
 template <int ND>
 __global__ __launch_bounds__(kThreads) void k_bonds_tile(const int8_t *__restrict__ spins, uint32_t *__restrict__ parent,
                                                          uint32_t lx, uint32_t ly, uint64_t nh, uint64_t key,
                                                          uint64_t step, uint64_t thr) {
   using T = Tile<ND>;
-  constexpr uint32_t G = ND == 2 ? 2 : 4, NB = ND * G / 4;
+  static_assert(T::N == 4 * kThreads, "one quad per thread");
+  constexpr uint32_t QR = T::X / 4;  // quads per tile row
   __shared__ uint32_t lp[T::N];
-  __shared__ int8_t sp[T::N];
+  __shared__ uint32_t sp4[kThreads];  // the quads' spins, one byte per site
+  __shared__ uint8_t xb[kThreads];    // the quads' x-bond nibbles
   const uint32_t tiles_x = lx / T::X, tiles_y = ly / T::Y;
   const uint32_t tile = blockIdx.x;
   const uint32_t ox = (tile % tiles_x) * T::X, oy = ((tile / tiles_x) % tiles_y) * T::Y,
                  oz = (tile / (tiles_x * tiles_y)) * T::Z;
-  // tile-local index i = x + X (y + Y z)
-  auto global_of = [&](uint32_t i) -> uint32_t {
-    const uint32_t x = i % T::X, y = (i / T::X) % T::Y, z = i / (T::X * T::Y);
-    return ox + x + lx * (oy + y + ly * (oz + z));
-  };
-  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads) {
-    const uint32_t x = i % T::X, y = (i / T::X) % T::Y, z = i / (T::X * T::Y);
-    const uint32_t s = global_of(i);
-    sp[i] = spins[(uint64_t)((ox + x + oy + y + oz + z) & 1u) * nh + (s >> 1)];
-    lp[i] = i;
+  const uint32_t q = threadIdx.x, i0 = 4 * q;
+  const uint32_t x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
+  const uint32_t s0 = ox + x0 + lx * (oy + y + ly * (oz + z));
+  const uint32_t c = (ox + x0 + oy + y + oz + z) & 1u;
+  const uint32_t own = (uint16_t)*reinterpret_cast<const int16_t *>(spins + (uint64_t)c * nh + s0 / 2);
+  const uint32_t oth = (uint16_t)*reinterpret_cast<const int16_t *>(spins + (uint64_t)(c ^ 1u) * nh + s0 / 2);
+  uint32_t w[4 * ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const U4 blk = cluster_block(key, (uint64_t)ND * s0 / 4 + k, step, kStreamClusterBond);
+    w[4 * k] = blk.x, w[4 * k + 1] = blk.y, w[4 * k + 2] = blk.z, w[4 * k + 3] = blk.w;
+  }
+  // bytes of sites s0 .. s0 + 3
+  sp4[q] = (own & 0xFFu) | (oth & 0xFFu) << 8 | (own >> 8) << 16 | (oth >> 8) << 24;
+  __syncthreads();
+  const uint8_t *sp = reinterpret_cast<const uint8_t *>(sp4);
+  auto bond = [&](uint32_t u, int a) { return (uint64_t)w[u * ND + a] < thr; };
+  uint32_t xm = 0;  // bit u: the bond from site u to u + 1 is active and inside the tile
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    if (x0 + u + 1 < T::X && sp[i0 + u] == sp[i0 + u + 1] && bond(u, 0)) xm |= 1u << u;
+  }
+  xb[q] = (uint8_t)xm;
+  __syncthreads();
+  // the row's x-bond mask, bit x for the bond from x to x + 1
+  const uint32_t r0 = q - q % QR;
+  uint32_t row = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < QR; ++k) row |= (uint32_t)xb[r0 + k] << (4 * k);
+  const uint32_t starts = ~(row << 1);  // bit x: no active bond from x - 1 to x
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t x = x0 + u;
+    const uint32_t first = 31u - (uint32_t)__builtin_clz(starts & (0xFFFFFFFFu >> (31u - x)));
+    lp[i0 + u] = i0 - x0 + first;
   }
   __syncthreads();
-  for (uint32_t g = threadIdx.x; g < T::N / G; g += kThreads) {
-    const uint32_t i0 = g * G, x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
-    const uint64_t s0 = global_of(i0);
-    uint32_t w[4 * NB];
 #pragma unroll
-    for (uint32_t k = 0; k < NB; ++k) {
-      const U4 b = cluster_block(key, (ND * s0) / 4 + k, step, kStreamClusterBond);
-      w[4 * k] = b.x, w[4 * k + 1] = b.y, w[4 * k + 2] = b.z, w[4 * k + 3] = b.w;
-    }
-#pragma unroll
-    for (uint32_t u = 0; u < G; ++u) {
-      const uint32_t i = i0 + u;
-#pragma unroll
-      for (int a = 0; a < ND; ++a) {
-        const bool inside = a == 0 ? x0 + u + 1 < T::X : (a == 1 ? y + 1 < T::Y : z + 1 < T::Z);
-        if (!inside) continue;
-        const uint32_t j = i + (a == 0 ? 1u : (a == 1 ? T::X : T::X * T::Y));
-        if (sp[j] == sp[i] && (uint64_t)w[u * ND + a] < thr) unite_lds(lp, i, j);
-      }
-    }
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t i = i0 + u;
+    if (y + 1 < T::Y && sp[i] == sp[i + T::X] && bond(u, 1)) unite_lds(lp, i, i + T::X);
+    if (ND == 3 && z + 1 < T::Z && sp[i] == sp[i + T::X * T::Y] && bond(u, ND - 1))
+      unite_lds(lp, i, i + T::X * T::Y);
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads) parent[global_of(i)] = global_of(find_lds(lp, i));
+  uint32_t r[4];
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t l = find_lds(lp, i0 + u);
+    r[u] = ox + l % T::X + lx * (oy + (l / T::X) % T::Y + ly * (oz + l / (T::X * T::Y)));
+  }
+  *reinterpret_cast<uint4 *>(parent + s0) = make_uint4(r[0], r[1], r[2], r[3]);
 }
 
 // Bonds from each tile's last layer along every axis to the next tile
@@ -246,8 +271,10 @@ __global__ void k_bonds_faces(const int8_t *__restrict__ spins, uint32_t *parent
     const uint64_t s = (uint64_t)x + (uint64_t)lx * ((uint64_t)y + (uint64_t)ly * z);
     const uint64_t t = (uint64_t)x2 + (uint64_t)lx * ((uint64_t)y2 + (uint64_t)ly * z2);
     if (spins[stencil_byte(s, StencilSite{x, y, z}, nh)] != spins[stencil_byte(t, StencilSite{x2, y2, z2}, nh)]) continue;
+    // start from the tile-local roots, so that the entry of a site that is
+    // not a local root keeps pointing at its own tile's local root
     if ((uint64_t)cluster_word(key, (uint64_t)ND * s + a, step, kStreamClusterBond) < thr)
-      unite(parent, (uint32_t)s, (uint32_t)t);
+      unite(parent, load_parent(parent + s), load_parent(parent + t));
   }
 }
 
@@ -335,9 +362,95 @@ __global__ void k_transform_stencil(int8_t *__restrict__ spins, uint32_t *parent
   }
 }
 
+// The transform after tile labelling, one block per tile.  A site that is not
+// a local root points at its tile's local root (the face merges start from
+// local roots and never rewrite such entries), so the block resolves each
+// local root once -- one global root walk and one target -- and every site
+// reads its local root's result from LDS.  A local root linked to another
+// tile's root walks on its own.
+template <int ND>
+__global__ __launch_bounds__(kThreads) void k_transform_tile(int8_t *__restrict__ spins, uint32_t *parent,
+                                                             uint32_t lx, uint32_t ly, uint64_t nh, uint64_t div_lx2,
+                                                             uint64_t div_ly, int kind, int q, int wolff, uint64_t key,
+                                                             uint64_t step, const uint32_t *__restrict__ seed) {
+  using T = Tile<ND>;
+  constexpr uint32_t kKeep = 0x100u;  // Wolff: the site keeps its state
+  __shared__ uint16_t res[T::N];
+  __shared__ uint8_t need[T::N];
+  const uint32_t seed_root = wolff ? seed[0] : 0u;
+  const uint32_t wolff_target = wolff ? seed[1] & 0xFFu : 0u;
+  const uint32_t tiles_x = lx / T::X, tiles_y = ly / T::Y;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t ox = (tile % tiles_x) * T::X, oy = ((tile / tiles_x) % tiles_y) * T::Y,
+                 oz = (tile / (tiles_x * tiles_y)) * T::Z;
+  auto global_of = [&](uint32_t i) -> uint32_t {
+    const uint32_t x = i % T::X, y = (i / T::X) % T::Y, z = i / (T::X * T::Y);
+    return ox + x + lx * (oy + y + ly * (oz + z));
+  };
+  // tile-local index of global site g, or T::N when g lies in another tile
+  auto local_of = [&](uint32_t g) -> uint32_t {
+    const uint32_t row = magic_div(g >> 1, div_lx2), x = g - row * lx;
+    const uint32_t z = magic_div(row, div_ly), y = row - z * ly;
+    const uint32_t dx = x - ox, dy = y - oy, dz = z - oz;
+    return dx < T::X && dy < T::Y && dz < T::Z ? dx + T::X * (dy + T::Y * dz) : T::N;
+  };
+  auto result_of = [&](uint32_t root) -> uint32_t {
+    if (wolff) return root == seed_root ? wolff_target : kKeep;
+    return (uint8_t)sw_target(root, kind, q, key, step);
+  };
+  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads) need[i] = 0;
+  __syncthreads();
+  // four consecutive sites along x per thread (4 | X)
+  const uint32_t i0 = 4 * threadIdx.x;
+  const uint32_t s0 = global_of(i0);
+  const uint4 p4 = *reinterpret_cast<const uint4 *>(parent + s0);
+  const uint32_t p[4] = {p4.x, p4.y, p4.z, p4.w};
+  uint32_t loc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    loc[k] = local_of(p[k]);
+    if (loc[k] < T::N) need[loc[k]] = 1;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads)
+    if (need[i]) res[i] = (uint16_t)result_of(find_root(parent, global_of(i)));
+  __syncthreads();
+  uint32_t v[4];
+  bool any = !wolff;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = loc[k] < T::N ? res[loc[k]] : result_of(find_root(parent, p[k]));
+    any |= v[k] != kKeep;
+  }
+  if (!any) return;
+  // sites s0, s0 + 2 of colour c at bytes s0/2, s0/2 + 1; s0 + 1, s0 + 3 of the other
+  const uint32_t x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
+  const uint32_t c = (ox + x0 + oy + y + oz + z) & 1u;
+  int16_t *at_c = reinterpret_cast<int16_t *>(spins + (uint64_t)c * nh + s0 / 2);
+  int16_t *at_o = reinterpret_cast<int16_t *>(spins + (uint64_t)(c ^ 1u) * nh + s0 / 2);
+  if (wolff) {
+    const uint32_t oc = (uint16_t)*at_c, oo = (uint16_t)*at_o;
+    v[0] = v[0] == kKeep ? oc & 0xFFu : v[0];
+    v[2] = v[2] == kKeep ? oc >> 8 : v[2];
+    v[1] = v[1] == kKeep ? oo & 0xFFu : v[1];
+    v[3] = v[3] == kKeep ? oo >> 8 : v[3];
+  }
+  *at_c = (int16_t)(v[0] | v[2] << 8);
+  *at_o = (int16_t)(v[1] | v[3] << 8);
+}
+
 uint32_t blocks_for(uint64_t n) {
   const uint64_t b = (n + kThreads - 1) / kThreads;
   return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
+// Stencils labelled tile by tile: 2-D or 3-D with every side a multiple of
+// the tile's.
+bool tiled(const StencilGeom &g) {
+  const uint32_t lx = 2 * g.lx2;
+  if (g.nd == 2) return lx % Tile<2>::X == 0 && g.ly % Tile<2>::Y == 0;
+  if (g.nd == 3) return lx % Tile<3>::X == 0 && g.ly % Tile<3>::Y == 0 && g.lz % Tile<3>::Z == 0;
+  return false;
 }
 
 }  // namespace
@@ -346,7 +459,7 @@ hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spin
                                         uint64_t step, uint64_t thr, hipStream_t stream) {
   const uint64_t n = 2 * g.nh;
   const uint32_t lxs = 2 * g.lx2;
-  if (g.nd == 2 && lxs % Tile<2>::X == 0 && g.ly % Tile<2>::Y == 0) {
+  if (g.nd == 2 && tiled(g)) {
     const uint64_t tiles = n / Tile<2>::N;
     hipLaunchKernelGGL(k_bonds_tile<2>, dim3((uint32_t)tiles), dim3(kThreads), 0, stream, spins, parent, lxs, g.ly,
                        g.nh, key, step, thr);
@@ -354,7 +467,7 @@ hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spin
                        g.ly, g.lz, g.nh, tiles, key, step, thr);
     return hipGetLastError();
   }
-  if (g.nd == 3 && lxs % Tile<3>::X == 0 && g.ly % Tile<3>::Y == 0 && g.lz % Tile<3>::Z == 0) {
+  if (g.nd == 3 && tiled(g)) {
     const uint64_t tiles = n / Tile<3>::N;
     hipLaunchKernelGGL(k_bonds_tile<3>, dim3((uint32_t)tiles), dim3(kThreads), 0, stream, spins, parent, lxs, g.ly,
                        g.nh, key, step, thr);
@@ -395,6 +508,16 @@ hipError_t launch_cluster_transform(const StencilGeom *g, uint64_t n_sites, int8
   if (wolff)
     hipLaunchKernelGGL(k_wolff_seed, dim3(1), dim3(64), 0, stream, spins, parent, stencil, lx, ly, nh, n_sites, kind,
                        q, key, step, scratch);
+  if (stencil && tiled(*g)) {
+    const uint32_t tiles = (uint32_t)(n_sites / Tile<2>::N);
+    if (g->nd == 2)
+      hipLaunchKernelGGL(k_transform_tile<2>, dim3(tiles), dim3(kThreads), 0, stream, spins, parent, lx, ly, nh,
+                         g->div_lx2, g->div_ly, kind, q, wolff, key, step, scratch);
+    else
+      hipLaunchKernelGGL(k_transform_tile<3>, dim3(tiles), dim3(kThreads), 0, stream, spins, parent, lx, ly, nh,
+                         g->div_lx2, g->div_ly, kind, q, wolff, key, step, scratch);
+    return hipGetLastError();
+  }
   if (stencil && g->nd >= 2 && lx % 4 == 0) {
     hipLaunchKernelGGL(k_transform_stencil, dim3(blocks_for(n_sites / 4)), dim3(kThreads), 0, stream, spins, parent,
                        lx, ly, nh, (uint32_t)n_sites, kind, q, wolff, key, step, scratch);
