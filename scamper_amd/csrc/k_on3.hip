@@ -98,12 +98,68 @@ __device__ __forceinline__ V3 local_field(const float *__restrict__ oth, const S
 // colour-1 sites after their update, E = -J sum s.h and the magnetisation
 // of both colours (each other-colour site is the centre neighbour of
 // exactly one colour-1 site).
+// One heat-bath draw (heat_bath_sample_site, models.rs:617-662): cos(theta)
+// about the local field h from the word w0, the azimuth from w1, rotated
+// into h's frame.  A vanishing field draws a uniform direction from a block
+// of its own (stream kStreamContFallback, by rank).
+__device__ __forceinline__ V3 heat_bath_site(V3 h, const ReplicaParams &P, uint64_t j, uint64_t sweep,
+                                             uint32_t colour, uint32_t w0, uint32_t w1) {
+  const float beta = (float)P.beta;
+  const float norm = sqrtf(h.x * h.x + h.y * h.y + h.z * h.z);
+  const float kappa = beta * norm;
+  if (kappa < 1e-12f)
+    return random_unit(philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
+                                     (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32)));
+  const float u = fmaxf((float)w0 * 2.3283064365386963e-10f, 1.17549435e-38f);
+  float c = 1.0f + log1pf((1.0f - u) * expm1f(-2.0f * kappa)) / kappa;
+  c = fminf(fmaxf(c, -1.0f), 1.0f);
+  const float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+  float sphi, cphi;
+  sincos_turns((float)w1 * 2.3283064365386963e-10f, &sphi, &cphi);
+  const float lx = sn * cphi, ly = sn * sphi;
+  const float inv_n = 1.0f / norm;
+  const float ux = h.x * inv_n, uy = h.y * inv_n, uz = h.z * inv_n;
+  const float tr = sqrtf(ux * ux + uy * uy);
+  if (tr < 1e-6f) return normalised(V3{lx, ly, uz >= 0.0f ? c : -c});
+  const float it = 1.0f / tr;
+  return normalised(V3{-uy * it * lx - ux * uz * it * ly + ux * c, ux * it * lx - uy * uz * it * ly + uy * c,
+                       tr * ly + uz * c});
+}
+
+// Over-relaxation (MicrocanonicalCore, microcanonical.rs:54-75): s reflected
+// about h, unchanged where h vanishes.
+__device__ __forceinline__ V3 reflect_site(V3 s, V3 h) {
+  const float nsq = h.x * h.x + h.y * h.y + h.z * h.z;
+  if (!(nsq >= 1e-28f)) return s;
+  const float proj = (s.x * h.x + s.y * h.y + s.z * h.z) / nsq;
+  return normalised(V3{2.0f * proj * h.x - s.x, 2.0f * proj * h.y - s.y, 2.0f * proj * h.z - s.z});
+}
+
+// The grid is one-dimensional over (replica, block of ranks), nbx blocks per
+// replica.  Hardware hands consecutive blocks to the eight XCDs in turn, so
+// block b works on item (b % 8) (total / 8) + b / 8 when 8 | total: each XCD
+// sweeps one contiguous eighth, and the neighbour rows and planes a block
+// reads sit in its own XCD's L2.
+// (A grid too large for one dimension keeps (block, replica) in x and y.)
+__device__ __forceinline__ void xcd_item(uint32_t nbx, uint32_t *rep, uint32_t *bx) {
+  if (gridDim.y > 1) {
+    *rep = blockIdx.y;
+    *bx = blockIdx.x;
+    return;
+  }
+  const uint32_t total = gridDim.x, b = blockIdx.x;
+  const uint32_t item = total % 8 == 0 ? (b % 8) * (total / 8) + b / 8 : b;
+  *rep = item / nbx;
+  *bx = item - *rep * nbx;
+}
+
 template <int ND, int ALG, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_on3(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
-                                                StencilGeom g, uint32_t colour, uint64_t sweep, float jc,
-                                                double *__restrict__ em) {
-  const uint32_t rep = blockIdx.y;
-  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+                                                StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
+                                                float jc, double *__restrict__ em) {
+  uint32_t rep, bx;
+  xcd_item(nbx, &rep, &bx);
+  const uint64_t j = (uint64_t)bx * kBlock + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (j < g.nh) {
     float *__restrict__ own = spins + (2ull * rep + colour) * 3ull * g.nh;
@@ -114,44 +170,14 @@ __global__ __launch_bounds__(kBlock) void k_on3(float *__restrict__ spins, const
     V3 centre{0.f, 0.f, 0.f};
     const V3 h = local_field<ND>(oth, g, j, row, i, y, zz, p, jc, &centre);
     const V3 s = load3(own, g.nh, j);
-    V3 ns = s;
+    V3 ns;
     if (ALG == kContinuousHeatBath) {
       const ReplicaParams &P = reps[rep];
-      const float beta = (float)P.beta;
-      const float norm = sqrtf(h.x * h.x + h.y * h.y + h.z * h.z);
-      const float kappa = beta * norm;
-      if (kappa < 1e-12f) {
-        ns = random_unit(philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
-                                       (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32)));
-      } else {
-        const U4 b = philox4x32_10((uint32_t)(j >> 1), (uint32_t)sweep, ctr2(sweep, colour, kStreamCont),
+      const U4 blk = philox4x32_10((uint32_t)(j >> 1), (uint32_t)sweep, ctr2(sweep, colour, kStreamCont),
                                    (uint32_t)(j >> 33), (uint32_t)P.key, (uint32_t)(P.key >> 32));
-        const uint32_t w0 = (j & 1) ? b.z : b.x, w1 = (j & 1) ? b.w : b.y;
-        const float u = fmaxf((float)w0 * 2.3283064365386963e-10f, 1.17549435e-38f);
-        float c = 1.0f + log1pf((1.0f - u) * expm1f(-2.0f * kappa)) / kappa;
-        c = fminf(fmaxf(c, -1.0f), 1.0f);
-        const float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
-        float sphi, cphi;
-        sincos_turns((float)w1 * 2.3283064365386963e-10f, &sphi, &cphi);
-        const float lx = sn * cphi, ly = sn * sphi;
-        const float inv_n = 1.0f / norm;
-        const float ux = h.x * inv_n, uy = h.y * inv_n, uz = h.z * inv_n;
-        const float tr = sqrtf(ux * ux + uy * uy);
-        if (tr < 1e-6f) {
-          ns = V3{lx, ly, uz >= 0.0f ? c : -c};
-        } else {
-          const float it = 1.0f / tr;
-          ns = V3{-uy * it * lx - ux * uz * it * ly + ux * c, ux * it * lx - uy * uz * it * ly + uy * c,
-                  tr * ly + uz * c};
-        }
-        ns = normalised(ns);
-      }
+      ns = heat_bath_site(h, P, j, sweep, colour, (j & 1) ? blk.z : blk.x, (j & 1) ? blk.w : blk.y);
     } else {
-      const float nsq = h.x * h.x + h.y * h.y + h.z * h.z;
-      if (nsq >= 1e-28f) {
-        const float proj = (s.x * h.x + s.y * h.y + s.z * h.z) / nsq;
-        ns = normalised(V3{2.0f * proj * h.x - s.x, 2.0f * proj * h.y - s.y, 2.0f * proj * h.z - s.z});
-      }
+      ns = reflect_site(s, h);
     }
     store3(own, g.nh, j, ns);
     if (MEAS) {
@@ -159,6 +185,98 @@ __global__ __launch_bounds__(kBlock) void k_on3(float *__restrict__ spins, const
       acc[1] = (double)ns.x + (double)centre.x;
       acc[2] = (double)ns.y + (double)centre.y;
       acc[3] = (double)ns.z + (double)centre.z;
+    }
+  }
+  if (MEAS) {
+    __shared__ double red[4];
+    if (threadIdx.x < 4) red[threadIdx.x] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if ((threadIdx.x & 63) == 0) atomicAdd(&red[k], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) atomicAdd(&em[(uint64_t)rep * kEmStride + threadIdx.x], red[threadIdx.x]);
+  }
+}
+
+// Two ranks per thread, j = 2t and j + 1, when Lx/2 is even (so both lie in
+// one row and every pair is 8-byte aligned): one Philox block (j >> 1) serves
+// both heat-bath draws, each component of a neighbour row is one float2
+// load, and the left/right neighbours are that row's pair plus one more rank
+// (i + 2 when the row's parity p = 1, else i - 1).
+template <int ND, int ALG, bool MEAS>
+__global__ __launch_bounds__(kBlock) void k_on3_pair(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
+                                                     StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
+                                                     float jc, double *__restrict__ em) {
+  uint32_t rep, bx;
+  xcd_item(nbx, &rep, &bx);
+  const uint64_t t = (uint64_t)bx * kBlock + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (t < g.nh / 2) {
+    const uint64_t j = 2 * t;
+    float *__restrict__ own = spins + (2ull * rep + colour) * 3ull * g.nh;
+    const float *__restrict__ oth = spins + (2ull * rep + 1 - colour) * 3ull * g.nh;
+    const SiteCoords c = site_coords(j, g);
+    const uint32_t p = (colour + c.y + c.z) & 1u;
+    const uint64_t rb = (uint64_t)c.row * g.lx2;
+    const uint64_t ex = rb + (p ? wrap_inc(c.i + 1, g.lx2) : wrap_dec(c.i, g.lx2));
+    uint64_t nb[2 * ND > 2 ? 2 * ND - 2 : 1];  // first ranks of the neighbour rows' pairs: up, down, back, front
+    if (ND >= 2) {
+      nb[0] = ((uint64_t)(wrap_dec(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+      nb[1] = ((uint64_t)(wrap_inc(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+    }
+    if (ND >= 3) {
+      nb[2] = ((uint64_t)(c.y + g.ly * wrap_dec(c.z, g.lz))) * g.lx2 + c.i;
+      nb[3] = ((uint64_t)(c.y + g.ly * wrap_inc(c.z, g.lz))) * g.lx2 + c.i;
+    }
+    float ha[3], hb[3], ca[3], cb[3], sa[3], sb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float *base = oth + (uint64_t)k * g.nh;
+      const float2 mid = *reinterpret_cast<const float2 *>(base + j);
+      const float e = base[ex];
+      // left then right, as nb[0], nb[1] of the one-rank kernel
+      float a = p ? mid.x : e, b = p ? mid.y : mid.x;
+      a += p ? mid.y : mid.x;
+      b += p ? e : mid.y;
+#pragma unroll
+      for (int n = 0; n < 2 * ND - 2; ++n) {
+        const float2 v = *reinterpret_cast<const float2 *>(base + nb[n]);
+        a += v.x;
+        b += v.y;
+      }
+      ha[k] = jc * a;
+      hb[k] = jc * b;
+      ca[k] = mid.x;
+      cb[k] = mid.y;
+      const float2 sv = *reinterpret_cast<const float2 *>(own + (uint64_t)k * g.nh + j);
+      sa[k] = sv.x;
+      sb[k] = sv.y;
+    }
+    const V3 hA{ha[0], ha[1], ha[2]}, hB{hb[0], hb[1], hb[2]};
+    V3 nA, nB;
+    if (ALG == kContinuousHeatBath) {
+      const ReplicaParams &P = reps[rep];
+      const U4 blk = philox4x32_10((uint32_t)t, (uint32_t)sweep, ctr2(sweep, colour, kStreamCont), (uint32_t)(t >> 32),
+                                   (uint32_t)P.key, (uint32_t)(P.key >> 32));
+      nA = heat_bath_site(hA, P, j, sweep, colour, blk.x, blk.y);
+      nB = heat_bath_site(hB, P, j + 1, sweep, colour, blk.z, blk.w);
+    } else {
+      nA = reflect_site(V3{sa[0], sa[1], sa[2]}, hA);
+      nB = reflect_site(V3{sb[0], sb[1], sb[2]}, hB);
+    }
+    *reinterpret_cast<float2 *>(own + j) = make_float2(nA.x, nB.x);
+    *reinterpret_cast<float2 *>(own + g.nh + j) = make_float2(nA.y, nB.y);
+    *reinterpret_cast<float2 *>(own + 2 * g.nh + j) = make_float2(nA.z, nB.z);
+    if (MEAS) {
+      acc[0] = (double)nA.x * (double)hA.x + (double)nA.y * (double)hA.y + (double)nA.z * (double)hA.z +
+               ((double)nB.x * (double)hB.x + (double)nB.y * (double)hB.y + (double)nB.z * (double)hB.z);
+      acc[1] = ((double)nA.x + (double)ca[0]) + ((double)nB.x + (double)cb[0]);
+      acc[2] = ((double)nA.y + (double)ca[1]) + ((double)nB.y + (double)cb[1]);
+      acc[3] = ((double)nA.z + (double)ca[2]) + ((double)nB.z + (double)cb[2]);
     }
   }
   if (MEAS) {
@@ -227,12 +345,24 @@ __global__ __launch_bounds__(kBlock) void k_on3_init(float *__restrict__ spins, 
 
 hipError_t launch_on3_sweep(const StencilGeom &g, int alg, float *spins, const ReplicaParams *reps, uint32_t n_rep,
                             uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t st) {
-  dim3 grid((unsigned)((g.nh + kBlock - 1) / kBlock), n_rep);
-#define SC_ON3(ND, A)                                                                                               \
-  if (em)                                                                                                           \
-    hipLaunchKernelGGL((k_on3<ND, A, true>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, j, em);      \
-  else                                                                                                              \
-    hipLaunchKernelGGL((k_on3<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, j, em);
+  // two ranks per thread for the heat bath (one Philox block for both);
+  // over-relaxation has no draws to share and runs faster one rank per thread
+  const bool pairs = alg == kContinuousHeatBath && g.lx2 % 2 == 0;
+  const uint64_t per_rep = pairs ? g.nh / 2 : g.nh;
+  const uint32_t nbx = (uint32_t)((per_rep + kBlock - 1) / kBlock);
+  const bool flat = (uint64_t)nbx * n_rep * kBlock < (1ull << 32);
+  const dim3 grid = flat ? dim3(nbx * n_rep) : dim3(nbx, n_rep);
+#define SC_ON3_K(K, ND, A)                                                                                             \
+  if (em)                                                                                                              \
+    hipLaunchKernelGGL((K<ND, A, true>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);        \
+  else                                                                                                                 \
+    hipLaunchKernelGGL((K<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);
+#define SC_ON3(ND, A)              \
+  if (pairs) {                     \
+    SC_ON3_K(k_on3_pair, ND, A)    \
+  } else {                         \
+    SC_ON3_K(k_on3, ND, A)         \
+  }
 #define SC_ON3_ND(ND)                                   \
   if (alg == kContinuousHeatBath) { SC_ON3(ND, kContinuousHeatBath) } \
   else { SC_ON3(ND, kOverRelaxation) }
@@ -244,6 +374,7 @@ hipError_t launch_on3_sweep(const StencilGeom &g, int alg, float *spins, const R
   }
 #undef SC_ON3_ND
 #undef SC_ON3
+#undef SC_ON3_K
   return hipGetLastError();
 }
 
