@@ -17,7 +17,7 @@ Other workloads, one JSON line each (not the headline):
     --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
     --workload potts3d  the tiled Potts kernel alone: q=3, 3-D L=256, measured sweeps
     --workload cluster  Swendsen-Wang and Wolff updates of the 2-D Ising model at
-                        beta_c, L=4096
+                        beta_c, L=4096, and of the 3-D O(3) model near T_c, L=96
     --workload serve  the serving block alone
     --workload calibrate  kernels of known traffic for scaling rocprofv3 counters
 
@@ -238,8 +238,9 @@ def potts3d(sc, args) -> dict:
 
 def cluster(sc, args) -> dict:
     """Swendsen-Wang and Wolff updates of the 2-D Ising model at beta_c on a
-    4096 x 4096 torus: bond activation, lock-free union-find labelling and the
-    cluster transform, per update over every site."""
+    4096 x 4096 torus (bond activation, tile and face labelling, the cluster
+    transform, per update over every site), and of the 3-D Heisenberg model
+    (O(3) embedding clusters) on a 96^3 torus near T_c."""
     L, beta = 4096, 0.4406868
     lat = sc.Lattice.square(L, L, True)
     n = lat.n_sites
@@ -258,6 +259,24 @@ def cluster(sc, args) -> dict:
         out[name] = {"updates": steps, "ms_per_update": round(ms / steps, 4),
                      "site_updates_per_ns": round(n * steps / (ms * 1e6), 3),
                      "energy_per_site": e / n, "abs_m": m}
+        st.close()
+    # O(3) embedding clusters: 3-D Heisenberg L=96 near its critical point
+    # (T_c ~ 1.443), one replica
+    L3, beta3 = 96, 1 / 1.443
+    lat3 = sc.Lattice.hypercubic([L3, L3, L3], True)
+    n3 = lat3.n_sites
+    for name, alg in (("on3_swendsen_wang", sc.SWENDSEN_WANG), ("on3_wolff", sc.WOLFF)):
+        st = sc.State(lat3, sc.model(sc.ON, 1.0, 2, 3), 1, 42)
+        st.init(True)
+        st.set_beta(beta3)
+        st.sweep(alg, 10)
+        st.synchronize()
+        steps = max(1, args.cluster_steps // 4)
+        ms = st.sweep_timed(alg, steps)
+        e, m = st.observables()
+        out[name] = {"L": L3, "beta": round(beta3, 6), "updates": steps, "ms_per_update": round(ms / steps, 4),
+                     "site_updates_per_ns": round(n3 * steps / (ms * 1e6), 3), "energy_per_site": e / n3,
+                     "abs_m": m}
         st.close()
     out["value"] = out["swendsen_wang"]["site_updates_per_ns"]
     return out

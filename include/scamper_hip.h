@@ -133,8 +133,8 @@ enum {
   SCAMPER_CONT_HEAT_BATH = 2,    /* ContinuousHeatBathCore (heat_bath.rs:75-134) */
   SCAMPER_OVER_RELAXATION = 3,   /* MicrocanonicalCore (algorithms/microcanonical.rs:13-85) */
   SCAMPER_HYBRID_HB_OR = 4,      /* HybridCore<ContinuousHeatBath, Microcanonical> (hybrid.rs:10-64) */
-  SCAMPER_SWENDSEN_WANG = 5,     /* SWCore (algorithms/swendsen_wang.rs:12-144), Ising and Potts */
-  SCAMPER_WOLFF = 6,             /* WolffCore (algorithms/wolff.rs:12-141), Ising and Potts */
+  SCAMPER_SWENDSEN_WANG = 5,     /* SWCore (algorithms/swendsen_wang.rs:12-144), Ising, Potts, O(3) */
+  SCAMPER_WOLFF = 6,             /* WolffCore (algorithms/wolff.rs:12-141), Ising, Potts, O(3) */
 };
 
 /* Algorithm::sweep_with_phase (algorithms/common.rs:35-55) for every
@@ -142,7 +142,9 @@ enum {
  * cluster update per "sweep" for SCAMPER_SWENDSEN_WANG / SCAMPER_WOLFF:
  * bonds, cluster targets and the Wolff seed come from counter-based draws of
  * their own streams; J * bond weight must be >= 0, as ClusterModel asserts
- * in models.rs:139-142, 321-324, else SCAMPER_EUNSUPPORTED).  With
+ * in models.rs:139-142, 321-324, 549-553, else SCAMPER_EUNSUPPORTED; O(3)
+ * spins take Wolff's embedding (models.rs:516-577): each update reflects
+ * clusters about one random direction, in f64, stored back as FP32).  With
  * measure != 0 each sweep's energy and magnetisation of every replica are
  * appended to the device-resident series (ClassicalMC::measure,
  * classical_mc.rs:187-190).  Asynchronous on the state's stream. */

@@ -3,8 +3,9 @@
     python -m scamper_amd.build        (or scamper_amd.build.build())
 
 Every HIP source is compiled with packed-FP32 instructions disabled; the
-threshold tables are compiled with -ffp-contract=off so their f64
-arithmetic rounds like the reference's.
+threshold tables and the cluster kernels (f64 O(3) bond probabilities and
+reflections) are compiled with -ffp-contract=off so their f64 arithmetic
+rounds like the reference's.
 """
 from __future__ import annotations
 
@@ -30,6 +31,7 @@ PER_FILE = {
     "capi.cpp": ["-ffp-contract=off"],
     "carlo.cpp": ["-ffp-contract=off"],
     "pool.cpp": ["-ffp-contract=off"],
+    "k_cluster.hip": ["-ffp-contract=off"],
 }
 
 

@@ -689,7 +689,7 @@ int ensure_cluster(scamper_state *st) {
   const uint64_t n = st->lat.n_sites;
   if (hipMalloc(&st->d_parent, std::max<uint64_t>(n, 1) * sizeof(uint32_t)) != hipSuccess)
     return fail(SCAMPER_ENOMEM, "cluster labels: %llu sites", (unsigned long long)n);
-  SC_HIP(hipMalloc(&st->d_cscratch, 2 * sizeof(uint32_t)));
+  SC_HIP(hipMalloc(&st->d_cscratch, 64));  // Wolff seed root and target; O(3): direction from byte 8
   SC_HIP(hipMalloc(&st->d_cthr, (size_t)st->n_rep * kClusterWeights * sizeof(uint64_t)));
   if (st->csr) {
     const std::vector<Bond> &e = st->lat.edges;
@@ -729,7 +729,6 @@ int prepare_cluster(scamper_state *st) {
 // algorithms/swendsen_wang.rs:73-135, wolff.rs:72-134); each counts as one
 // sweep of the draw counter.
 int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
-  if (st->continuous) return fail(SCAMPER_EUNSUPPORTED, "cluster updates take Ising or Potts spins");
   const uint64_t n = st->lat.n_sites;
   if (n >= (1ull << 32)) return fail(SCAMPER_EUNSUPPORTED, "cluster updates need fewer than 2^32 sites");
   // the reference asserts J * w >= 0 (models.rs:139-142, 321-324)
@@ -740,7 +739,8 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
   } else if (st->model.j < 0.0) {
     return fail(SCAMPER_EUNSUPPORTED, "cluster updates require J * bond.weight >= 0");
   }
-  if (int rc = prepare_cluster(st)) return rc;
+  // O(3) bonds depend on the spins: no threshold tables
+  if (int rc = st->continuous ? ensure_cluster(st) : prepare_cluster(st)) return rc;
   const std::vector<uint64_t> &thr = st->cthr;
   if (measure) {
     if (int rc = ensure_series(st, n_sweeps)) return rc;
@@ -752,7 +752,10 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
     const uint64_t step = st->sweeps + t;
     for (uint32_t r = 0; r < st->n_rep; ++r) {
       hipError_t e;
-      if (st->csr) {
+      if (st->continuous) {
+        e = launch_cluster_on3(st->geom, st->d_vec + 6ull * r * st->geom.nh, st->d_parent, st->keys[r], step,
+                               st->beta[r], st->model.j, wolff, st->d_cscratch, st->stream);
+      } else if (st->csr) {
         int8_t *spins = st->d_spins + (uint64_t)r * n;
         e = launch_cluster_bonds_csr(n, st->lat.edges.size(), st->d_cedges, st->d_cweights, spins, st->d_parent,
                                      st->keys[r], step, st->d_cthr + (size_t)r * kClusterWeights, st->stream);
@@ -771,8 +774,11 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
     }
     if (measure) {
       long long *em = st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride;
-      const hipError_t e = st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream)
-                                   : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream);
+      const hipError_t e =
+          st->continuous ? launch_on3_measure(st->geom, st->d_vec, st->n_rep, (float)st->model.j,
+                                              reinterpret_cast<double *>(em), st->stream)
+          : st->csr      ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream)
+                         : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream);
       if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure launch: %s", hipGetErrorString(e));
     }
   }
@@ -865,8 +871,8 @@ int scamper_sweep_timed(scamper_state *st, int algorithm, uint64_t n_sweeps, int
   const int table_alg = st->continuous ? kContinuousHeatBath : algorithm;
   if (st->tables_for != table_alg && (st->continuous || algorithm == kMetropolis || algorithm == kHeatBath))
     if (int rc = upload_tables(st, table_alg)) return rc;
-  if ((algorithm == kSwendsenWang || algorithm == kWolff) && !st->continuous && st->lat.n_sites < (1ull << 32))
-    if (int rc = prepare_cluster(st)) return rc;
+  if ((algorithm == kSwendsenWang || algorithm == kWolff) && st->lat.n_sites < (1ull << 32))
+    if (int rc = st->continuous ? ensure_cluster(st) : prepare_cluster(st)) return rc;
   if (measure) {
     if (int rc = ensure_series(st, n_sweeps)) return rc;
     if (int rc = ensure_ring(st, algorithm)) return rc;

@@ -10,8 +10,12 @@ bond), for Ising and Potts, with several replicas at different
 temperatures, and the measured per-update energy and magnetisation equal the
 oracle's.  Statistics: Swendsen-Wang and Wolff chains of the 2-D Ising model
 at beta_c on a 32 x 32 torus reproduce Kaufman's exact finite-torus energy.
-Unsupported inputs (O(3) spins, J w < 0) are refused with EUNSUPPORTED, as
-the reference asserts (models.rs:139-142, 321-324).
+O(3) spins take Wolff's embedding clusters: from the same FP32 spins the
+device's reflected spins equal the oracle's bit for bit (both compute in f64
+in the reference's operation order and round to FP32), and device chains on
+a periodic ring reproduce the exact transfer-matrix energy.  J w < 0 is
+refused with EUNSUPPORTED, as the reference asserts (models.rs:139-142,
+321-324, 549-553).
 """
 import os
 import sys
@@ -113,8 +117,59 @@ def test_energy_matches_kaufman_at_criticality(sc, alg):
     assert all(abs(z) < 4.5 for z in zs), zs
 
 
+@pytest.mark.parametrize("dims", [(8, 6, 4), (16, 16), (14,)])
+@pytest.mark.parametrize("wolff", [False, True])
+def test_on3_bit_exact_against_oracle(sc, orc, dims, wolff):
+    seed, n_rep, steps = 23, 3, 6
+    betas = [0.3, 0.8, 1.5]
+    print(f"case O(3) {dims} wolff={wolff}", flush=True)
+    lat, olat = sc.Lattice.hypercubic(list(dims)), orc.Lattice.hypercubic(list(dims), True)
+    om = orc.model(orc.ON, 1.0, 2, 3)
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 3), n_rep, seed)
+    for r in range(n_rep):
+        st.init(True, r)
+        st.set_beta(betas[r], r)
+    start = [st.download_f64(r) for r in range(n_rep)]
+    st.sweep(sc.WOLFF if wolff else sc.SWENDSEN_WANG, steps, measure=True)
+    for r in range(n_rep):
+        key = orc.replica_key(seed, r)
+        s = start[r].copy()
+        es = []
+        for t in range(steps):
+            orc.cluster_steps(olat, om, s, betas[r], key, t, 1, wolff)
+            es.append(orc.lib().orc_total_energy(orc.C.byref(om), olat.ptr, orc._p(s)))
+        got = st.download_f64(r)
+        assert np.array_equal(got, s), f"replica {r}: {np.count_nonzero(got != s)} components differ"
+        e_dev, _ = st.series(r)
+        assert np.allclose(e_dev, es, rtol=2e-6, atol=1e-6), (r, e_dev, es)
+    assert st.sweeps == steps
+
+
+@pytest.mark.parametrize("alg", ["sw", "wolff"])
+def test_on3_ring_energy_matches_transfer_matrix(sc, alg):
+    from scipy.special import spherical_in
+    n, beta = 8, 1.0
+    ls = np.arange(60)
+    il, dil = spherical_in(ls, beta), spherical_in(ls, beta, derivative=True)
+    want = -n * np.sum((2 * ls + 1) * il ** (n - 1) * dil) / np.sum((2 * ls + 1) * il**n)
+    st = sc.State(sc.Lattice.hypercubic([n]), sc.model(sc.ON, 1.0, 2, 3), 4, 29)
+    for r in range(4):
+        st.init(True, r)
+        st.set_beta(beta, r)
+    a = sc.SWENDSEN_WANG if alg == "sw" else sc.WOLFF
+    st.sweep(a, 500)
+    st.sweep(a, 8000, measure=True)
+    zs = []
+    for r in range(4):
+        e, _ = st.series(r)
+        est = sc.binned_estimate(e, 200)
+        zs.append((est.mean - want) / est.stderr)
+    print(alg, want, zs)
+    assert all(abs(z) < 4.5 for z in zs), zs
+
+
 def test_unsupported_inputs(sc):
-    on3 = sc.State(sc.Lattice.cubic(4), sc.model(sc.ON, 1.0, 2, 3), 1, 1)
+    on3 = sc.State(sc.Lattice.cubic(4), sc.model(sc.ON, -1.0, 2, 3), 1, 1)
     with pytest.raises(sc.ScamperError) as err:
         on3.sweep(sc.SWENDSEN_WANG, 1)
     assert err.value.code == 5  # SCAMPER_EUNSUPPORTED

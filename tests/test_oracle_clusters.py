@@ -11,6 +11,19 @@ wolff.rs:72-134, ClusterModel rules in lattice/models.rs:113-156, 295-338).
 * both updates sample the Boltzmann distribution: <E> and <m^2> from long
   chains on small lattices agree with exact enumeration (Ising on a torus
   and on a weighted graph with a self-loop and a parallel bond, Potts q=3).
+
+O(3) embedding clusters (ClusterModel for ONModel<D>, models.rs:516-577):
+* the direction draw equals an independent Python restatement bit for bit
+  and is uniform on the sphere (first and second moments);
+* for spins collinear with the direction the bond probability is the
+  embedded Ising bond, K = ceil(p 2^32) of the Ising threshold; it is zero
+  for opposite projections, for J = 0 and for a spin perpendicular to r;
+* beta = 0 gives singleton clusters: Swendsen-Wang reflects each site on
+  its own coin, Wolff the seed alone;
+* chains reproduce exact energies: the periodic ring from its transfer
+  spectrum 4 pi i_l(K) (modified spherical Bessel functions, checked to
+  reduce to the Langevin sum of the open chain), and a weighted tree from
+  the Langevin function of every bond.
 """
 import math
 import os
@@ -168,3 +181,150 @@ def test_potts_samples_boltzmann(orc, wolff):
     chains = _chain_stats(orc, lat, orc.model(orc.POTTS, 1.0, q), beta, wolff, range(4), 15000)
     zs = _zs(orc, chains, ex["E"], ex["M2"])
     assert np.all(np.abs(zs) < 4.5), zs
+
+
+# ------------------------------------------------ O(3) embedding clusters
+# (ClusterModel for ONModel<D>, models.rs:516-577)
+
+def _word_py(key, idx, step, stream):
+    ctr = [(idx >> 2) & 0xFFFFFFFF, step & 0xFFFFFFFF, ((step >> 32) & 0xFFFF) | (stream << 24), (idx >> 34) & 0xFFFFFFFF]
+    return _philox_py(ctr, [key & 0xFFFFFFFF, key >> 32])[idx & 3]
+
+
+def _direction_py(key, step):
+    """The rejection draw restated: try k on seed-stream words 4 + 3k ..."""
+    for k in range(64):
+        x = [2.0 * (_word_py(key, 4 + 3 * k + c, step, 13) * 2.0**-32) - 1.0 for c in range(3)]
+        n2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]
+        if 2.0**-40 < n2 <= 1.0:
+            n = math.sqrt(n2)
+            return [v / n for v in x]
+    return [0.0, 0.0, 1.0]
+
+
+def _direction(orc, key, step):
+    r = np.zeros(3)
+    orc.lib().orc_on3_direction(key, step, orc._p(r))
+    return r
+
+
+def test_on3_direction_is_the_rejection_draw_and_uniform(orc):
+    key = orc.replica_key(7, 1)
+    for step in [0, 1, 5, 2**33 + 3]:
+        assert list(_direction(orc, key, step)) == _direction_py(key, step)  # bit for bit
+    rs = np.array([_direction(orc, key, t) for t in range(20000)])
+    assert np.allclose(np.linalg.norm(rs, axis=1), 1.0, atol=2e-16)
+    n = len(rs)
+    assert np.all(np.abs(rs.mean(axis=0)) < 4.5 * math.sqrt(1 / 3 / n))  # uniform on S^2: <r> = 0
+    # <r_c^2> = 1/3 with variance 1/5 - 1/9
+    assert np.all(np.abs((rs**2).mean(axis=0) - 1 / 3) < 4.5 * math.sqrt((1 / 5 - 1 / 9) / n))
+
+
+def _bond_p(orc, j, w, beta, a, b, r):
+    a, b, r = (np.ascontiguousarray(v, np.float64) for v in (a, b, r))
+    return orc.lib().orc_on3_bond_probability(j, w, beta, orc._p(a), orc._p(b), orc._p(r))
+
+
+def test_on3_bond_probability_reduces_to_the_ising_embedding(orc):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        r = rng.normal(size=3)
+        r /= np.linalg.norm(r)
+        beta, w = rng.uniform(0.05, 2.0), rng.uniform(0.2, 3.0)
+        # spins collinear with r: the embedded Ising bond, 1 - exp(-2 beta J w)
+        p = _bond_p(orc, 1.0, w, beta, r, r, r)
+        K = orc.lib().orc_cluster_threshold(orc.ISING, 1.0, w, beta)
+        assert K == math.ceil(p * 2**32)
+        assert _bond_p(orc, 1.0, w, beta, r, -r, r) == 0.0  # anti-aligned projections: no bond
+        a, b = rng.normal(size=3), rng.normal(size=3)
+        a, b = a / np.linalg.norm(a), b / np.linalg.norm(b)
+        assert _bond_p(orc, 1.0, w, beta, a, b, r) == _bond_p(orc, 1.0, w, beta, b, a, r)
+        assert _bond_p(orc, 0.0, w, beta, a, b, r) == 0.0
+        # a spin perpendicular to r never bonds
+        perp = np.cross(r, a)
+        perp /= np.linalg.norm(perp)
+        assert _bond_p(orc, 1.0, w, beta, perp, b, r) <= 1e-15
+
+
+def _reflect_py(s, r):
+    proj = (s[0] * r[0] + s[1] * r[1]) + s[2] * r[2]
+    v = [s[c] - 2.0 * proj * r[c] for c in range(3)]
+    n = math.sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])
+    return [float(np.float32(x / n)) for x in v]
+
+
+def test_on3_beta_zero(orc):
+    """No bond at beta = 0: Swendsen-Wang reflects each site on its own coin,
+    Wolff reflects the seed alone; a reflection keeps s.r up to its sign."""
+    lat = orc.Lattice.hypercubic([4, 4, 2], True)
+    m = orc.model(orc.ON, 1.0, 2, 3)
+    key = orc.replica_key(2, 0)
+    s = orc.random_start(lat, m, key).astype(np.float32).astype(np.float64)
+    r = _direction_py(key, 3)
+    sw = s.copy()
+    assert orc.cluster_steps(lat, m, sw, 0.0, key, 3, 1, False) == lat.n_sites
+    for site in range(lat.n_sites):
+        old = s[3 * site:3 * site + 3]
+        if _word_py(key, site, 3, 12) < 2**31:
+            assert list(sw[3 * site:3 * site + 3]) == _reflect_py(old, r)
+            assert abs(np.dot(sw[3 * site:3 * site + 3], r) + np.dot(old, r)) < 1e-6
+        else:
+            assert np.array_equal(sw[3 * site:3 * site + 3], old)
+    wf = s.copy()
+    assert orc.cluster_steps(lat, m, wf, 0.0, key, 3, 1, True) == 1
+    seed = (_word_py(key, 0, 3, 13) * lat.n_sites) >> 32
+    moved = np.nonzero(np.any((wf != s).reshape(-1, 3), axis=1))[0]
+    assert list(moved) == [seed]
+
+
+def _ring_energy(n, K):
+    """<E>/J of the periodic classical Heisenberg ring of n sites at K = beta
+    J: the transfer kernel exp(K s.s') has eigenvalues 4 pi i_l(K) (2l + 1
+    times), so Z = sum (2l+1) (4 pi i_l)^n and <E> = -d ln Z / dK."""
+    from scipy.special import spherical_in
+    ls = np.arange(60)
+    il = spherical_in(ls, K)
+    dil = spherical_in(ls, K, derivative=True)
+    return -n * np.sum((2 * ls + 1) * il ** (n - 1) * dil) / np.sum((2 * ls + 1) * il**n)
+
+
+def _langevin(x):
+    return 1.0 / math.tanh(x) - 1.0 / x
+
+
+def _on3_chain_energy(orc, lat, beta, wolff, seeds, steps, burn=500):
+    m = orc.model(orc.ON, 1.0, 2, 3)
+    out = []
+    for seed in seeds:
+        key = orc.replica_key(seed, 0)
+        s = orc.random_start(lat, m, key)
+        es = []
+        for t in range(steps):
+            orc.cluster_steps(lat, m, s, beta, key, t, 1, wolff)
+            if t >= burn:
+                es.append(orc.lib().orc_total_energy(orc.C.byref(m), lat.ptr, orc._p(s)))
+        out.append(np.array(es))
+    return out
+
+
+@pytest.mark.parametrize("wolff", [False, True])
+def test_on3_ring_samples_boltzmann(orc, wolff):
+    n, beta = 4, 0.9
+    want = _ring_energy(n, beta)
+    # the transfer-matrix sum reduces to the open chain's Langevin sum as n grows
+    assert abs(_ring_energy(40, beta) / 40 + _langevin(beta)) < 1e-6
+    chains = _on3_chain_energy(orc, orc.Lattice.hypercubic([n], True), beta, wolff, range(4), 12000)
+    zs = [(orc.binned_estimate(e, 200).mean - want) / orc.binned_estimate(e, 200).stderr for e in chains]
+    assert np.all(np.abs(zs) < 4.5), (want, zs)
+
+
+@pytest.mark.parametrize("wolff", [False, True])
+def test_on3_weighted_tree_samples_boltzmann(orc, wolff):
+    """On a tree every bond is independent: <E> = -sum_b J w_b L(beta J w_b)."""
+    src, dst, w = [0, 1, 1, 3], [1, 2, 3, 4], [1.0, 2.0, 0.5, 1.5]
+    beta = 0.7
+    lat = orc.Lattice.from_edges(5, src, dst, w)
+    want = -sum(wb * _langevin(beta * wb) for wb in w)
+    chains = _on3_chain_energy(orc, lat, beta, wolff, range(4), 12000)
+    zs = [(orc.binned_estimate(e, 200).mean - want) / orc.binned_estimate(e, 200).stderr for e in chains]
+    assert np.all(np.abs(zs) < 4.5), (want, zs)
