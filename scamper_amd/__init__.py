@@ -538,3 +538,4 @@ class Pool:
 
 
 from . import snapshot  # noqa: E402  (cmc-rs-snapshot-v2 export/import)
+from . import params  # noqa: E402  (FromParams / build_lattice_from_params)
