@@ -77,6 +77,81 @@ def cpu_baseline(seconds_hint: float = 15.0):
                       f"{dt:.1f} s; host nproc={os.cpu_count()}"}
 
 
+def _cpu_timed(step, units_per_call: float, unit: str, what: str, seconds: float = 6.0, max_calls: int = 400) -> dict:
+    """A bounded single-thread timing of one oracle path (the port of the
+    reference's CPU path): `step(i)` runs call i; one untimed call first."""
+    step(0)
+    t0 = time.perf_counter()
+    calls = 0
+    while True:
+        calls += 1
+        step(calls)
+        if time.perf_counter() - t0 > seconds or calls >= max_calls:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": units_per_call * calls / (dt * 1e9), "unit": unit, "cores": 1, "kind": "port",
+            "sample": f"{calls} x {what}, 1 thread, {dt:.1f} s; host nproc={os.cpu_count()}"}
+
+
+def cpu_pt3d() -> dict:
+    """MultiSpinIsing's sweep restated with one beta per bit (oracle
+    orc_multispin_sweep), 3-D L=32, the 64 temperatures of the GPU run."""
+    import numpy as np
+
+    import oracle
+
+    lat = oracle.Lattice.hypercubic([32, 32, 32], True)
+    words = np.random.default_rng(1).integers(0, 2**63, lat.n_sites, dtype=np.uint64)
+    betas = [1.0 / (4.30 * (4.72 / 4.30) ** (k / 63)) for k in range(64)]
+    return _cpu_timed(lambda i: oracle.multispin_sweep(lat, words, betas, 1.0, 42, 0, i), 64 * lat.n_sites,
+                      "spin-flips/ns", "64-bit multi-spin sweeps of a 32^3 periodic lattice (orc_multispin_sweep)")
+
+
+def cpu_on3() -> dict:
+    """The O(3) heat bath and over-relaxation in the reference's layout
+    (f64 site-major spins over the CSR lattice, orc_sweep_on3_heat_bath and
+    orc_sweep_over_relaxation), 3-D L=24 at T = 1.40."""
+    import oracle
+
+    lat = oracle.Lattice.hypercubic([24, 24, 24], True)
+    m = oracle.model(oracle.ON, 1.0, 2, 3)
+    key = oracle.replica_key(42, 0)
+    ch = oracle.Chain(lat, m, oracle.random_start(lat, m, key))
+
+    def step(i):
+        ch.on3_heat_bath(1 / 1.40, key, i)
+        ch.over_relaxation()
+    return _cpu_timed(step, 2 * lat.n_sites, "site-updates/ns",
+                      "heat-bath + over-relaxation sweeps of a 24^3 periodic lattice (f64, CSR)")
+
+
+def cpu_potts3d() -> dict:
+    """The 3-state Potts Metropolis sweep in the reference's layout (f64
+    spins, CSR lattice, orc_sweep_metropolis), 3-D L=32 at beta = 1."""
+    import oracle
+
+    lat = oracle.Lattice.hypercubic([32, 32, 32], True)
+    m = oracle.model(oracle.POTTS, 1.0, 3)
+    key = oracle.replica_key(42, 0)
+    ch = oracle.Chain(lat, m, oracle.random_start(lat, m, key))
+    return _cpu_timed(lambda i: ch.metropolis(1.0, key, i), lat.n_sites, "spin-flips/ns",
+                      "Metropolis sweeps of 3-state Potts on a 32^3 periodic lattice (f64, CSR)")
+
+
+def cpu_cluster() -> dict:
+    """Swendsen-Wang updates restated (orc_cluster_steps: the reference's
+    edge loop and union-find over the CSR lattice), 2-D Ising 1024^2 at
+    beta_c."""
+    import oracle
+
+    lat = oracle.Lattice.square(1024, 1024, True)
+    m = oracle.model(oracle.ISING)
+    key = oracle.replica_key(42, 0)
+    s = oracle.random_start(lat, m, key)
+    return _cpu_timed(lambda i: oracle.cluster_steps(lat, m, s, 0.4406868, key, i, 1, False), lat.n_sites,
+                      "site-updates/ns", "Swendsen-Wang updates of a 1024^2 periodic lattice (f64, CSR)")
+
+
 def ising2d(sc, args) -> dict:
     L = args.L
     T = 2.269185
@@ -180,7 +255,8 @@ def pt3d(sc, args) -> dict:
                        "ladder": "geometric", "seed": 42, "exchanges_accepted": acc,
                        "near_tie_flags": ties, "exchange_steps": ex},
             "roofline": {"bound": "valu (cache-resident state)",
-                         "hbm_equivalent_GBps": round(bytes_per_step * args.pt_steps / (ms * 1e-3) / 1e9, 1)}}
+                         "hbm_equivalent_GBps": round(bytes_per_step * args.pt_steps / (ms * 1e-3) / 1e9, 1)},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_pt3d()})}
 
 
 def on3(sc, args) -> dict:
@@ -206,7 +282,8 @@ def on3(sc, args) -> dict:
             "higher_is_better": True, "dtype": "f32 spins, f64 observables", "data": "synthetic",
             "config": {"workload": "on3_hybrid", "L": L, "replicas": R, "T": [1.40, 1.49], "seed": 42},
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_on3()})}
 
 
 def potts3d(sc, args) -> dict:
@@ -233,7 +310,8 @@ def potts3d(sc, args) -> dict:
             "higher_is_better": True, "dtype": "int8 states", "data": "synthetic",
             "config": {"workload": "potts3d_tiles", "L": L, "q": q, "beta": 1.0, "seed": 42},
             "roofline": {"bound": "valu", "hbm_equivalent_GBps": round(gbs, 1),
-                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}}
+                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_potts3d()})}
 
 
 def cluster(sc, args) -> dict:
@@ -279,6 +357,8 @@ def cluster(sc, args) -> dict:
                      "abs_m": m}
         st.close()
     out["value"] = out["swendsen_wang"]["site_updates_per_ns"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_cluster()
     return out
 
 
