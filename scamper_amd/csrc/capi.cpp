@@ -717,8 +717,9 @@ int prepare_cluster(scamper_state *st) {
   st->cthr.assign((size_t)st->n_rep * kClusterWeights, 0);
   for (uint32_t r = 0; r < st->n_rep; ++r)
     for (int w = 0; w < kClusterWeights; ++w)
+      // by |w| with |J|: cluster updates require J w >= 0, so J w = |J| |w|
       st->cthr[(size_t)r * kClusterWeights + w] =
-          cluster_bond_threshold(st->model.kind, st->beta[r], st->model.j, (double)w);
+          cluster_bond_threshold(st->model.kind, st->beta[r], std::fabs(st->model.j), (double)w);
   SC_HIP(hipStreamSynchronize(st->stream));
   SC_HIP(hipMemcpy(st->d_cthr, st->cthr.data(), st->cthr.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   st->cthr_ready = true;

@@ -211,7 +211,8 @@ struct ClusterEdge {
   uint32_t source, target;
 };
 // Bond thresholds: an aligned bond of integer weight w is active when its
-// draw is below thr[w] (thr[1] for stencils).
+// draw is below thr[|w|] (thr[1] for stencils), built with |J| (J w >= 0 on
+// every edge, so the coupling J w is |J| |w|).
 constexpr int kClusterWeights = 128;
 hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spins, uint32_t *parent,
                                         uint64_t key, uint64_t step, uint64_t thr, hipStream_t stream);

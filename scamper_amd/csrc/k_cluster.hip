@@ -119,7 +119,10 @@ __global__ void k_bonds_csr(uint64_t n_edges, const ClusterEdge *__restrict__ ed
   for (uint64_t e = thread_id(); e < n_edges; e += grid_threads()) {
     const ClusterEdge b = edges[e];
     if (b.source == b.target || spins[b.source] != spins[b.target]) continue;  // a self-loop joins nothing
-    if ((uint64_t)cluster_word(key, e, step, kStreamClusterBond) < thr[w[e]]) unite(parent, b.source, b.target);
+    // J w >= 0 on every edge, so the coupling is |J| |w| (thr is indexed by |w|)
+    const int wi = w[e];
+    if ((uint64_t)cluster_word(key, e, step, kStreamClusterBond) < thr[wi < 0 ? -wi : wi])
+      unite(parent, b.source, b.target);
   }
 }
 

@@ -81,6 +81,29 @@ def test_bit_exact_against_oracle(sc, orc, lat_name, kind, wolff):
     assert st.sweeps == steps
 
 
+@pytest.mark.parametrize("kind", ["ising", "potts"])
+@pytest.mark.parametrize("wolff", [False, True])
+def test_negative_coupling_with_negative_weights(sc, orc, kind, wolff):
+    """J < 0 on bonds of negative weight is ferromagnetic (J w > 0), which
+    the reference accepts; the coupling is |J| |w|."""
+    src, dst, w = [0, 1, 2, 3, 4, 0, 2], [1, 2, 3, 4, 5, 3, 5], [-1.0, -2.0, -1.0, -3.0, -1.0, -2.0, 0.0]
+    lat, olat = sc.Lattice.from_edges(6, src, dst, w), orc.Lattice.from_edges(6, src, dst, w)
+    if kind == "ising":
+        m, om = sc.model(sc.ISING, -0.7), orc.model(orc.ISING, -0.7)
+    else:
+        m, om = sc.model(sc.POTTS, -0.7, 3), orc.model(orc.POTTS, -0.7, 3)
+    st = sc.State(lat, m, 2, 37)
+    for r, beta in enumerate([0.4, 1.1]):
+        st.init(True, r)
+        st.set_beta(beta, r)
+    start = [st.download_f64(r) for r in range(2)]
+    st.sweep(sc.WOLFF if wolff else sc.SWENDSEN_WANG, 9)
+    for r, beta in enumerate([0.4, 1.1]):
+        s = start[r].copy()
+        orc.cluster_steps(olat, om, s, beta, orc.replica_key(37, r), 0, 9, wolff)
+        assert np.array_equal(st.download_f64(r), s)
+
+
 def test_continues_after_metropolis(sc, orc):
     """Cluster draws have their own streams: Metropolis sweeps, then cluster
     updates numbered by the same counter, equal the oracle's sequence."""
