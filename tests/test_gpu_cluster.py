@@ -191,6 +191,24 @@ def test_on3_ring_energy_matches_transfer_matrix(sc, alg):
     assert all(abs(z) < 4.5 for z in zs), zs
 
 
+@pytest.mark.parametrize("alg", ["sw", "wolff"])
+def test_run_one_with_cluster_updates(sc, alg):
+    """Scheduler::run_one (scamper_run_one) drives cluster updates like any
+    other algorithm: the Energy estimate is the Accumulator's over the
+    returned series and matches Kaufman's finite-torus energy at beta_c."""
+    L, beta = 32, 0.4406868
+    a = sc.SWENDSEN_WANG if alg == "sw" else sc.WOLFF
+    meas = 8000 if alg == "sw" else 40000
+    res, e, _ = sc.run_one(sc.Lattice.square(L, L), sc.model(sc.ISING, 1.0), a, beta, True, 1000, meas, 100, 11,
+                           series=True)
+    assert res["Energy"].n_bins == meas // 100
+    assert res["Energy"].mean == pytest.approx(e.mean(), rel=1e-12)
+    est = sc.binned_estimate(e, meas // 40)  # 40 blocks, longer than the autocorrelation
+    z = (est.mean - exact.kaufman_energy(L, L, beta)) / est.stderr
+    print(alg, est.mean, z)
+    assert abs(z) < 4.5
+
+
 def test_unsupported_inputs(sc):
     on3 = sc.State(sc.Lattice.cubic(4), sc.model(sc.ON, -1.0, 2, 3), 1, 1)
     with pytest.raises(sc.ScamperError) as err:
