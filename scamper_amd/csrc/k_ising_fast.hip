@@ -11,7 +11,9 @@
 //   anti  = number of anti-aligned neighbours per byte (spins are 0x01/0xFF,
 //           so (s ^ n) has bit 1 set exactly when the two differ)
 //   t8,m8 = top and second byte of the threshold K[anti]   (v_perm lookups)
-//   flip  = (top, mid) < (t8, m8), or K[anti] = 2^32 (every word accepts)
+//   flip  = (top, mid) < (t8, m8), or K[anti] = 2^32 (every word accepts);
+//           the pairs are interleaved into 16-bit lanes by v_perm and
+//           compared two sites at a time with packed u16 max/min
 //   tie   = (top, mid) == (t8, m8)   (one uphill site in ~65536)
 //
 // A tie needs the low 16 bits of the draw from a third block.  The row loop
@@ -50,20 +52,21 @@ __device__ __forceinline__ void store16(int8_t *p, const Chunk &c) {
 // 0x02 in every byte where a and b differ (spins 0x01 / 0xFF).
 __device__ __forceinline__ uint32_t differ2(uint32_t a, uint32_t b) { return (a ^ b) & 0x02020202u; }
 
-// Byte-wise unsigned x < y, reported in bit 7 of each byte.  The subtraction
-// is done on 7-bit fields (no borrow crosses a byte), bit 7 is repaired, and
-// the borrow out of bit 7 is the comparison.
-__device__ __forceinline__ uint32_t lt_bytes(uint32_t x, uint32_t y) {
-  const uint32_t same = ~(x ^ y);
-  const uint32_t d = ((x | kMsb) - (y & ~kMsb)) ^ (same & kMsb);
-  return ((~x & y) | (same & d)) & kMsb;
+// Packed unsigned 16-bit max / min of two lanes (v_pk_max_u16, v_pk_min_u16).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t,
+                            __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t,
+                            __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
 }
 
-// Byte-wise x == y, reported in bit 7 of each byte.
-__device__ __forceinline__ uint32_t eq_bytes(uint32_t x, uint32_t y) {
-  const uint32_t t = x ^ y;
-  return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & kMsb;
-}
+// v_perm selectors: the 16-bit values (top << 8 | mid) of sites 0, 1 (kPair01)
+// or 2, 3 (kPair23) of a word, from perm(top word, mid word); and the bytes 0
+// and 2 of two such lane words gathered back into site order (kGather).
+constexpr uint32_t kPair01 = 0x05010400u, kPair23 = 0x07030602u, kGather = 0x06040200u;
 
 // Settle one tied site: its spin is still the pre-sweep value; recount its
 // anti-aligned neighbours and compare the low 24 bits of its draw.
@@ -109,12 +112,12 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
   const ReplicaParams &P = reps[rep];
 
   // byte LUTs indexed by the anti-aligned count (<= 6 < 8): top and second
-  // byte of K, and 0xff where K = 2^32
+  // byte of K, and 1 where K = 2^32
   uint32_t t_lo = 0, t_hi = 0, m_lo = 0, m_hi = 0, al_lo = 0, al_hi = 0;
 #pragma unroll
   for (uint32_t a = 0; a <= z; ++a) {
     const uint32_t k8 = P.thr[a].k8;
-    const uint32_t tb = k8 > 0xFFu ? 0xFFu : k8, ab = k8 > 0xFFu ? 0xFFu : 0u;
+    const uint32_t tb = k8 > 0xFFu ? 0xFFu : k8, ab = k8 > 0xFFu ? 1u : 0u;
     const uint32_t mb = P.thr[a].m8;
     if (a < 4) {
       t_lo |= tb << (8 * a);
@@ -126,6 +129,11 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
       al_hi |= ab << (8 * (a - 4));
     }
   }
+
+  // 1 in both 16-bit lanes, opaque to the optimiser, which would otherwise
+  // turn min(x, 1) into per-lane compares and selects
+  uint32_t lanes1 = 0x00010001u;
+  asm volatile("" : "+v"(lanes1));
 
   const uint32_t cpr = g.lx2 >> 4;
   const uint64_t tid = (uint64_t)b * kBlock + threadIdx.x;
@@ -204,14 +212,21 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
         const uint32_t a = a2 >> 1;  // anti-aligned count per byte, 0..z
         const uint32_t t8 = __builtin_amdgcn_perm(t_hi, t_lo, a);
         const uint32_t m8 = __builtin_amdgcn_perm(m_hi, m_lo, a);
-        const uint32_t always = __builtin_amdgcn_perm(al_hi, al_lo, a) & kMsb;
+        const uint32_t always = __builtin_amdgcn_perm(al_hi, al_lo, a);  // 1 where K = 2^32
         const uint32_t ht = word_of(T, k), hm = word_of(M, k);
-        // word < K from the top two bytes: top < t8, or top == t8 and mid < m8
-        const uint32_t eq_t = eq_bytes(ht, t8);
-        const uint32_t accept = lt_bytes(ht, t8) | (eq_t & lt_bytes(hm, m8)) | always;
-        tpos |= (eq_t & eq_bytes(hm, m8) & ~always) >> (7 - k);
-        const uint32_t m = accept >> 7;
-        const uint32_t flip = ((m << 8) - m) & 0xFEFEFEFEu;
+        // word < K from its top two bytes: (top, mid) < (t8, m8) as 16-bit
+        // lanes, two sites per packed compare; u < k is max(u, k) != u
+        const uint32_t u01 = __builtin_amdgcn_perm(ht, hm, kPair01), u23 = __builtin_amdgcn_perm(ht, hm, kPair23);
+        const uint32_t k01 = __builtin_amdgcn_perm(t8, m8, kPair01), k23 = __builtin_amdgcn_perm(t8, m8, kPair23);
+        const uint32_t lt01 = pk_min_u16(pk_max_u16(u01, k01) ^ u01, lanes1);
+        const uint32_t lt23 = pk_min_u16(pk_max_u16(u23, k23) ^ u23, lanes1);
+        const uint32_t m = __builtin_amdgcn_perm(lt23, lt01, kGather) | always;  // 1 per accepted site
+        // (top, mid) == (t8, m8): a tie, settled from the low bytes unless K = 2^32
+        const uint32_t ne01 = pk_min_u16(u01 ^ k01, lanes1), ne23 = pk_min_u16(u23 ^ k23, lanes1);
+        const uint32_t tie = ~__builtin_amdgcn_perm(ne23, ne01, kGather) & ~always & 0x01010101u;
+        tpos |= tie << k;
+        // bytes 0x00 / 0xFF from selectors 0 / 1 into 0x0000FF00
+        const uint32_t flip = __builtin_amdgcn_perm(0u, 0x0000FF00u, m) & 0xFEFEFEFEu;
         if (MEAS) {
           // bond sum over the chunk's sites with their new spins:
           // sum (z - 2 * anti_new), anti_new = z - anti where flipped
