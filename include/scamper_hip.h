@@ -82,7 +82,8 @@ void scamper_lattice_free(scamper_lattice *lat);
 enum { SCAMPER_ISING = 0, SCAMPER_POTTS = 1, SCAMPER_ON = 2 };
 
 /* IsingModel{j} (models.rs:29-33), PottsModel{j,q} (models.rs:190-194),
- * ONModel<D>{j} (models.rs:384-388; spin_dim = D, only D = 3 on the GPU). */
+ * ONModel<D>{j} (models.rs:384-388; spin_dim = D: D = 2, the XY model, and
+ * D = 3, the Heisenberg model, on periodic hypercubic lattices). */
 typedef struct {
   int32_t kind;
   int32_t q;
@@ -113,7 +114,7 @@ int scamper_state_init(scamper_state *st, uint32_t replica, int random);
  * Hamiltonian::validate_configuration (interaction.rs:40-58). */
 int scamper_state_upload_f64(scamper_state *st, uint32_t replica, const double *spins);
 int scamper_state_download_f64(scamper_state *st, uint32_t replica, double *spins);
-/* Site-major int8 (Ising +-1, Potts 0..q-1) or f32 (O(3), 3 per site). */
+/* Site-major int8 (Ising +-1, Potts 0..q-1) or f32 (O(N), spin_dim per site). */
 int scamper_state_upload_i8(scamper_state *st, uint32_t replica, const int8_t *spins);
 int scamper_state_download_i8(scamper_state *st, uint32_t replica, int8_t *spins);
 int scamper_state_upload_f32(scamper_state *st, uint32_t replica, const float *spins);
@@ -133,8 +134,8 @@ enum {
   SCAMPER_CONT_HEAT_BATH = 2,    /* ContinuousHeatBathCore (heat_bath.rs:75-134) */
   SCAMPER_OVER_RELAXATION = 3,   /* MicrocanonicalCore (algorithms/microcanonical.rs:13-85) */
   SCAMPER_HYBRID_HB_OR = 4,      /* HybridCore<ContinuousHeatBath, Microcanonical> (hybrid.rs:10-64) */
-  SCAMPER_SWENDSEN_WANG = 5,     /* SWCore (algorithms/swendsen_wang.rs:12-144), Ising, Potts, O(3) */
-  SCAMPER_WOLFF = 6,             /* WolffCore (algorithms/wolff.rs:12-141), Ising, Potts, O(3) */
+  SCAMPER_SWENDSEN_WANG = 5,     /* SWCore (algorithms/swendsen_wang.rs:12-144), Ising, Potts, O(3) (not O(2)) */
+  SCAMPER_WOLFF = 6,             /* WolffCore (algorithms/wolff.rs:12-141), Ising, Potts, O(3) (not O(2)) */
 };
 
 /* Algorithm::sweep_with_phase (algorithms/common.rs:35-55) for every

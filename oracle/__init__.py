@@ -81,6 +81,9 @@ def _declare(L: C.CDLL) -> None:
         "orc_sweep_metropolis": (u64, [vp, vp, vp, vp, dbl, u64, u64]),
         "orc_sweep_heat_bath": (None, [vp, vp, vp, vp, dbl, u64, u64]),
         "orc_sweep_on3_heat_bath": (None, [vp, vp, vp, vp, dbl, u64, u64]),
+        "orc_sweep_on2_heat_bath": (None, [vp, vp, vp, vp, dbl, u64, u64]),
+        "orc_von_mises": (dbl, [u64, u64, u64, u32, dbl]),
+        "orc_vm_block": (None, [u64, u64, u64, u32, u32, vp]),
         "orc_sweep_over_relaxation": (None, [vp, vp, vp, vp]),
         "orc_random_start": (None, [vp, vp, u64, vp]),
         "orc_ising2d_i8_sweeps": (u64, [vp, u64, u64, dbl, dbl, u64, u64, u64, i32]),
@@ -257,6 +260,10 @@ class Chain:
 
     def on3_heat_bath(self, beta, key, sweep) -> None:
         lib().orc_sweep_on3_heat_bath(C.byref(self.m), self.lat.ptr, _p(self.spins),
+                                      C.byref(self.energy), beta, key, sweep)
+
+    def on2_heat_bath(self, beta, key, sweep) -> None:
+        lib().orc_sweep_on2_heat_bath(C.byref(self.m), self.lat.ptr, _p(self.spins),
                                       C.byref(self.energy), beta, key, sweep)
 
     def over_relaxation(self) -> None:

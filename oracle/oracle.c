@@ -44,9 +44,11 @@
 #define ORC_STREAM_CLUSTER_BOND 11u
 #define ORC_STREAM_CLUSTER_TARGET 12u
 #define ORC_STREAM_CLUSTER_SEED 13u
+#define ORC_STREAM_VON_MISES 14u
 
 static const double TWO_M32 = 2.3283064365386963e-10; /* 2^-32 */
 static const double TAU = 6.283185307179586;          /* std::f64::consts::TAU */
+static const double PI = 3.141592653589793;           /* std::f64::consts::PI */
 
 /* ------------------------------------------------------------ SplitMix64 */
 
@@ -712,6 +714,59 @@ static void on3_heat_bath(const orc_model *m, const orc_lattice *lat, const doub
   }
 }
 
+/* Best-Fisher von Mises sampler (sample_von_mises, models.rs:666-689) on
+ * counter-based draws: try k of site j takes block (j, k) of the von Mises
+ * stream, word 0 for the angle pi u, word 1 for the acceptance uniform; the
+ * accepted try's word 2 gives the sign (+acos below 2^31, else -acos).  With
+ * kappa < 1e-8 the angle is 2 pi u from word 0 of try 0.  A 64th rejection in
+ * a row (probability below 0.35^64) keeps that try's value. */
+#define ORC_VM_TRIES 64
+
+void orc_vm_block(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t k, uint32_t w[4]) {
+  block(key, (uint32_t)j, sweep, colour, ORC_STREAM_VON_MISES, (uint32_t)(j >> 32) | (k << 20), w);
+}
+
+double orc_von_mises(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, double kappa) {
+  uint32_t w[4];
+  if (kappa < 1e-8) {
+    orc_vm_block(key, j, sweep, colour, 0, w);
+    return (double)w[0] * TWO_M32 * TAU;
+  }
+  const double tau = 1.0 + sqrt(1.0 + 4.0 * kappa * kappa);
+  const double rho = (tau - sqrt(2.0 * tau)) / (2.0 * kappa);
+  const double r = (1.0 + rho * rho) / (2.0 * rho);
+  double value = 0.0;
+  for (uint32_t k = 0; k < ORC_VM_TRIES; ++k) {
+    orc_vm_block(key, j, sweep, colour, k, w);
+    const double z = cos(PI * ((double)w[0] * TWO_M32));
+    value = (1.0 + r * z) / (r + z);
+    const double c = kappa * (r - value);
+    const double u = fmax((double)w[1] * TWO_M32, DBL_MIN);
+    if (c * (2.0 - c) > u || log(c / u) + 1.0 >= c) break;
+  }
+  const double a = acos(fmin(fmax(value, -1.0), 1.0));
+  return w[2] < 0x80000000u ? a : -a;
+}
+
+/* ContinuousHeatBathable for ONModel<2> (models.rs:596-614): the angle about
+ * the local field from the von Mises draw; a vanishing field (|h| < 1e-15)
+ * draws a uniform direction from the fallback words. */
+static void on2_heat_bath(const orc_model *m, const orc_lattice *lat, const double *spins, uint64_t site,
+                          double beta, uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, double *out) {
+  double f[2];
+  local_field(m, lat, spins, site, f);
+  const double norm = sqrt((-0.0 + f[0] * f[0]) + f[1] * f[1]);
+  if (norm < 1e-15) {
+    uint32_t fb[4];
+    cont_fallback_words(key, j, sweep, colour, fb);
+    random_unit_vector(2, fb, out);
+    return;
+  }
+  const double theta = orc_von_mises(key, j, sweep, colour, beta * norm) + atan2(f[1], f[0]);
+  out[0] = cos(theta);
+  out[1] = sin(theta);
+}
+
 /* ------------------------------------------------------ colour-ordered sweeps */
 
 /* MetropolisCore::sweep_with_phase (algorithms/metropolis.rs:67-106) with the
@@ -774,6 +829,22 @@ void orc_sweep_on3_heat_bath(const orc_model *m, const orc_lattice *lat, double 
       on3_heat_bath(m, lat, spins, site, beta, w, fb, proposed);
       double de = orc_delta_energy(m, lat, spins, site, proposed);
       memcpy(spins + 3 * site, proposed, sizeof proposed);
+      *energy += de;
+    }
+  }
+}
+
+/* ContinuousHeatBathCore::sweep_with_phase (algorithms/heat_bath.rs:102-134)
+ * for ONModel<2>. */
+void orc_sweep_on2_heat_bath(const orc_model *m, const orc_lattice *lat, double *spins, double *energy,
+                             double beta, uint64_t key, uint64_t sweep) {
+  for (uint32_t c = 0; c < lat->n_colours; ++c) {
+    for (uint64_t idx = lat->class_start[c]; idx < lat->class_start[c + 1]; ++idx) {
+      uint64_t site = lat->class_sites[idx], j = idx - lat->class_start[c];
+      double proposed[2];
+      on2_heat_bath(m, lat, spins, site, beta, key, j, sweep, c, proposed);
+      double de = orc_delta_energy(m, lat, spins, site, proposed);
+      memcpy(spins + 2 * site, proposed, sizeof proposed);
       *energy += de;
     }
   }

@@ -132,6 +132,13 @@ void orc_sweep_heat_bath(const orc_model *m, const orc_lattice *lat, double *spi
 /* O(3) continuous heat bath (heat_bath.rs:102-134; models.rs:617-662). */
 void orc_sweep_on3_heat_bath(const orc_model *m, const orc_lattice *lat, double *spins,
                              double *energy, double beta, uint64_t key, uint64_t sweep);
+/* Continuous heat bath of ONModel<2> (models.rs:596-614) with the von Mises
+ * angle from orc_von_mises (Best-Fisher, models.rs:666-689, on the words of
+ * orc_vm_block: try k of site j uses block (j, k) of stream 14). */
+void orc_sweep_on2_heat_bath(const orc_model *m, const orc_lattice *lat, double *spins, double *energy,
+                             double beta, uint64_t key, uint64_t sweep);
+double orc_von_mises(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, double kappa);
+void orc_vm_block(uint64_t key, uint64_t j, uint64_t sweep, uint32_t colour, uint32_t k, uint32_t w[4]);
 /* Over-relaxation (microcanonical.rs:42-80). */
 void orc_sweep_over_relaxation(const orc_model *m, const orc_lattice *lat, double *spins,
                                double *energy);

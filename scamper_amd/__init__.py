@@ -282,8 +282,13 @@ class State:
         check(lib().scamper_state_download_i8(self.h, replica, _p(a)))
         return a
 
+    def _components(self) -> int:
+        return self.lattice.n_sites * max(1, self.model.spin_dim)
+
     def upload_f64(self, spins, replica: int = 0):
-        a = np.ascontiguousarray(spins, dtype=np.float64)
+        a = np.ascontiguousarray(spins, dtype=np.float64).ravel()
+        if a.size != self._components():
+            raise ValueError(f"expected {self._components()} values, got {a.size}")
         check(lib().scamper_state_upload_f64(self.h, replica, _p(a)))
 
     def download_f64(self, replica: int = 0) -> np.ndarray:
@@ -292,12 +297,14 @@ class State:
         return a
 
     def upload_f32(self, spins, replica: int = 0):
-        """O(3) states: 3 f32 components per site, site-major."""
-        a = np.ascontiguousarray(spins, dtype=np.float32)
+        """O(N) states: N f32 components per site, site-major."""
+        a = np.ascontiguousarray(spins, dtype=np.float32).ravel()
+        if a.size != self._components():
+            raise ValueError(f"expected {self._components()} values, got {a.size}")
         check(lib().scamper_state_upload_f32(self.h, replica, _p(a)))
 
     def download_f32(self, replica: int = 0) -> np.ndarray:
-        a = np.zeros(self.lattice.n_sites * 3, np.float32)
+        a = np.zeros(self._components(), np.float32)
         check(lib().scamper_state_download_f32(self.h, replica, _p(a)))
         return a
 

@@ -79,7 +79,7 @@ struct scamper_state {
   hipStream_t stream = nullptr;
   int8_t *d_spins = nullptr;       // d_spins_alloc + kSpinSlack
   int8_t *d_spins_alloc = nullptr;
-  bool continuous = false;  // O(3): FP32 component planes in d_vec
+  bool continuous = false;  // O(N): FP32 component planes in d_vec
   float *d_vec = nullptr;
   ReplicaParams *d_params = nullptr;
   double *d_hbexp = nullptr;
@@ -165,7 +165,7 @@ int upload_tables(scamper_state *st, int alg) {
       if (alg == kMetropolis) ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
       else ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
     } else if (st->continuous) {
-      // the O(3) kernels read only key and beta
+      // the O(N) kernels read only key and beta
     } else if (st->model.kind == kPotts) {
       if (alg == kMetropolis) potts_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
       else potts_heat_bath_exp(st->beta[r], st->model.j, z, hb.data() + (size_t)r * (kMaxZ + 1) * (kMaxZ + 1));
@@ -446,9 +446,9 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   if (model->kind != kIsing && model->kind != kPotts && model->kind != kON)
     return fail(SCAMPER_EINVAL, "unknown model kind %d", model->kind);
   const Lattice &l = lh->lat;
-  if (model->kind == kON && (model->spin_dim != 3 || !l.stencil))
+  if (model->kind == kON && ((model->spin_dim != 2 && model->spin_dim != 3) || !l.stencil))
     return fail(SCAMPER_EUNSUPPORTED,
-                "O(N) spins run with N = 3 on periodic hypercubic lattices with even sides (got N = %d)",
+                "O(N) spins run with N = 2 or 3 on periodic hypercubic lattices with even sides (got N = %d)",
                 model->spin_dim);
   auto st = std::unique_ptr<scamper_state>(new (std::nothrow) scamper_state);
   if (!st) return fail(SCAMPER_ENOMEM, "state handle");
@@ -516,7 +516,7 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   }
   st->continuous = model->kind == kON;
   if (st->continuous) {
-    const size_t bytes = (size_t)n_replicas * l.n_sites * 3 * sizeof(float);
+    const size_t bytes = (size_t)n_replicas * l.n_sites * model->spin_dim * sizeof(float);
     if (hipMalloc(&st->d_vec, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
   } else {
     const size_t bytes = (size_t)n_replicas * l.n_sites;
@@ -527,7 +527,7 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   SC_HIP(hipMalloc(&st->d_params, sizeof(ReplicaParams) * n_replicas));
   SC_HIP(hipMalloc(&st->d_em, sizeof(long long) * kEmStride * n_replicas));
   for (uint32_t r = 0; r < n_replicas; ++r) {
-    hipError_t e = st->continuous ? launch_on3_init(st->geom, st->d_vec, st->keys[r], r, 0, st->stream)
+    hipError_t e = st->continuous ? launch_on3_init(st->geom, st->model.spin_dim, st->d_vec, st->keys[r], r, 0, st->stream)
                    : st->csr ? launch_csr_init(l.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream)
                              : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream);
     if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
@@ -557,46 +557,58 @@ int scamper_state_get_beta(const scamper_state *st, uint32_t replica, double *be
 
 int scamper_state_init(scamper_state *st, uint32_t replica, int random) {
   if (int rc = check_state(st, replica)) return rc;
-  hipError_t e = st->continuous ? launch_on3_init(st->geom, st->d_vec, st->keys[replica], replica, random != 0, st->stream)
+  hipError_t e = st->continuous ? launch_on3_init(st->geom, st->model.spin_dim, st->d_vec, st->keys[replica], replica, random != 0, st->stream)
                  : st->csr ? launch_csr_init(st->lat.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream)
                            : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
   return SCAMPER_OK;
 }
 
+// O(N) replicas: D planes per colour, [c][component][rank], 2 D nh floats.
 static int vec_upload(scamper_state *st, uint32_t replica, const float *xyz) {
   const uint64_t n = st->lat.n_sites, nh = st->geom.nh;
-  std::vector<float> planes(6 * nh);
+  const int d = st->model.spin_dim;
+  std::vector<float> planes(2ull * d * nh);
   for (uint64_t s = 0; s < n; ++s) {
     uint32_t c;
     const uint64_t i = split_index(st->geom, s, &c);
-    for (int k = 0; k < 3; ++k) planes[(3ull * c + k) * nh + i] = xyz[3 * s + k];
+    for (int k = 0; k < d; ++k) planes[((uint64_t)d * c + k) * nh + i] = xyz[(uint64_t)d * s + k];
   }
   SC_HIP(hipStreamSynchronize(st->stream));
-  SC_HIP(hipMemcpy(st->d_vec + 6ull * replica * nh, planes.data(), planes.size() * sizeof(float), hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(st->d_vec + 2ull * d * replica * nh, planes.data(), planes.size() * sizeof(float),
+                   hipMemcpyHostToDevice));
   return SCAMPER_OK;
 }
 
 static int vec_download(scamper_state *st, uint32_t replica, float *xyz) {
   const uint64_t n = st->lat.n_sites, nh = st->geom.nh;
-  std::vector<float> planes(6 * nh);
+  const int d = st->model.spin_dim;
+  std::vector<float> planes(2ull * d * nh);
   SC_HIP(hipStreamSynchronize(st->stream));
-  SC_HIP(hipMemcpy(planes.data(), st->d_vec + 6ull * replica * nh, planes.size() * sizeof(float), hipMemcpyDeviceToHost));
+  SC_HIP(hipMemcpy(planes.data(), st->d_vec + 2ull * d * replica * nh, planes.size() * sizeof(float),
+                   hipMemcpyDeviceToHost));
   for (uint64_t s = 0; s < n; ++s) {
     uint32_t c;
     const uint64_t i = split_index(st->geom, s, &c);
-    for (int k = 0; k < 3; ++k) xyz[3 * s + k] = planes[(3ull * c + k) * nh + i];
+    for (int k = 0; k < d; ++k) xyz[(uint64_t)d * s + k] = planes[((uint64_t)d * c + k) * nh + i];
   }
   return SCAMPER_OK;
 }
 
-// ONModel::validate_spin (models.rs:454-474): finite, |s|^2 within 1e-10 D
-// of one; f32 input is checked to single-precision accuracy.
-static int vec_validate(const double *xyz, uint64_t n, double tol) {
+// ONModel::validate_spin (models.rs:454-474): finite, |s|^2 within tol of
+// one (the reference's 1e-10 D for f64 input; f32 input is checked to
+// single-precision accuracy).
+static int vec_validate(const double *xyz, uint64_t n, int d, double tol) {
   for (uint64_t s = 0; s < n; ++s) {
-    const double a = xyz[3 * s], b = xyz[3 * s + 1], c = xyz[3 * s + 2];
-    if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c) || std::fabs(a * a + b * b + c * c - 1.0) > tol)
-      return fail(SCAMPER_EINVAL, "O(3) spin at site %llu is not a finite unit vector", (unsigned long long)s);
+    double sq = 0.0;
+    bool finite = true;
+    for (int k = 0; k < d; ++k) {
+      const double v = xyz[(uint64_t)d * s + k];
+      finite = finite && std::isfinite(v);
+      sq += v * v;
+    }
+    if (!finite || std::fabs(sq - 1.0) > tol)
+      return fail(SCAMPER_EINVAL, "O(%d) spin at site %llu is not a finite unit vector", d, (unsigned long long)s);
   }
   return SCAMPER_OK;
 }
@@ -620,9 +632,10 @@ int scamper_state_upload_f64(scamper_state *st, uint32_t replica, const double *
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
   const uint64_t n = st->lat.n_sites;
   if (st->continuous) {
-    if (int rc = vec_validate(spins, n, 3e-10)) return rc;
-    std::vector<float> f(3 * n);
-    for (uint64_t k = 0; k < 3 * n; ++k) f[k] = (float)spins[k];
+    const int d = st->model.spin_dim;
+    if (int rc = vec_validate(spins, n, d, 1e-10 * d)) return rc;
+    std::vector<float> f((uint64_t)d * n);
+    for (uint64_t k = 0; k < f.size(); ++k) f[k] = (float)spins[k];
     return vec_upload(st, replica, f.data());
   }
   std::vector<int8_t> v(n);
@@ -641,7 +654,7 @@ int scamper_state_download_f64(scamper_state *st, uint32_t replica, double *spin
   if (int rc = check_state(st, replica)) return rc;
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
   if (st->continuous) {
-    std::vector<float> f(3 * st->lat.n_sites);
+    std::vector<float> f((uint64_t)st->model.spin_dim * st->lat.n_sites);
     if (int rc = vec_download(st, replica, f.data())) return rc;
     for (uint64_t k = 0; k < f.size(); ++k) spins[k] = (double)f[k];
     return SCAMPER_OK;
@@ -655,17 +668,17 @@ int scamper_state_download_f64(scamper_state *st, uint32_t replica, double *spin
 int scamper_state_upload_f32(scamper_state *st, uint32_t replica, const float *spins) {
   if (int rc = check_state(st, replica)) return rc;
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
-  if (!st->continuous) return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
-  std::vector<double> d(3 * st->lat.n_sites);
+  if (!st->continuous) return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(N) states");
+  std::vector<double> d((uint64_t)st->model.spin_dim * st->lat.n_sites);
   for (uint64_t k = 0; k < d.size(); ++k) d[k] = spins[k];
-  if (int rc = vec_validate(d.data(), st->lat.n_sites, 1e-5)) return rc;
+  if (int rc = vec_validate(d.data(), st->lat.n_sites, st->model.spin_dim, 1e-5)) return rc;
   return vec_upload(st, replica, spins);
 }
 
 int scamper_state_download_f32(scamper_state *st, uint32_t replica, float *spins) {
   if (int rc = check_state(st, replica)) return rc;
   if (!spins) return fail(SCAMPER_EINVAL, "null spins");
-  if (!st->continuous) return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(3) states");
+  if (!st->continuous) return fail(SCAMPER_EUNSUPPORTED, "f32 spins belong to O(N) states");
   return vec_download(st, replica, spins);
 }
 
@@ -731,6 +744,8 @@ int prepare_cluster(scamper_state *st) {
 // sweep of the draw counter.
 int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
   const uint64_t n = st->lat.n_sites;
+  if (st->continuous && st->model.spin_dim != 3)
+    return fail(SCAMPER_EUNSUPPORTED, "O(N) cluster updates run with N = 3");
   if (n >= (1ull << 32)) return fail(SCAMPER_EUNSUPPORTED, "cluster updates need fewer than 2^32 sites");
   // the reference asserts J * w >= 0 (models.rs:139-142, 321-324)
   if (st->csr) {
@@ -754,7 +769,8 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
     for (uint32_t r = 0; r < st->n_rep; ++r) {
       hipError_t e;
       if (st->continuous) {
-        e = launch_cluster_on3(st->geom, st->d_vec + 6ull * r * st->geom.nh, st->d_parent, st->keys[r], step,
+        e = launch_cluster_on3(st->geom, st->d_vec + 2ull * st->model.spin_dim * r * st->geom.nh, st->d_parent,
+                               st->keys[r], step,
                                st->beta[r], st->model.j, wolff, st->d_cscratch, st->stream);
       } else if (st->csr) {
         int8_t *spins = st->d_spins + (uint64_t)r * n;
@@ -776,7 +792,7 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
     if (measure) {
       long long *em = st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride;
       const hipError_t e =
-          st->continuous ? launch_on3_measure(st->geom, st->d_vec, st->n_rep, (float)st->model.j,
+          st->continuous ? launch_on3_measure(st->geom, st->model.spin_dim, st->d_vec, st->n_rep, (float)st->model.j,
                                               reinterpret_cast<double *>(em), st->stream)
           : st->csr      ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream)
                          : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream);
@@ -795,7 +811,7 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
   if (algorithm == kSwendsenWang || algorithm == kWolff) return cluster_sweeps(st, algorithm, n_sweeps, measure);
   if (st->continuous) {
     if (algorithm != kContinuousHeatBath && algorithm != kOverRelaxation && algorithm != kHybridHeatBathOverRelaxation)
-      return fail(SCAMPER_EUNSUPPORTED, "O(3) spins take the continuous heat bath, over-relaxation or their hybrid");
+      return fail(SCAMPER_EUNSUPPORTED, "O(N) spins take the continuous heat bath, over-relaxation or their hybrid");
   } else if (algorithm != kMetropolis && algorithm != kHeatBath) {
     return fail(SCAMPER_EUNSUPPORTED, "algorithm %d needs continuous spins", algorithm);
   }
@@ -821,7 +837,7 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
           double *em = (measure && last && c == 1)
                            ? reinterpret_cast<double *>(st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride)
                            : nullptr;
-          hipError_t e = launch_on3_sweep(st->geom, parts[pi], st->d_vec, st->d_params, st->n_rep, c, sweep,
+          hipError_t e = launch_on3_sweep(st->geom, st->model.spin_dim, parts[pi], st->d_vec, st->d_params, st->n_rep, c, sweep,
                                           (float)st->model.j, em, st->stream);
           if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
         }
@@ -903,7 +919,7 @@ int scamper_state_observables(scamper_state *st, uint32_t replica, double *energ
   if (int rc = check_state(st, replica)) return rc;
   SC_HIP(hipMemsetAsync(st->d_em, 0, sizeof(long long) * kEmStride * st->n_rep, st->stream));
   hipError_t e = st->continuous
-                     ? launch_on3_measure(st->geom, st->d_vec, st->n_rep, (float)st->model.j,
+                     ? launch_on3_measure(st->geom, st->model.spin_dim, st->d_vec, st->n_rep, (float)st->model.j,
                                           reinterpret_cast<double *>(st->d_em), st->stream)
                  : st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream)
                            : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);

@@ -107,13 +107,13 @@ hipError_t launch_csr_measure(const CsrGeom &g, int kind, int q, const int8_t *s
 hipError_t launch_csr_init(uint64_t n_sites, int kind, int q, int8_t *spins, uint64_t key, uint32_t rep,
                            int random, hipStream_t stream);
 
-// O(3) spins (FP32 component planes per colour); em gets double sums
-// {s.h, Mx, My, Mz} per replica.
-hipError_t launch_on3_sweep(const StencilGeom &g, int alg, float *spins, const ReplicaParams *reps, uint32_t n_rep,
-                            uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t stream);
-hipError_t launch_on3_measure(const StencilGeom &g, const float *spins, uint32_t n_rep, float j, double *em,
+// O(N) spins, N = dim = 2 or 3 (FP32 component planes per colour, dim per
+// colour per replica); em gets double sums {s.h, Mx, My, Mz} per replica.
+hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
+                            uint32_t n_rep, uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t stream);
+hipError_t launch_on3_measure(const StencilGeom &g, int dim, const float *spins, uint32_t n_rep, float j, double *em,
                               hipStream_t stream);
-hipError_t launch_on3_init(const StencilGeom &g, float *spins, uint64_t key, uint32_t rep, int random,
+hipError_t launch_on3_init(const StencilGeom &g, int dim, float *spins, uint64_t key, uint32_t rep, int random,
                            hipStream_t stream);
 
 // Multi-spin coded Ising (u64 words, bit r = replica 64b + r).

@@ -341,10 +341,242 @@ __global__ __launch_bounds__(kBlock) void k_on3_init(float *__restrict__ spins, 
   store3(spins + (2ull * rep + colour) * 3ull * g.nh, g.nh, j, v);
 }
 
+// ------------------------------------------------------------ O(2) (XY)
+// Two component planes per colour ([c][x, y][rank]).  The heat bath draws the
+// angle about the local field from the Best-Fisher von Mises sampler
+// (ContinuousHeatBathable for ONModel<2>, models.rs:596-614, and
+// sample_von_mises, models.rs:666-689) in f64, so that its rejection
+// branches follow the f64 rule: try k of rank j takes Philox block (j, k) of
+// stream kStreamVonMises (word 0 the angle pi u, word 1 the acceptance
+// uniform, word 2 the sign of the accepted try); kappa < 1e-8 takes 2 pi u
+// from word 0 of try 0; a 64th rejection in a row keeps that try's value.
+
+struct V2 {
+  float x, y;
+};
+
+__device__ __forceinline__ V2 load2(const float *base, uint64_t nh, uint64_t j) { return V2{base[j], base[nh + j]}; }
+__device__ __forceinline__ void store2(float *base, uint64_t nh, uint64_t j, V2 v) {
+  base[j] = v.x;
+  base[nh + j] = v.y;
+}
+
+constexpr double kTwoM32 = 2.3283064365386963e-10, kTau = 6.283185307179586, kPi = 3.141592653589793;
+
+__device__ __forceinline__ U4 vm_block(const ReplicaParams &P, uint64_t j, uint64_t sweep, uint32_t colour,
+                                       uint32_t k) {
+  return philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamVonMises),
+                       (uint32_t)(j >> 32) | (k << 20), (uint32_t)P.key, (uint32_t)(P.key >> 32));
+}
+
+__device__ double von_mises(const ReplicaParams &P, uint64_t j, uint64_t sweep, uint32_t colour, double kappa) {
+  if (kappa < 1e-8) return (double)vm_block(P, j, sweep, colour, 0).x * kTwoM32 * kTau;
+  const double tau = 1.0 + sqrt(1.0 + 4.0 * kappa * kappa);
+  const double rho = (tau - sqrt(2.0 * tau)) / (2.0 * kappa);
+  const double r = (1.0 + rho * rho) / (2.0 * rho);
+  double value = 0.0;
+  uint32_t sign = 0;
+  for (uint32_t k = 0; k < 64; ++k) {
+    const U4 w = vm_block(P, j, sweep, colour, k);
+    const double z = cos(kPi * ((double)w.x * kTwoM32));
+    value = (1.0 + r * z) / (r + z);
+    const double c = kappa * (r - value);
+    const double u = fmax((double)w.y * kTwoM32, 2.2250738585072014e-308);
+    sign = w.z;
+    if (c * (2.0 - c) > u || log(c / u) + 1.0 >= c) break;
+  }
+  const double a = acos(fmin(fmax(value, -1.0), 1.0));
+  return sign < 0x80000000u ? a : -a;
+}
+
+// ONModel<2>::random_unit_vector (models.rs:405-422) in f64 from two words.
+__device__ __forceinline__ void random_unit2(uint32_t w0, uint32_t w1, double *out) {
+  const double u1 = fmax((double)w0 * kTwoM32, 2.2250738585072014e-308), u2 = (double)w1 * kTwoM32;
+  const double radius = sqrt(-2.0 * log(u1)), angle = kTau * u2;
+  const double x = radius * cos(angle), y = radius * sin(angle);
+  const double norm = sqrt(x * x + y * y);
+  out[0] = norm > 1e-15 ? x / norm : x;
+  out[1] = norm > 1e-15 ? y / norm : y;
+}
+
+template <int ND, int ALG, bool MEAS>
+__global__ __launch_bounds__(kBlock) void k_on2(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
+                                                StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
+                                                float jc, double *__restrict__ em) {
+  uint32_t rep, bx;
+  xcd_item(nbx, &rep, &bx);
+  const uint64_t j = (uint64_t)bx * kBlock + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (j < g.nh) {
+    float *__restrict__ own = spins + (2ull * rep + colour) * 2ull * g.nh;
+    const float *__restrict__ oth = spins + (2ull * rep + 1 - colour) * 2ull * g.nh;
+    const SiteCoords c = site_coords(j, g);
+    const uint32_t p = (colour + c.y + c.z) & 1u;
+    const uint64_t rb = (uint64_t)c.row * g.lx2;
+    uint64_t nb[2 * ND];
+    nb[0] = rb + (p ? c.i : wrap_dec(c.i, g.lx2));
+    nb[1] = rb + (p ? wrap_inc(c.i, g.lx2) : c.i);
+    if (ND >= 2) {
+      nb[2] = ((uint64_t)(wrap_dec(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+      nb[3] = ((uint64_t)(wrap_inc(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+    }
+    if (ND >= 3) {
+      nb[4] = ((uint64_t)(c.y + g.ly * wrap_dec(c.z, g.lz))) * g.lx2 + c.i;
+      nb[5] = ((uint64_t)(c.y + g.ly * wrap_inc(c.z, g.lz))) * g.lx2 + c.i;
+    }
+    // local field in f64 (models.rs:579-595), the centre neighbour kept for M
+    double hx = 0.0, hy = 0.0;
+    V2 centre{0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 2 * ND; ++k) {
+      const V2 v = load2(oth, g.nh, nb[k]);
+      hx += (double)jc * (double)v.x;
+      hy += (double)jc * (double)v.y;
+      if (nb[k] == j) centre = v;
+    }
+    const V2 s = load2(own, g.nh, j);
+    V2 ns = s;
+    if (ALG == kContinuousHeatBath) {
+      const ReplicaParams &P = reps[rep];
+      const double norm = sqrt(hx * hx + hy * hy);
+      double out[2];
+      if (norm < 1e-15) {
+        const U4 fb = philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
+                                    (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32));
+        random_unit2(fb.x, fb.y, out);
+      } else {
+        const double theta = von_mises(P, j, sweep, colour, P.beta * norm) + atan2(hy, hx);
+        out[0] = cos(theta);
+        out[1] = sin(theta);
+      }
+      ns = V2{(float)out[0], (float)out[1]};
+    } else {
+      // over-relaxation (microcanonical.rs:54-75): s reflected about h
+      const float fx = (float)hx, fy = (float)hy;
+      const float nsq = fx * fx + fy * fy;
+      if (nsq >= 1e-28f) {
+        const float proj = (s.x * fx + s.y * fy) / nsq;
+        float rx = 2.0f * proj * fx - s.x, ry = 2.0f * proj * fy - s.y;
+        const float inv = rsqrtf(rx * rx + ry * ry);
+        ns = V2{rx * inv, ry * inv};
+      }
+    }
+    store2(own, g.nh, j, ns);
+    if (MEAS) {
+      acc[0] = (double)ns.x * hx + (double)ns.y * hy;
+      acc[1] = (double)ns.x + (double)centre.x;
+      acc[2] = (double)ns.y + (double)centre.y;
+    }
+  }
+  if (MEAS) {
+    __shared__ double red[4];
+    if (threadIdx.x < 4) red[threadIdx.x] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if ((threadIdx.x & 63) == 0) atomicAdd(&red[k], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) atomicAdd(&em[(uint64_t)rep * kEmStride + threadIdx.x], red[threadIdx.x]);
+  }
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_on2_measure(const float *__restrict__ spins, StencilGeom g, float jc,
+                                                        double *__restrict__ em) {
+  const uint32_t rep = blockIdx.y;
+  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (j < g.nh) {
+    const float *own = spins + (2ull * rep + 1) * 2ull * g.nh;
+    const float *oth = spins + (2ull * rep) * 2ull * g.nh;
+    const SiteCoords c = site_coords(j, g);
+    const uint32_t p = (1 + c.y + c.z) & 1u;
+    const uint64_t rb = (uint64_t)c.row * g.lx2;
+    uint64_t nb[2 * ND];
+    nb[0] = rb + (p ? c.i : wrap_dec(c.i, g.lx2));
+    nb[1] = rb + (p ? wrap_inc(c.i, g.lx2) : c.i);
+    if (ND >= 2) {
+      nb[2] = ((uint64_t)(wrap_dec(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+      nb[3] = ((uint64_t)(wrap_inc(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+    }
+    if (ND >= 3) {
+      nb[4] = ((uint64_t)(c.y + g.ly * wrap_dec(c.z, g.lz))) * g.lx2 + c.i;
+      nb[5] = ((uint64_t)(c.y + g.ly * wrap_inc(c.z, g.lz))) * g.lx2 + c.i;
+    }
+    double hx = 0.0, hy = 0.0;
+    V2 centre{0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 2 * ND; ++k) {
+      const V2 v = load2(oth, g.nh, nb[k]);
+      hx += (double)jc * (double)v.x;
+      hy += (double)jc * (double)v.y;
+      if (nb[k] == j) centre = v;
+    }
+    const V2 s = load2(own, g.nh, j);
+    acc[0] = (double)s.x * hx + (double)s.y * hy;
+    acc[1] = (double)s.x + (double)centre.x;
+    acc[2] = (double)s.y + (double)centre.y;
+  }
+  __shared__ double red[3];
+  if (threadIdx.x < 3) red[threadIdx.x] = 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&red[k], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) atomicAdd(&em[(uint64_t)rep * kEmStride + threadIdx.x], red[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_on2_init(float *__restrict__ spins, StencilGeom g, uint64_t key,
+                                                     uint32_t rep, int random) {
+  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t colour = blockIdx.y;
+  if (j >= g.nh) return;
+  const SiteCoords c = site_coords(j, g);
+  const uint64_t site = 2ull * c.i + ((colour + c.y + c.z) & 1u) + 2ull * g.lx2 * c.row;
+  V2 v{1.f, 0.f};  // ordered_spin (models.rs:479-484)
+  if (random) {
+    const U4 w = philox4x32_10((uint32_t)site, 0u, ctr2(0, 0, kStreamContInit), (uint32_t)(site >> 32),
+                               (uint32_t)key, (uint32_t)(key >> 32));
+    double out[2];
+    random_unit2(w.x, w.y, out);
+    v = V2{(float)out[0], (float)out[1]};
+  }
+  store2(spins + (2ull * rep + colour) * 2ull * g.nh, g.nh, j, v);
+}
+
 }  // namespace
 
-hipError_t launch_on3_sweep(const StencilGeom &g, int alg, float *spins, const ReplicaParams *reps, uint32_t n_rep,
-                            uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t st) {
+hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
+                            uint32_t n_rep, uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t st) {
+  if (dim == 2) {
+    const uint32_t nbx = (uint32_t)((g.nh + kBlock - 1) / kBlock);
+    const bool flat = (uint64_t)nbx * n_rep * kBlock < (1ull << 32);
+    const dim3 grid = flat ? dim3(nbx * n_rep) : dim3(nbx, n_rep);
+#define SC_ON2(ND, A)                                                                                                 \
+  if (em)                                                                                                             \
+    hipLaunchKernelGGL((k_on2<ND, A, true>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);   \
+  else                                                                                                                \
+    hipLaunchKernelGGL((k_on2<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);
+#define SC_ON2_ND(ND)                                                  \
+  if (alg == kContinuousHeatBath) { SC_ON2(ND, kContinuousHeatBath) } \
+  else { SC_ON2(ND, kOverRelaxation) }
+    switch (g.nd) {
+      case 1: SC_ON2_ND(1) break;
+      case 2: SC_ON2_ND(2) break;
+      case 3: SC_ON2_ND(3) break;
+      default: return hipErrorInvalidValue;
+    }
+#undef SC_ON2_ND
+#undef SC_ON2
+    return hipGetLastError();
+  }
   // two ranks per thread for the heat bath (one Philox block for both);
   // over-relaxation has no draws to share and runs faster one rank per thread
   const bool pairs = alg == kContinuousHeatBath && g.lx2 % 2 == 0;
@@ -378,9 +610,18 @@ hipError_t launch_on3_sweep(const StencilGeom &g, int alg, float *spins, const R
   return hipGetLastError();
 }
 
-hipError_t launch_on3_measure(const StencilGeom &g, const float *spins, uint32_t n_rep, float j, double *em,
+hipError_t launch_on3_measure(const StencilGeom &g, int dim, const float *spins, uint32_t n_rep, float j, double *em,
                               hipStream_t st) {
   dim3 grid((unsigned)((g.nh + kBlock - 1) / kBlock), n_rep);
+  if (dim == 2) {
+    switch (g.nd) {
+      case 1: hipLaunchKernelGGL((k_on2_measure<1>), grid, dim3(kBlock), 0, st, spins, g, j, em); break;
+      case 2: hipLaunchKernelGGL((k_on2_measure<2>), grid, dim3(kBlock), 0, st, spins, g, j, em); break;
+      case 3: hipLaunchKernelGGL((k_on2_measure<3>), grid, dim3(kBlock), 0, st, spins, g, j, em); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   switch (g.nd) {
     case 1: hipLaunchKernelGGL((k_on3_measure<1>), grid, dim3(kBlock), 0, st, spins, g, j, em); break;
     case 2: hipLaunchKernelGGL((k_on3_measure<2>), grid, dim3(kBlock), 0, st, spins, g, j, em); break;
@@ -390,7 +631,13 @@ hipError_t launch_on3_measure(const StencilGeom &g, const float *spins, uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_on3_init(const StencilGeom &g, float *spins, uint64_t key, uint32_t rep, int random, hipStream_t st) {
+hipError_t launch_on3_init(const StencilGeom &g, int dim, float *spins, uint64_t key, uint32_t rep, int random,
+                           hipStream_t st) {
+  if (dim == 2) {
+    hipLaunchKernelGGL(k_on2_init, dim3((unsigned)((g.nh + kBlock - 1) / kBlock), 2), dim3(kBlock), 0, st, spins, g,
+                       key, rep, random);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_on3_init, dim3((unsigned)((g.nh + kBlock - 1) / kBlock), 2), dim3(kBlock), 0, st, spins, g, key,
                      rep, random);
   return hipGetLastError();

@@ -42,6 +42,7 @@ enum Stream : uint32_t {
   kStreamClusterBond = 11,   // cluster updates: bond activation by physical edge index
   kStreamClusterTarget = 12, // Swendsen-Wang cluster target by the cluster's smallest site
   kStreamClusterSeed = 13,   // Wolff seed site (index 0) and Potts target (index 1)
+  kStreamVonMises = 14,      // O(2) heat bath: block (rank, try) of the von Mises rejection sampler
 };
 
 struct U4 {

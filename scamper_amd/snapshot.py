@@ -29,7 +29,7 @@ and loaded into the same replica index then continues the original
 trajectory bit for bit, as the reference's split-run checkpoint test
 requires (CMC.rs/tests/integration/checkpoint.rs:27-73, 154-193).
 
-O(3) spins live on the device as FP32 unit vectors, with norms within ~1e-7
+O(N) spins live on the device as FP32 unit vectors, with norms within ~1e-7
 of 1.  The reference accepts an O(N) spin only when |norm^2 - 1| <= 1e-10 D
 (models.rs:454-473).  So "spins" holds every vector renormalised in f64,
 which the reference loads, and "scamper"."spins_f32" holds the device's

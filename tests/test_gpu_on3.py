@@ -112,3 +112,118 @@ def test_statistics_match_fp64_oracle(sc, orc):
     z = (gpu_mean - est.mean) / math.sqrt(gpu_se**2 + est.stderr**2)
     print(f"E/N gpu {gpu_mean:.5f}+-{gpu_se:.5f} oracle {est.mean:.5f}+-{est.stderr:.5f} z={z:.2f}")
     assert abs(z) < 4
+
+
+# ------------------------------------------------------------------ O(2) (XY)
+# The device draws the von Mises angle in f64 from the same Philox words as
+# the oracle and stores FP32 components, so one sweep from the same start
+# agrees per component to single-precision rounding of cos/sin (2e-6), plus,
+# on the second colour, the shift of atan2(h) that FP32 storage of the first
+# colour's new spins causes: at most 2^-23 per neighbour component over |h|
+# (1.35e-5 at a site with |h| = 0.002).  The over-relaxation sweep reflects
+# in FP32 (1e-5).
+
+
+def _xy_field_norm(spins, dims):
+    """|h| at every site (J = 1, periodic, x fastest) from site-major spins."""
+    s = spins.reshape(tuple(reversed(dims)) + (2,))
+    h = sum(np.roll(s, sh, ax) for ax in range(len(dims)) for sh in (1, -1))
+    return np.linalg.norm(h, axis=-1).reshape(-1)
+
+
+def _setup_xy(sc, orc, dims, seed, beta, n_rep=1):
+    lat = sc.Lattice.hypercubic(list(dims))
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 2), n_rep, seed)
+    for r in range(n_rep):
+        st.init(True, r)
+        st.set_beta(beta, r)
+    olat = orc.Lattice.hypercubic(list(dims))
+    m = orc.model(orc.ON, 1.0, 2, 2)
+    return lat, st, olat, m
+
+
+@pytest.mark.parametrize("dims", [(16, 16), (8, 4, 6), (32,)])
+def test_xy_random_start_matches(sc, orc, dims):
+    _, st, olat, m = _setup_xy(sc, orc, dims, 5, 1.0)
+    got = st.download_f64()
+    want = orc.random_start(olat, m, orc.replica_key(5, 0))
+    assert got.shape == want.shape
+    assert np.max(np.abs(got - want)) < 2e-7
+    assert np.allclose(np.linalg.norm(got.reshape(-1, 2), axis=1), 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("dims,beta", [((16, 16), 1.12), ((8, 8, 8), 0.45), ((64,), 2.0), ((16, 16), 1e-9)])
+def test_xy_one_heat_bath_sweep(sc, orc, dims, beta):
+    seed = 21
+    _, st, olat, m = _setup_xy(sc, orc, dims, seed, beta)
+    start = st.download_f64()
+    chain = orc.Chain(olat, m, start.copy())
+    st.sweep(2, 1, measure=True)
+    chain.on2_heat_bath(beta, orc.replica_key(seed, 0), 0)
+    got = st.download_f64()
+    err = np.max(np.abs(got - chain.spins).reshape(-1, 2), axis=1)
+    tol = 2e-6 + 2.0**-23 * 2 * len(dims) / _xy_field_norm(chain.spins, dims)
+    print(f"XY {dims} beta {beta}: max per-component error after one heat-bath sweep {err.max():.2e}")
+    assert np.all(err < tol), (err.max(), np.argmax(err / tol))
+    assert np.median(err) < 5e-7
+    e, mag = st.series()
+    assert abs(e[-1] - chain.total_energy()) < 1e-5 * max(abs(chain.total_energy()), 1.0)
+    assert abs(mag[-1] - chain.magnetization()) < 1e-5
+
+
+def test_xy_over_relaxation_sweep(sc, orc):
+    _, st, olat, m = _setup_xy(sc, orc, (16, 16), 8, 1.0)
+    start = st.download_f64()
+    chain = orc.Chain(olat, m, start.copy())
+    e0, _ = st.observables()
+    st.sweep(3, 1)
+    chain.over_relaxation()
+    assert np.max(np.abs(st.download_f64() - chain.spins)) < 1e-5
+    st.sweep(3, 20)
+    e1, _ = st.observables()
+    assert abs(e1 - e0) < 1e-4 * abs(e0)
+
+
+def test_xy_hybrid_sweeps_follow_oracle(sc, orc):
+    dims, seed, beta = (16, 16), 31, 1.0
+    _, st, olat, m = _setup_xy(sc, orc, dims, seed, beta)
+    chain = orc.Chain(olat, m, st.download_f64())
+    key = orc.replica_key(seed, 0)
+    st.sweep(4, 3, measure=True)
+    for t in range(3):
+        chain.on2_heat_bath(beta, key, t)
+        chain.over_relaxation()
+    e, _ = st.series()
+    rel = abs(e[-1] - chain.total_energy()) / abs(chain.total_energy())
+    print(f"XY hybrid x3: relative energy error {rel:.2e}")
+    assert rel < 1e-3
+
+
+def test_xy_ring_energy_exact(sc):
+    """64 replicas of a periodic XY ring of 8 sites: <E> from the transfer
+    spectrum I_m(K), Z = sum_m I_m(K)^N, within |z| < 4.5."""
+    from scipy.special import iv
+
+    n, beta, n_rep, therm, meas = 8, 1.3, 64, 200, 4000
+    mm = np.arange(-60, 61)
+    i = iv(mm, beta)
+    di = 0.5 * (iv(mm - 1, beta) + iv(mm + 1, beta))
+    want = -n * np.sum(i ** (n - 1) * di) / np.sum(i**n)
+    lat = sc.Lattice.hypercubic([n])
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 2), n_rep, 77)
+    for r in range(n_rep):
+        st.init(True, r)
+        st.set_beta(beta, r)
+    st.sweep(2, therm)
+    st.sweep(2, meas, measure=True)
+    means = np.array([st.series(r)[0].mean() for r in range(n_rep)])
+    z = (means.mean() - want) / (means.std(ddof=1) / math.sqrt(n_rep))
+    print(f"XY ring: <E> {means.mean():.5f} exact {want:.5f} z={z:.2f}")
+    assert abs(z) < 4.5
+
+
+def test_xy_clusters_unsupported(sc):
+    lat = sc.Lattice.hypercubic([8, 8])
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 2), 1, 1)
+    with pytest.raises(sc.ScamperError):
+        st.sweep(5, 1)
