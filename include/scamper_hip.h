@@ -134,8 +134,8 @@ enum {
   SCAMPER_CONT_HEAT_BATH = 2,    /* ContinuousHeatBathCore (heat_bath.rs:75-134) */
   SCAMPER_OVER_RELAXATION = 3,   /* MicrocanonicalCore (algorithms/microcanonical.rs:13-85) */
   SCAMPER_HYBRID_HB_OR = 4,      /* HybridCore<ContinuousHeatBath, Microcanonical> (hybrid.rs:10-64) */
-  SCAMPER_SWENDSEN_WANG = 5,     /* SWCore (algorithms/swendsen_wang.rs:12-144), Ising, Potts, O(3) (not O(2)) */
-  SCAMPER_WOLFF = 6,             /* WolffCore (algorithms/wolff.rs:12-141), Ising, Potts, O(3) (not O(2)) */
+  SCAMPER_SWENDSEN_WANG = 5,     /* SWCore (algorithms/swendsen_wang.rs:12-144), Ising, Potts, O(2), O(3) */
+  SCAMPER_WOLFF = 6,             /* WolffCore (algorithms/wolff.rs:12-141), Ising, Potts, O(2), O(3) */
 };
 
 /* Algorithm::sweep_with_phase (algorithms/common.rs:35-55) for every

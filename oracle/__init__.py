@@ -99,6 +99,8 @@ def _declare(L: C.CDLL) -> None:
         "orc_cluster_threshold": (u64, [i32, dbl, dbl, dbl]),
         "orc_cluster_steps": (u64, [vp, vp, vp, dbl, u64, u64, u64, i32]),
         "orc_on3_direction": (None, [u64, u64, vp]),
+        "orc_on_direction": (None, [C.c_int, u64, u64, vp]),
+        "orc_on_bond_probability": (dbl, [C.c_int, dbl, dbl, dbl, vp, vp, vp]),
         "orc_on3_bond_probability": (dbl, [dbl, dbl, dbl, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():

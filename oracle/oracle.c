@@ -1392,71 +1392,87 @@ static void uf_union(uint64_t *parent, uint64_t a, uint64_t b) {
   else parent[a] = b;
 }
 
-/* O(3) embedding clusters (ClusterModel for ONModel<D>, models.rs:516-577).
- * The embedding direction r of an update is uniform on the sphere, drawn by
- * rejection from the cube: try k takes words 4 + 3k .. 6 + 3k of the seed
- * stream, x = 2 u - 1 per component, and accepts 2^-40 < |x|^2 <= 1; r is x
- * normalised as in normalize (models.rs:432-439).  The 64th rejection in a
- * row (probability 0.476^64) leaves r = (0, 0, 1).  Only +, *, / and sqrt
- * are used, all correctly rounded, so the device draws the same bits. */
-#define ORC_ON3_TRIES 64
+/* O(N) embedding clusters, N = 2 or 3 (ClusterModel for ONModel<D>,
+ * models.rs:516-577).  The embedding direction r of an update is uniform on
+ * the circle or sphere, drawn by rejection from the square or cube: try k
+ * takes words 4 + Dk .. 3 + D(k + 1) of the seed stream, x = 2 u - 1 per
+ * component, and accepts 2^-40 < |x|^2 <= 1; r is x normalised as in
+ * normalize (models.rs:432-439).  The 64th rejection in a row (probability
+ * 0.215^64 or 0.476^64) leaves r = the last unit axis.  Sums run left to
+ * right; only +, *, / and sqrt are used, all correctly rounded, so the device
+ * draws the same bits. */
+#define ORC_ON_TRIES 64
 
-void orc_on3_direction(uint64_t key, uint64_t step, double *r) {
-  for (uint64_t k = 0; k < ORC_ON3_TRIES; ++k) {
+static double dotn(int d, const double *a, const double *b) {
+  double acc = a[0] * b[0];
+  for (int c = 1; c < d; ++c) acc = acc + a[c] * b[c];
+  return acc;
+}
+
+void orc_on_direction(int d, uint64_t key, uint64_t step, double *r) {
+  for (uint64_t k = 0; k < ORC_ON_TRIES; ++k) {
     double x[3];
-    for (uint64_t c = 0; c < 3; ++c)
-      x[c] = 2.0 * ((double)orc_cluster_word(key, 4 + 3 * k + c, step, ORC_STREAM_CLUSTER_SEED) * 0x1p-32) - 1.0;
-    const double n2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
+    for (int c = 0; c < d; ++c)
+      x[c] = 2.0 * ((double)orc_cluster_word(key, 4 + (uint64_t)d * k + c, step, ORC_STREAM_CLUSTER_SEED) * 0x1p-32) -
+             1.0;
+    const double n2 = dotn(d, x, x);
     if (n2 > 0x1p-40 && n2 <= 1.0) {
       const double norm = sqrt(n2);
-      for (int c = 0; c < 3; ++c) r[c] = x[c] / norm;
+      for (int c = 0; c < d; ++c) r[c] = x[c] / norm;
       return;
     }
   }
-  r[0] = 0.0, r[1] = 0.0, r[2] = 1.0;
+  for (int c = 0; c < d; ++c) r[c] = c == d - 1 ? 1.0 : 0.0;
 }
 
-static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+void orc_on3_direction(uint64_t key, uint64_t step, double *r) { orc_on_direction(3, key, step, r); }
 
 /* cluster_bond_probability (models.rs:537-558): p = 1 - exp(-2 beta J w
  * (s_i . r)(s_j . r)) when the product is positive, else 0. */
-double orc_on3_bond_probability(double j, double weight, double beta, const double *left, const double *right,
-                                const double *r) {
+double orc_on_bond_probability(int d, double j, double weight, double beta, const double *left, const double *right,
+                               const double *r) {
   const double coupling = j * weight;
-  const double product = dot3(left, r) * dot3(right, r);
+  const double product = dotn(d, left, r) * dotn(d, right, r);
   if (coupling == 0.0 || product <= 0.0) return 0.0;
   return clamp_probability(1.0 - exp(-2.0 * beta * coupling * product));
 }
 
-/* transform_cluster_spin (models.rs:560-575): s - 2 (s . r) r, normalised;
- * then rounded to FP32, the device's storage. */
-static void on3_reflect(double *s, const double *r) {
-  const double proj = dot3(s, r);
-  double v[3];
-  for (int c = 0; c < 3; ++c) v[c] = s[c] - 2.0 * proj * r[c];
-  const double norm = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
-  if (norm > 1e-15)
-    for (int c = 0; c < 3; ++c) v[c] /= norm;
-  for (int c = 0; c < 3; ++c) s[c] = (double)(float)v[c];
+double orc_on3_bond_probability(double j, double weight, double beta, const double *left, const double *right,
+                                const double *r) {
+  return orc_on_bond_probability(3, j, weight, beta, left, right, r);
 }
 
-/* n Swendsen-Wang or Wolff updates of O(3) spins (3 components per site,
+/* transform_cluster_spin (models.rs:560-575): s - 2 (s . r) r, normalised;
+ * then rounded to FP32, the device's storage. */
+static void on_reflect(int d, double *s, const double *r) {
+  const double proj = dotn(d, s, r);
+  double v[3];
+  for (int c = 0; c < d; ++c) v[c] = s[c] - 2.0 * proj * r[c];
+  const double norm = sqrt(dotn(d, v, v));
+  if (norm > 1e-15)
+    for (int c = 0; c < d; ++c) v[c] /= norm;
+  for (int c = 0; c < d; ++c) s[c] = (double)(float)v[c];
+}
+
+/* n Swendsen-Wang or Wolff updates of O(N) spins (D components per site,
  * site-major).  Bonds: word e of the bond stream below p 2^32.  Swendsen-Wang
  * reflects a cluster when its root's target word is below 2^31 (the
  * reference's coin, sw_cluster_auxiliary); Wolff reflects the seed's. */
-static uint64_t on3_cluster_steps(const orc_model *m, const orc_lattice *lat, double *spins, double beta,
-                                  uint64_t key, uint64_t step0, uint64_t n, int wolff) {
+static uint64_t on_cluster_steps(const orc_model *m, const orc_lattice *lat, double *spins, double beta,
+                                 uint64_t key, uint64_t step0, uint64_t n, int wolff) {
+  const int d = m->spin_dim;
   const uint64_t ns = lat->n_sites, ne = lat->n_edges;
   uint64_t *parent = (uint64_t *)malloc((ns ? ns : 1) * sizeof(uint64_t));
   uint64_t result = 0;
   for (uint64_t k = 0; k < n; ++k) {
     const uint64_t step = step0 + k;
     double r[3];
-    orc_on3_direction(key, step, r);
+    orc_on_direction(d, key, step, r);
     for (uint64_t s = 0; s < ns; ++s) parent[s] = s;
     for (uint64_t e = 0; e < ne; ++e) {
       const orc_bond *b = &lat->edges[e];
-      const double p = orc_on3_bond_probability(m->j, b->weight, beta, spins + 3 * b->source, spins + 3 * b->target, r);
+      const double p =
+          orc_on_bond_probability(d, m->j, b->weight, beta, spins + d * b->source, spins + d * b->target, r);
       if ((double)orc_cluster_word(key, e, step, ORC_STREAM_CLUSTER_BOND) * 0x1p-32 < p)
         uf_union(parent, b->source, b->target);
     }
@@ -1465,14 +1481,14 @@ static uint64_t on3_cluster_steps(const orc_model *m, const orc_lattice *lat, do
       for (uint64_t s = 0; s < ns; ++s) {
         const uint64_t root = uf_find(parent, s);
         if (root == s) ++result;
-        if (orc_cluster_word(key, root, step, ORC_STREAM_CLUSTER_TARGET) < 0x80000000u) on3_reflect(spins + 3 * s, r);
+        if (orc_cluster_word(key, root, step, ORC_STREAM_CLUSTER_TARGET) < 0x80000000u) on_reflect(d, spins + d * s, r);
       }
     } else if (ns) {
       const uint64_t seed = ((uint64_t)orc_cluster_word(key, 0, step, ORC_STREAM_CLUSTER_SEED) * ns) >> 32;
       const uint64_t root = uf_find(parent, seed);
       for (uint64_t s = 0; s < ns; ++s)
         if (uf_find(parent, s) == root) {
-          on3_reflect(spins + 3 * s, r);
+          on_reflect(d, spins + d * s, r);
           ++result;
         }
     }
@@ -1483,7 +1499,8 @@ static uint64_t on3_cluster_steps(const orc_model *m, const orc_lattice *lat, do
 
 uint64_t orc_cluster_steps(const orc_model *m, const orc_lattice *lat, double *spins, double beta,
                            uint64_t key, uint64_t step0, uint64_t n, int wolff) {
-  if (m->kind == ORC_ON) return m->spin_dim == 3 ? on3_cluster_steps(m, lat, spins, beta, key, step0, n, wolff) : 0;
+  if (m->kind == ORC_ON)
+    return m->spin_dim == 2 || m->spin_dim == 3 ? on_cluster_steps(m, lat, spins, beta, key, step0, n, wolff) : 0;
   const uint64_t ns = lat->n_sites, ne = lat->n_edges;
   uint64_t *parent = (uint64_t *)malloc((ns ? ns : 1) * sizeof(uint64_t));
   uint64_t result = 0;

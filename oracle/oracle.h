@@ -230,17 +230,20 @@ uint64_t orc_cluster_threshold(int kind, double j, double weight, double beta);
  *   Wolff: the seed is floor(u N) of word(13, 0); its cluster (connected by
  *     active bonds) takes -seed (Ising) or, for Potts, the other state
  *     r = floor(u' (q-1)), r += (r >= seed state), u' from word(13, 1).
- * O(3) (spin_dim 3, three components per site): every update draws a
- *   direction r (orc_on3_direction); a bond is active when word(11, e) 2^-32
- *   < orc_on3_bond_probability; SW reflects a cluster about r when
+ * O(N) (spin_dim 2 or 3, D components per site): every update draws a
+ *   direction r (orc_on_direction); a bond is active when word(11, e) 2^-32
+ *   < orc_on_bond_probability; SW reflects a cluster about r when
  *   word(12, root) < 2^31, Wolff reflects the seed's; reflected spins are
  *   renormalised in f64 and rounded to FP32.
  * Returns the number of SW clusters, or the size of the last Wolff cluster,
  * of the last step. */
 uint64_t orc_cluster_steps(const orc_model *m, const orc_lattice *lat, double *spins, double beta,
                            uint64_t key, uint64_t step0, uint64_t n, int wolff);
-/* O(3) embedding clusters: the update's direction (3 doubles) and the bond
- * probability of two spins along it. */
+/* O(N) embedding clusters (D = 2 or 3): the update's direction (D doubles)
+ * and the bond probability of two spins along it; the on3 forms are D = 3. */
+void orc_on_direction(int d, uint64_t key, uint64_t step, double *r);
+double orc_on_bond_probability(int d, double j, double weight, double beta, const double *left, const double *right,
+                               const double *r);
 void orc_on3_direction(uint64_t key, uint64_t step, double *r);
 double orc_on3_bond_probability(double j, double weight, double beta, const double *left, const double *right,
                                 const double *r);

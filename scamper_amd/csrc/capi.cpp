@@ -702,7 +702,7 @@ int ensure_cluster(scamper_state *st) {
   const uint64_t n = st->lat.n_sites;
   if (hipMalloc(&st->d_parent, std::max<uint64_t>(n, 1) * sizeof(uint32_t)) != hipSuccess)
     return fail(SCAMPER_ENOMEM, "cluster labels: %llu sites", (unsigned long long)n);
-  SC_HIP(hipMalloc(&st->d_cscratch, 64));  // Wolff seed root and target; O(3): direction from byte 8
+  SC_HIP(hipMalloc(&st->d_cscratch, 64));  // Wolff seed root and target; O(N): direction from byte 8
   SC_HIP(hipMalloc(&st->d_cthr, (size_t)st->n_rep * kClusterWeights * sizeof(uint64_t)));
   if (st->csr) {
     const std::vector<Bond> &e = st->lat.edges;
@@ -744,8 +744,6 @@ int prepare_cluster(scamper_state *st) {
 // sweep of the draw counter.
 int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
   const uint64_t n = st->lat.n_sites;
-  if (st->continuous && st->model.spin_dim != 3)
-    return fail(SCAMPER_EUNSUPPORTED, "O(N) cluster updates run with N = 3");
   if (n >= (1ull << 32)) return fail(SCAMPER_EUNSUPPORTED, "cluster updates need fewer than 2^32 sites");
   // the reference asserts J * w >= 0 (models.rs:139-142, 321-324)
   if (st->csr) {
@@ -755,7 +753,7 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
   } else if (st->model.j < 0.0) {
     return fail(SCAMPER_EUNSUPPORTED, "cluster updates require J * bond.weight >= 0");
   }
-  // O(3) bonds depend on the spins: no threshold tables
+  // O(N) bonds depend on the spins: no threshold tables
   if (int rc = st->continuous ? ensure_cluster(st) : prepare_cluster(st)) return rc;
   const std::vector<uint64_t> &thr = st->cthr;
   if (measure) {
@@ -769,9 +767,9 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
     for (uint32_t r = 0; r < st->n_rep; ++r) {
       hipError_t e;
       if (st->continuous) {
-        e = launch_cluster_on3(st->geom, st->d_vec + 2ull * st->model.spin_dim * r * st->geom.nh, st->d_parent,
-                               st->keys[r], step,
-                               st->beta[r], st->model.j, wolff, st->d_cscratch, st->stream);
+        e = launch_cluster_on(st->geom, st->model.spin_dim, st->d_vec + 2ull * st->model.spin_dim * r * st->geom.nh,
+                              st->d_parent, st->keys[r], step, st->beta[r], st->model.j, wolff, st->d_cscratch,
+                              st->stream);
       } else if (st->csr) {
         int8_t *spins = st->d_spins + (uint64_t)r * n;
         e = launch_cluster_bonds_csr(n, st->lat.edges.size(), st->d_cedges, st->d_cweights, spins, st->d_parent,

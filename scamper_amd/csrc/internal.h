@@ -225,11 +225,12 @@ hipError_t launch_cluster_bonds_csr(uint64_t n_sites, uint64_t n_edges, const Cl
 hipError_t launch_cluster_transform(const StencilGeom *g, uint64_t n_sites, int8_t *spins, uint32_t *parent,
                                     int kind, int q, int wolff, uint64_t key, uint64_t step, uint32_t *scratch,
                                     hipStream_t stream);
-// One Swendsen-Wang or Wolff update of O(3) spins on a stencil lattice
-// (`planes`: one replica's FP32 component planes); scratch holds the Wolff
-// seed's root (4 B) and, from byte 8, the update's direction (3 doubles).
-hipError_t launch_cluster_on3(const StencilGeom &g, float *planes, uint32_t *parent, uint64_t key, uint64_t step,
-                              double beta, double j, int wolff, void *scratch, hipStream_t stream);
+// One Swendsen-Wang or Wolff update of O(N) spins, N = dim = 2 or 3, on a
+// stencil lattice (`planes`: one replica's FP32 component planes); scratch
+// holds the Wolff seed's root (4 B) and, from byte 8, the update's direction
+// (dim doubles).
+hipError_t launch_cluster_on(const StencilGeom &g, int dim, float *planes, uint32_t *parent, uint64_t key,
+                             uint64_t step, double beta, double j, int wolff, void *scratch, hipStream_t stream);
 
 // Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
 bool ising_fast_path_ok(const StencilGeom &g);

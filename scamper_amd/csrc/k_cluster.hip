@@ -443,61 +443,74 @@ __global__ __launch_bounds__(kThreads) void k_transform_tile(int8_t *__restrict_
   *at_o = (int16_t)(v[1] | v[3] << 8);
 }
 
-// ------------------------------------------------ O(3) embedding clusters
-// (ClusterModel for ONModel<D>, lattice/models.rs:516-577) on stencil
-// lattices whose spins are FP32 component planes [colour][x, y, z][rank].
-// Arithmetic is f64 in the reference's operation order, and this file is
-// compiled without contraction, so that the bond probabilities, the
-// direction and the reflections round as the oracle's do.
+// ------------------------------------------------ O(N) embedding clusters
+// (ClusterModel for ONModel<D>, lattice/models.rs:516-577), D = 2 or 3, on
+// stencil lattices whose spins are FP32 component planes
+// [colour][component][rank].  Arithmetic is f64 in the reference's operation
+// order (sums left to right), and this file is compiled without contraction,
+// so that the bond probabilities, the direction and the reflections round as
+// the oracle's do.
 
-// The update's direction: rejection from the cube, try k on words
-// 4 + 3k .. 6 + 3k of the seed stream, accepted when 2^-40 < |x|^2 <= 1 and
-// normalised as normalize (models.rs:432-439); 64 rejections in a row
-// (probability 0.476^64) leave (0, 0, 1).
-__global__ void k_on3_direction(uint64_t key, uint64_t step, double *r) {
+template <int D>
+__device__ __forceinline__ double dotn(const double (&a)[D], const double *b) {
+  double acc = a[0] * b[0];
+#pragma unroll
+  for (int c = 1; c < D; ++c) acc = acc + a[c] * b[c];
+  return acc;
+}
+
+// The update's direction: rejection from the square or cube, try k on words
+// 4 + Dk .. 3 + D(k + 1) of the seed stream, accepted when
+// 2^-40 < |x|^2 <= 1 and normalised as normalize (models.rs:432-439); 64
+// rejections in a row (probability 0.215^64 or 0.476^64) leave the last
+// unit axis.
+template <int D>
+__global__ void k_on_direction(uint64_t key, uint64_t step, double *r) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (uint64_t k = 0; k < 64; ++k) {
-    double x[3];
-    for (uint64_t c = 0; c < 3; ++c)
-      x[c] = 2.0 * ((double)cluster_word(key, 4 + 3 * k + c, step, kStreamClusterSeed) * 0x1p-32) - 1.0;
-    const double n2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
+    double x[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      x[c] = 2.0 * ((double)cluster_word(key, 4 + D * k + c, step, kStreamClusterSeed) * 0x1p-32) - 1.0;
+    const double n2 = dotn<D>(x, x);
     if (n2 > 0x1p-40 && n2 <= 1.0) {
       const double norm = sqrt(n2);
-      for (int c = 0; c < 3; ++c) r[c] = x[c] / norm;
+#pragma unroll
+      for (int c = 0; c < D; ++c) r[c] = x[c] / norm;
       return;
     }
   }
-  r[0] = 0.0, r[1] = 0.0, r[2] = 1.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = c == D - 1 ? 1.0 : 0.0;
 }
 
-struct On3Planes {
-  float *v;     // one replica: planes (3 colour + component) * nh
+struct OnPlanes {
+  float *v;     // one replica: planes (D colour + component) * nh
   uint64_t nh;  // ranks per colour
 };
 
-__device__ __forceinline__ void load_on3(const On3Planes &P, uint64_t s, const StencilSite &c, double (&out)[3]) {
+template <int D>
+__device__ __forceinline__ void load_on(const OnPlanes &P, uint64_t s, const StencilSite &c, double (&out)[D]) {
   const uint64_t col = (c.x + c.y + c.z) & 1u, j = s >> 1;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) out[k] = (double)P.v[(3 * col + k) * P.nh + j];
-}
-
-__device__ __forceinline__ double dot3(const double (&a)[3], const double *b) {
-  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+  for (int k = 0; k < D; ++k) out[k] = (double)P.v[(D * col + k) * P.nh + j];
 }
 
 // Edges e = nd s + a to +1 along axis a (build_hypercubic order), weight 1:
 // active when word(11, e) 2^-32 < p = 1 - exp(-2 beta J (s_i.r)(s_j.r)) for
 // a positive product (cluster_bond_probability, models.rs:537-558).
-template <int ND>
-__global__ void k_bonds_on3(On3Planes P, uint32_t *parent, uint32_t lx, uint32_t ly, uint32_t lz, uint64_t key,
-                            uint64_t step, double beta, double coupling, const double *__restrict__ r_) {
+template <int D, int ND>
+__global__ void k_bonds_on(OnPlanes P, uint32_t *parent, uint32_t lx, uint32_t ly, uint32_t lz, uint64_t key,
+                           uint64_t step, double beta, double coupling, const double *__restrict__ r_) {
   const uint64_t n = 2 * P.nh;
-  const double r[3] = {r_[0], r_[1], r_[2]};
+  double r[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = r_[c];
   for (uint64_t s = thread_id(); s < n; s += grid_threads()) {
     const StencilSite c = stencil_site(s, lx, ly);
-    double v[3];
-    load_on3(P, s, c, v);
-    const double ps = dot3(v, r);
+    double v[D];
+    load_on<D>(P, s, c, v);
+    const double ps = dotn<D>(v, r);
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
       StencilSite t = c;
@@ -512,9 +525,9 @@ __global__ void k_bonds_on3(On3Planes P, uint32_t *parent, uint32_t lx, uint32_t
         t.z = c.z + 1 == lz ? 0u : c.z + 1;
         ts = s + ((uint64_t)t.z - c.z) * (uint64_t)lx * ly;
       }
-      double w[3];
-      load_on3(P, ts, t, w);
-      const double product = ps * dot3(w, r);
+      double w[D];
+      load_on<D>(P, ts, t, w);
+      const double product = ps * dotn<D>(w, r);
       if (coupling == 0.0 || product <= 0.0) continue;
       double p = 1.0 - exp(-2.0 * beta * coupling * product);
       p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);  // clamp_probability; p is finite here
@@ -524,7 +537,7 @@ __global__ void k_bonds_on3(On3Planes P, uint32_t *parent, uint32_t lx, uint32_t
   }
 }
 
-__global__ void k_on3_seed(uint32_t *parent, uint64_t n_sites, uint64_t key, uint64_t step, uint32_t *out) {
+__global__ void k_on_seed(uint32_t *parent, uint64_t n_sites, uint64_t key, uint64_t step, uint32_t *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint64_t seed = ((uint64_t)cluster_word(key, 0, step, kStreamClusterSeed) * n_sites) >> 32;
   out[0] = find_root(parent, (uint32_t)seed);
@@ -533,36 +546,61 @@ __global__ void k_on3_seed(uint32_t *parent, uint64_t n_sites, uint64_t key, uin
 // Reflection about r (transform_cluster_spin, models.rs:560-575), renormalised
 // in f64 and stored as FP32: Swendsen-Wang reflects a cluster when its root's
 // target word is below 2^31 (sw_cluster_auxiliary's coin), Wolff the seed's.
-__global__ void k_transform_on3(On3Planes P, uint32_t *parent, uint32_t lx, uint32_t ly, int wolff, uint64_t key,
-                                uint64_t step, const uint32_t *__restrict__ seed, const double *__restrict__ r_) {
+template <int D>
+__global__ void k_transform_on(OnPlanes P, uint32_t *parent, uint32_t lx, uint32_t ly, int wolff, uint64_t key,
+                               uint64_t step, const uint32_t *__restrict__ seed, const double *__restrict__ r_) {
   const uint64_t n = 2 * P.nh;
   const uint32_t seed_root = wolff ? seed[0] : 0u;
-  const double r[3] = {r_[0], r_[1], r_[2]};
+  double r[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = r_[c];
   for (uint64_t s = thread_id(); s < n; s += grid_threads()) {
     const uint32_t root = find_root(parent, (uint32_t)s);
     const bool flip = wolff ? root == seed_root
                             : cluster_word(key, root, step, kStreamClusterTarget) < 0x80000000u;
     if (!flip) continue;
     const StencilSite c = stencil_site(s, lx, ly);
-    double v[3];
-    load_on3(P, s, c, v);
-    const double proj = dot3(v, r);
-    double u[3];
+    double v[D];
+    load_on<D>(P, s, c, v);
+    const double proj = dotn<D>(v, r);
+    double u[D];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) u[k] = v[k] - 2.0 * proj * r[k];
-    const double norm = sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    for (int k = 0; k < D; ++k) u[k] = v[k] - 2.0 * proj * r[k];
+    const double norm = sqrt(dotn<D>(u, u));
     if (norm > 1e-15)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) u[k] /= norm;
+      for (int k = 0; k < D; ++k) u[k] /= norm;
     const uint64_t col = (c.x + c.y + c.z) & 1u, j = s >> 1;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) P.v[(3 * col + k) * P.nh + j] = (float)u[k];
+    for (int k = 0; k < D; ++k) P.v[(D * col + k) * P.nh + j] = (float)u[k];
   }
 }
 
 uint32_t blocks_for(uint64_t n) {
   const uint64_t b = (n + kThreads - 1) / kThreads;
   return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
+template <int D>
+void launch_on_update(const StencilGeom &g, float *planes, uint32_t *parent, uint64_t key, uint64_t step,
+                      double beta, double j, int wolff, uint32_t *seed, double *r, hipStream_t stream) {
+  const uint64_t n = 2 * g.nh;
+  const uint32_t lx = 2 * g.lx2;
+  const OnPlanes P{planes, g.nh};
+  hipLaunchKernelGGL(k_on_direction<D>, dim3(1), dim3(64), 0, stream, key, step, r);
+  hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n)), dim3(kThreads), 0, stream, parent, n);
+  if (g.nd == 1)
+    hipLaunchKernelGGL((k_bonds_on<D, 1>), dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, g.lz,
+                       key, step, beta, j, r);
+  else if (g.nd == 2)
+    hipLaunchKernelGGL((k_bonds_on<D, 2>), dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, g.lz,
+                       key, step, beta, j, r);
+  else
+    hipLaunchKernelGGL((k_bonds_on<D, 3>), dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, g.lz,
+                       key, step, beta, j, r);
+  if (wolff) hipLaunchKernelGGL(k_on_seed, dim3(1), dim3(64), 0, stream, parent, n, key, step, seed);
+  hipLaunchKernelGGL(k_transform_on<D>, dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, wolff,
+                     key, step, seed, r);
 }
 
 // Stencils labelled tile by tile: 2-D or 3-D with every side a multiple of
@@ -610,27 +648,13 @@ hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spin
   return hipGetLastError();
 }
 
-hipError_t launch_cluster_on3(const StencilGeom &g, float *planes, uint32_t *parent, uint64_t key, uint64_t step,
-                              double beta, double j, int wolff, void *scratch, hipStream_t stream) {
-  const uint64_t n = 2 * g.nh;
-  const uint32_t lx = 2 * g.lx2;
+hipError_t launch_cluster_on(const StencilGeom &g, int dim, float *planes, uint32_t *parent, uint64_t key,
+                             uint64_t step, double beta, double j, int wolff, void *scratch, hipStream_t stream) {
   uint32_t *seed = static_cast<uint32_t *>(scratch);
   double *r = reinterpret_cast<double *>(static_cast<char *>(scratch) + 8);
-  const On3Planes P{planes, g.nh};
-  hipLaunchKernelGGL(k_on3_direction, dim3(1), dim3(64), 0, stream, key, step, r);
-  hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n)), dim3(kThreads), 0, stream, parent, n);
-  if (g.nd == 1)
-    hipLaunchKernelGGL(k_bonds_on3<1>, dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, g.lz, key,
-                       step, beta, j, r);
-  else if (g.nd == 2)
-    hipLaunchKernelGGL(k_bonds_on3<2>, dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, g.lz, key,
-                       step, beta, j, r);
-  else
-    hipLaunchKernelGGL(k_bonds_on3<3>, dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, g.lz, key,
-                       step, beta, j, r);
-  if (wolff) hipLaunchKernelGGL(k_on3_seed, dim3(1), dim3(64), 0, stream, parent, n, key, step, seed);
-  hipLaunchKernelGGL(k_transform_on3, dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, wolff, key,
-                     step, seed, r);
+  if (dim == 2) launch_on_update<2>(g, planes, parent, key, step, beta, j, wolff, seed, r, stream);
+  else if (dim == 3) launch_on_update<3>(g, planes, parent, key, step, beta, j, wolff, seed, r, stream);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -10,10 +10,10 @@ bond), for Ising and Potts, with several replicas at different
 temperatures, and the measured per-update energy and magnetisation equal the
 oracle's.  Statistics: Swendsen-Wang and Wolff chains of the 2-D Ising model
 at beta_c on a 32 x 32 torus reproduce Kaufman's exact finite-torus energy.
-O(3) spins take Wolff's embedding clusters: from the same FP32 spins the
-device's reflected spins equal the oracle's bit for bit (both compute in f64
-in the reference's operation order and round to FP32), and device chains on
-a periodic ring reproduce the exact transfer-matrix energy.  J w < 0 is
+O(3) and O(2) spins take Wolff's embedding clusters: from the same FP32
+spins the device's reflected spins equal the oracle's bit for bit (both
+compute in f64 in the reference's operation order and round to FP32), and
+device chains on periodic rings reproduce the exact transfer-matrix energies.  J w < 0 is
 refused with EUNSUPPORTED, as the reference asserts (models.rs:139-142,
 321-324, 549-553).
 """
@@ -142,13 +142,14 @@ def test_energy_matches_kaufman_at_criticality(sc, alg):
 
 @pytest.mark.parametrize("dims", [(8, 6, 4), (16, 16), (14,)])
 @pytest.mark.parametrize("wolff", [False, True])
-def test_on3_bit_exact_against_oracle(sc, orc, dims, wolff):
+@pytest.mark.parametrize("n_comp", [3, 2])
+def test_on3_bit_exact_against_oracle(sc, orc, dims, wolff, n_comp):
     seed, n_rep, steps = 23, 3, 6
     betas = [0.3, 0.8, 1.5]
-    print(f"case O(3) {dims} wolff={wolff}", flush=True)
+    print(f"case O({n_comp}) {dims} wolff={wolff}", flush=True)
     lat, olat = sc.Lattice.hypercubic(list(dims)), orc.Lattice.hypercubic(list(dims), True)
-    om = orc.model(orc.ON, 1.0, 2, 3)
-    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 3), n_rep, seed)
+    om = orc.model(orc.ON, 1.0, 2, n_comp)
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, n_comp), n_rep, seed)
     for r in range(n_rep):
         st.init(True, r)
         st.set_beta(betas[r], r)
@@ -176,6 +177,30 @@ def test_on3_ring_energy_matches_transfer_matrix(sc, alg):
     il, dil = spherical_in(ls, beta), spherical_in(ls, beta, derivative=True)
     want = -n * np.sum((2 * ls + 1) * il ** (n - 1) * dil) / np.sum((2 * ls + 1) * il**n)
     st = sc.State(sc.Lattice.hypercubic([n]), sc.model(sc.ON, 1.0, 2, 3), 4, 29)
+    for r in range(4):
+        st.init(True, r)
+        st.set_beta(beta, r)
+    a = sc.SWENDSEN_WANG if alg == "sw" else sc.WOLFF
+    st.sweep(a, 500)
+    st.sweep(a, 8000, measure=True)
+    zs = []
+    for r in range(4):
+        e, _ = st.series(r)
+        est = sc.binned_estimate(e, 200)
+        zs.append((est.mean - want) / est.stderr)
+    print(alg, want, zs)
+    assert all(abs(z) < 4.5 for z in zs), zs
+
+
+@pytest.mark.parametrize("alg", ["sw", "wolff"])
+def test_xy_ring_energy_matches_transfer_spectrum(sc, alg):
+    """XY ring of 8 sites: Z = sum_m I_m(K)^N over the transfer spectrum."""
+    from scipy.special import iv
+    n, beta = 8, 1.1
+    m = np.arange(-60, 61)
+    i, di = iv(m, beta), 0.5 * (iv(m - 1, beta) + iv(m + 1, beta))
+    want = -n * np.sum(i ** (n - 1) * di) / np.sum(i**n)
+    st = sc.State(sc.Lattice.hypercubic([n]), sc.model(sc.ON, 1.0, 2, 2), 4, 31)
     for r in range(4):
         st.init(True, r)
         st.set_beta(beta, r)

@@ -222,8 +222,8 @@ def test_xy_ring_energy_exact(sc):
     assert abs(z) < 4.5
 
 
-def test_xy_clusters_unsupported(sc):
+def test_xy_rejects_other_components(sc):
     lat = sc.Lattice.hypercubic([8, 8])
-    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 2), 1, 1)
-    with pytest.raises(sc.ScamperError):
-        st.sweep(5, 1)
+    for n_comp in (1, 4):
+        with pytest.raises(sc.ScamperError):
+            sc.State(lat, sc.model(sc.ON, 1.0, 2, n_comp), 1, 1)

@@ -328,3 +328,105 @@ def test_on3_weighted_tree_samples_boltzmann(orc, wolff):
     chains = _on3_chain_energy(orc, lat, beta, wolff, range(4), 12000)
     zs = [(orc.binned_estimate(e, 200).mean - want) / orc.binned_estimate(e, 200).stderr for e in chains]
     assert np.all(np.abs(zs) < 4.5), (want, zs)
+
+
+# ------------------------------------------------------- O(2) embedding clusters
+
+def _direction2_py(key, step):
+    """The rejection draw from the square: try k on seed-stream words 4 + 2k, 5 + 2k."""
+    for k in range(64):
+        x = [2.0 * (_word_py(key, 4 + 2 * k + c, step, 13) * 2.0**-32) - 1.0 for c in range(2)]
+        n2 = x[0] * x[0] + x[1] * x[1]
+        if 2.0**-40 < n2 <= 1.0:
+            n = math.sqrt(n2)
+            return [v / n for v in x]
+    return [0.0, 1.0]
+
+
+def test_on2_direction_is_the_rejection_draw_and_uniform(orc):
+    key = orc.replica_key(7, 2)
+
+    def draw(step):
+        r = np.zeros(2)
+        orc.lib().orc_on_direction(2, key, step, orc._p(r))
+        return r
+
+    for step in [0, 1, 5, 2**33 + 3]:
+        assert list(draw(step)) == _direction2_py(key, step)  # bit for bit
+    rs = np.array([draw(t) for t in range(20000)])
+    n = len(rs)
+    assert np.allclose(np.linalg.norm(rs, axis=1), 1.0, atol=2e-16)
+    assert np.all(np.abs(rs.mean(axis=0)) < 4.5 * math.sqrt(0.5 / n))  # uniform on S^1: <r> = 0
+    # <cos^2> = 1/2 with variance 3/8 - 1/4; <cos 4 phi> = 0 (no square anisotropy)
+    assert np.all(np.abs((rs**2).mean(axis=0) - 0.5) < 4.5 * math.sqrt(1 / 8 / n))
+    c4 = np.cos(4 * np.arctan2(rs[:, 1], rs[:, 0]))
+    assert abs(c4.mean()) < 4.5 * math.sqrt(0.5 / n)
+    # the three-component draw is unchanged by the generalisation
+    r3 = np.zeros(3)
+    orc.lib().orc_on_direction(3, key, 9, orc._p(r3))
+    assert list(r3) == _direction_py(key, 9)
+
+
+def test_on2_bond_probability_reduces_to_the_ising_embedding(orc):
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        phi = rng.uniform(0, 2 * math.pi)
+        r = np.array([math.cos(phi), math.sin(phi)])
+        beta, w = rng.uniform(0.05, 2.0), rng.uniform(0.2, 3.0)
+        bp = lambda a, b, j=1.0: orc.lib().orc_on_bond_probability(  # noqa: E731
+            2, j, w, beta, orc._p(np.ascontiguousarray(a)), orc._p(np.ascontiguousarray(b)), orc._p(r))
+        assert orc.lib().orc_cluster_threshold(orc.ISING, 1.0, w, beta) == math.ceil(bp(r, r) * 2**32)
+        assert bp(r, -r) == 0.0
+        perp = np.array([-r[1], r[0]])
+        assert bp(perp, r) <= 1e-15
+        a, b = rng.normal(size=2), rng.normal(size=2)
+        a, b = a / np.linalg.norm(a), b / np.linalg.norm(b)
+        assert bp(a, b) == bp(b, a) and bp(a, b, 0.0) == 0.0
+        # the product of projections: p = 1 - exp(-2 beta w (a.r)(b.r))
+        pa, pb = a @ r, b @ r
+        want = 1 - math.exp(-2 * beta * w * pa * pb) if pa * pb > 0 else 0.0
+        assert abs(bp(a, b) - want) < 1e-14
+
+
+def _xy_ring_energy(n, k):
+    from scipy.special import iv
+    m = np.arange(-60, 61)
+    i, di = iv(m, k), 0.5 * (iv(m - 1, k) + iv(m + 1, k))
+    return -n * np.sum(i ** (n - 1) * di) / np.sum(i**n)
+
+
+def _on2_chain_energy(orc, lat, beta, wolff, seeds, steps, burn=500):
+    m = orc.model(orc.ON, 1.0, 2, 2)
+    out = []
+    for seed in seeds:
+        key = orc.replica_key(seed, 0)
+        s = orc.random_start(lat, m, key)
+        es = []
+        for t in range(steps):
+            orc.cluster_steps(lat, m, s, beta, key, t, 1, wolff)
+            if t >= burn:
+                es.append(orc.lib().orc_total_energy(orc.C.byref(m), lat.ptr, orc._p(s)))
+        out.append(np.array(es))
+    return out
+
+
+@pytest.mark.parametrize("wolff", [False, True])
+def test_on2_ring_samples_boltzmann(orc, wolff):
+    n, beta = 5, 1.0
+    want = _xy_ring_energy(n, beta)
+    chains = _on2_chain_energy(orc, orc.Lattice.hypercubic([n], True), beta, wolff, range(4), 12000)
+    zs = [(orc.binned_estimate(e, 200).mean - want) / orc.binned_estimate(e, 200).stderr for e in chains]
+    assert np.all(np.abs(zs) < 4.5), (want, zs)
+
+
+@pytest.mark.parametrize("wolff", [False, True])
+def test_on2_weighted_tree_samples_boltzmann(orc, wolff):
+    """On a tree every bond is independent: <E> = -sum_b J w_b I1/I0(beta J w_b)."""
+    from scipy.special import iv
+    src, dst, w = [0, 1, 1, 3], [1, 2, 3, 4], [1.0, 2.0, 0.5, 1.5]
+    beta = 0.7
+    lat = orc.Lattice.from_edges(5, src, dst, w)
+    want = -sum(wb * iv(1, beta * wb) / iv(0, beta * wb) for wb in w)
+    chains = _on2_chain_energy(orc, lat, beta, wolff, range(4), 12000)
+    zs = [(orc.binned_estimate(e, 200).mean - want) / orc.binned_estimate(e, 200).stderr for e in chains]
+    assert np.all(np.abs(zs) < 4.5), (want, zs)
