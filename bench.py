@@ -15,6 +15,7 @@ as plain per-job launches on the same GPU.
 Other workloads, one JSON line each (not the headline):
     --workload pt3d   64-replica multi-spin parallel tempering, 3-D L=128
     --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
+    --workload xy     32-replica O(2) (XY) von Mises heat bath + over-relaxation, 3-D L=96
     --workload potts3d  the tiled Potts kernel alone: q=3, 3-D L=256, measured sweeps
     --workload cluster  Swendsen-Wang and Wolff updates of the 2-D Ising model at
                         beta_c, L=4096, and of the 3-D O(3) model near T_c, L=96
@@ -123,6 +124,25 @@ def cpu_on3() -> dict:
         ch.over_relaxation()
     return _cpu_timed(step, 2 * lat.n_sites, "site-updates/ns",
                       "heat-bath + over-relaxation sweeps of a 24^3 periodic lattice (f64, CSR)")
+
+
+def cpu_xy() -> dict:
+    """The XY heat bath (Best-Fisher von Mises draws) and over-relaxation in
+    the reference's layout (f64 site-major spins over the CSR lattice,
+    orc_sweep_on2_heat_bath and orc_sweep_over_relaxation), 3-D L=24 at
+    T = 2.15."""
+    import oracle
+
+    lat = oracle.Lattice.hypercubic([24, 24, 24], True)
+    m = oracle.model(oracle.ON, 1.0, 2, 2)
+    key = oracle.replica_key(42, 0)
+    ch = oracle.Chain(lat, m, oracle.random_start(lat, m, key))
+
+    def step(i):
+        ch.on2_heat_bath(1 / 2.15, key, i)
+        ch.over_relaxation()
+    return _cpu_timed(step, 2 * lat.n_sites, "site-updates/ns",
+                      "XY heat-bath + over-relaxation sweeps of a 24^3 periodic lattice (f64, CSR)")
 
 
 def cpu_potts3d() -> dict:
@@ -286,6 +306,33 @@ def on3(sc, args) -> dict:
             **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_on3()})}
 
 
+def xy(sc, args) -> dict:
+    """32 replicas of the 3-D XY model, L=96, T = 2.15 + 0.1 r/31 (around
+    T_c = 2.2018); each step one von Mises heat-bath sweep then one
+    over-relaxation sweep."""
+    L, R = 96, 32
+    lat = sc.Lattice.hypercubic([L, L, L], True)
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 2), R, 42)
+    for r in range(R):
+        st.init(True, r)
+        st.set_beta(1.0 / (2.15 + 0.1 * r / 31), r)
+    st.sweep(sc.HYBRID_HB_OR, args.warmup)
+    st.synchronize()
+    ms = st.sweep_timed(sc.HYBRID_HB_OR, args.on3_steps)
+    n = lat.n_sites
+    updates = 2 * R * n * args.on3_steps
+    # per site-update: read own 8 B + neighbours' planes 8 B (with reuse) + write 8 B = 24 B
+    gbs = 24 * updates / (ms * 1e-3) / 1e9
+    return {"metric": "site-updates/ns (3-D XY L=96, 32 replicas, HB+OR)",
+            "value": round(updates / (ms * 1e6), 3), "unit": "site-updates/ns", "n_gpus": 1,
+            "steps": args.on3_steps, "warmup": args.warmup, "ms_per_step": ms / args.on3_steps,
+            "higher_is_better": True, "dtype": "f32 spins, f64 draws and observables", "data": "synthetic",
+            "config": {"workload": "xy_hybrid", "L": L, "replicas": R, "T": [2.15, 2.25], "seed": 42},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_xy()})}
+
+
 def potts3d(sc, args) -> dict:
     """The serving pool's kernel on one large lattice through the plain state
     path (no graphs, so a profiler sees every launch): 3-state Potts, 3-D
@@ -383,7 +430,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=16384)
-    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "potts3d", "cluster", "serve", "calibrate"],
+    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "xy", "potts3d", "cluster", "serve", "calibrate"],
                     default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -408,6 +455,8 @@ def main() -> int:
         line = pt3d(sc, args)
     elif args.workload == "on3":
         line = on3(sc, args)
+    elif args.workload == "xy":
+        line = xy(sc, args)
     elif args.workload == "potts3d":
         line = potts3d(sc, args)
     elif args.workload == "cluster":
