@@ -350,6 +350,9 @@ __global__ __launch_bounds__(kBlock) void k_on3_init(float *__restrict__ spins, 
 // stream kStreamVonMises (word 0 the angle pi u, word 1 the acceptance
 // uniform, word 2 the sign of the accepted try); kappa < 1e-8 takes 2 pi u
 // from word 0 of try 0; a 64th rejection in a row keeps that try's value.
+// The new spin (cos theta, sin theta), theta = a + atan2(hy, hx), is formed
+// as the field's unit direction rotated by a, from cos a and sin a, which
+// equals the angle form to f64 rounding and costs no acos, atan2, cos or sin.
 
 struct V2 {
   float x, y;
@@ -369,8 +372,15 @@ __device__ __forceinline__ U4 vm_block(const ReplicaParams &P, uint64_t j, uint6
                        (uint32_t)(j >> 32) | (k << 20), (uint32_t)P.key, (uint32_t)(P.key >> 32));
 }
 
-__device__ double von_mises(const ReplicaParams &P, uint64_t j, uint64_t sweep, uint32_t colour, double kappa) {
-  if (kappa < 1e-8) return (double)vm_block(P, j, sweep, colour, 0).x * kTwoM32 * kTau;
+// The von Mises draw as (cos a, sin a): the accepted try's value is cos a
+// itself, and sin a = +-sqrt((1 - v)(1 + v)) with the sign word's sign, so
+// the angle a = +-acos(v) is never formed; kappa < 1e-8 gives a = 2 pi u.
+__device__ void von_mises_cs(const ReplicaParams &P, uint64_t j, uint64_t sweep, uint32_t colour, double kappa,
+                             double *ca, double *sa) {
+  if (kappa < 1e-8) {
+    sincos((double)vm_block(P, j, sweep, colour, 0).x * kTwoM32 * kTau, sa, ca);
+    return;
+  }
   const double tau = 1.0 + sqrt(1.0 + 4.0 * kappa * kappa);
   const double rho = (tau - sqrt(2.0 * tau)) / (2.0 * kappa);
   const double r = (1.0 + rho * rho) / (2.0 * rho);
@@ -385,8 +395,10 @@ __device__ double von_mises(const ReplicaParams &P, uint64_t j, uint64_t sweep, 
     sign = w.z;
     if (c * (2.0 - c) > u || log(c / u) + 1.0 >= c) break;
   }
-  const double a = acos(fmin(fmax(value, -1.0), 1.0));
-  return sign < 0x80000000u ? a : -a;
+  const double v = fmin(fmax(value, -1.0), 1.0);
+  const double sn = sqrt((1.0 - v) * (1.0 + v));
+  *ca = v;
+  *sa = sign < 0x80000000u ? sn : -sn;
 }
 
 // ONModel<2>::random_unit_vector (models.rs:405-422) in f64 from two words.
@@ -445,9 +457,12 @@ __global__ __launch_bounds__(kBlock) void k_on2(float *__restrict__ spins, const
                                     (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32));
         random_unit2(fb.x, fb.y, out);
       } else {
-        const double theta = von_mises(P, j, sweep, colour, P.beta * norm) + atan2(hy, hx);
-        out[0] = cos(theta);
-        out[1] = sin(theta);
+        // theta = a + atan2(hy, hx) as a rotation of the field's direction by a
+        double ca, sa;
+        von_mises_cs(P, j, sweep, colour, P.beta * norm, &ca, &sa);
+        const double ux = hx / norm, uy = hy / norm;
+        out[0] = ca * ux - sa * uy;
+        out[1] = sa * ux + ca * uy;
       }
       ns = V2{(float)out[0], (float)out[1]};
     } else {
