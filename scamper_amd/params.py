@@ -157,7 +157,8 @@ def _finite_coupling(params: Mapping) -> float:
 def model_from_params(kind: int, params: Mapping, spin_dim: int = 3):
     """FromHamiltonianParams (classical_mc.rs:506-529) for ISING, POTTS (q
     before J, as the reference reads them) and ON (spin_dim >= 2; the
-    device runs N = 3)."""
+    device runs N = 2 and N = 3 and refuses others when the state is
+    created)."""
     if kind == ISING:
         return model(ISING, _finite_coupling(params))
     if kind == POTTS:

@@ -18,10 +18,14 @@ pytestmark = pytest.mark.gpu
      lambda: (sc.Lattice.kagome(4, 4), sc.model(sc.POTTS, 1.5, 4))),
     ({"Lx": 8, "Ly": 8, "Lz": 4, "beta": 0.7}, sc.ON,
      lambda: (sc.Lattice.hypercubic([8, 8, 4]), sc.model(sc.ON, 1.0, 2, 3))),
+    ({"lattice_type": "square", "Lx": 16, "beta": 1.1, "J": 0.5}, -2,
+     lambda: (sc.Lattice.hypercubic([16, 16]), sc.model(sc.ON, 0.5, 2, 2))),
 ])
 def test_state_from_params_equals_direct_state(params, kind, direct):
+    n_comp = 2 if kind == -2 else 3
+    kind = sc.ON if kind == -2 else kind
     alg = sc.CONT_HEAT_BATH if kind == sc.ON else sc.METROPOLIS
-    a = state_from_params(params, kind, seed=9, n_replicas=2)
+    a = state_from_params(params, kind, seed=9, n_replicas=2, spin_dim=n_comp)
     lat, m = direct()
     b = sc.State(lat, m, 2, 9)
     for r in range(2):

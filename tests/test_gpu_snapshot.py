@@ -28,6 +28,8 @@ CASES = [
     (lambda sc: sc.Lattice.square(16, 16), lambda sc: sc.model(sc.POTTS, 1.0, 3), 1, 1.005052, 12, 20, 30),
     (lambda sc: sc.Lattice.triangular(6, 6), lambda sc: sc.model(sc.ISING, 1.0), 0, 0.3, 13, 50, 70),
     (lambda sc: sc.Lattice.cubic(4), lambda sc: sc.model(sc.ON, 1.0, 2, 3), 4, 0.7, 14, 10, 15),
+    (lambda sc: sc.Lattice.square(8, 8), lambda sc: sc.model(sc.ON, 1.0, 2, 2), 4, 1.1, 15, 10, 15),
+    (lambda sc: sc.Lattice.square(8, 8), lambda sc: sc.model(sc.ON, 1.0, 2, 2), 5, 1.1, 16, 10, 15),
 ]
 
 
@@ -80,16 +82,18 @@ def test_invalid_spins_rejected_and_state_kept(sc):
     assert np.array_equal(st.download_f64(), keep) and st.get_beta() == 0.8
 
 
-def test_o3_snapshot_meets_the_reference_norm_rule(sc):
-    """Exported O(3) spins pass Hamiltonian::validate_configuration's
+@pytest.mark.parametrize("n_comp", [3, 2])
+def test_o3_snapshot_meets_the_reference_norm_rule(sc, n_comp):
+    """Exported O(N) spins (N = 3, 2) pass Hamiltonian::validate_configuration's
     |norm^2 - 1| <= 1e-10 D (models.rs:454-473); the exact FP32 copy agrees,
     and a document without it (as the reference writes one) loads too."""
-    lat, m = sc.Lattice.cubic(4), sc.model(sc.ON, 1.0, 2, 3)
+    lat, m = sc.Lattice.cubic(4), sc.model(sc.ON, 1.0, 2, n_comp)
     st = _state(sc, lat, m, 0.7, 21)
     st.init(True)
     st.sweep(4, 5)
     snap = st.save_snapshot()
-    v = np.asarray(snap["spins"]).reshape(-1, 3)
+    assert snap["spin_dim"] == n_comp
+    v = np.asarray(snap["spins"]).reshape(-1, n_comp)
     assert np.all(np.abs((v * v).sum(axis=1) - 1.0) <= 3e-10)
     f = np.asarray(snap["scamper"]["spins_f32"])
     assert np.array_equal(f.astype(np.float32), st.download_f32())
