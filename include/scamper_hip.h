@@ -83,7 +83,9 @@ enum { SCAMPER_ISING = 0, SCAMPER_POTTS = 1, SCAMPER_ON = 2 };
 
 /* IsingModel{j} (models.rs:29-33), PottsModel{j,q} (models.rs:190-194),
  * ONModel<D>{j} (models.rs:384-388; spin_dim = D: D = 2, the XY model, and
- * D = 3, the Heisenberg model, on periodic hypercubic lattices). */
+ * D = 3, the Heisenberg model, on any lattice: periodic hypercubic ones with
+ * even sides as stencils, every other CSR lattice, with any finite bond
+ * weights, class by class). */
 typedef struct {
   int32_t kind;
   int32_t q;

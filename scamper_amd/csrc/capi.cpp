@@ -76,6 +76,9 @@ struct scamper_state {
   double self_loop_weight = 0.0;  // sum of w over self-loops (constant energy)
   uint32_t *d_offsets = nullptr, *d_neighbors = nullptr, *d_class_sites = nullptr;
   int8_t *d_weights = nullptr;
+  // O(N) on a graph lattice: f64 weights of the non-loop incidences
+  OnCsrGeom ogeom{};
+  double *d_fweights = nullptr;
   hipStream_t stream = nullptr;
   int8_t *d_spins = nullptr;       // d_spins_alloc + kSpinSlack
   int8_t *d_spins_alloc = nullptr;
@@ -103,6 +106,7 @@ struct scamper_state {
   uint64_t *d_cthr = nullptr;
   ClusterEdge *d_cedges = nullptr;
   int8_t *d_cweights = nullptr;
+  double *d_cweights_f64 = nullptr;  // O(N) on a graph lattice: f64 edge weights
   std::vector<uint64_t> cthr;  // host copy of d_cthr, valid while cthr_ready
   bool cthr_ready = false;
   int device = 0;
@@ -115,6 +119,8 @@ struct scamper_state {
     hipFree(d_cthr);
     hipFree(d_cedges);
     hipFree(d_cweights);
+    hipFree(d_cweights_f64);
+    hipFree(d_fweights);
     hipFree(d_vec);
     hipFree(d_params);
     hipFree(d_hbexp);
@@ -291,6 +297,14 @@ int download_i8_impl(scamper_state *st, uint32_t replica, int8_t *spins) {
   return SCAMPER_OK;
 }
 
+// Adds the O(N) counters of every replica into em (kEmStride doubles each).
+hipError_t measure_on(const scamper_state *st, double *em) {
+  return st->csr ? launch_on_csr_measure(st->ogeom, st->model.spin_dim, st->d_vec, st->n_rep, st->model.j, em,
+                                         st->stream)
+                 : launch_on3_measure(st->geom, st->model.spin_dim, st->d_vec, st->n_rep, (float)st->model.j, em,
+                                      st->stream);
+}
+
 // Energy and magnetisation from the accumulated counters of one replica.
 // On the stencil every bond is counted once (from its colour-1 end); on a
 // graph both ends count it, and self-loops add the constant -J w.
@@ -301,7 +315,8 @@ void observables_from_em(const scamper_state *st, const long long *em, double *e
     // is |sum s| / N (models.rs:500-514)
     double d[4];
     std::memcpy(d, em, sizeof d);
-    if (e) *e = -d[0];
+    // a graph counts every bond from both ends; self-loops add -J w (s.s = 1)
+    if (e) *e = st->csr ? -0.5 * d[0] - st->model.j * st->self_loop_weight : -d[0];
     if (m) *m = std::sqrt(d[1] * d[1] + d[2] * d[2] + d[3] * d[3]) / n;
     return;
   }
@@ -446,10 +461,8 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   if (model->kind != kIsing && model->kind != kPotts && model->kind != kON)
     return fail(SCAMPER_EINVAL, "unknown model kind %d", model->kind);
   const Lattice &l = lh->lat;
-  if (model->kind == kON && ((model->spin_dim != 2 && model->spin_dim != 3) || !l.stencil))
-    return fail(SCAMPER_EUNSUPPORTED,
-                "O(N) spins run with N = 2 or 3 on periodic hypercubic lattices with even sides (got N = %d)",
-                model->spin_dim);
+  if (model->kind == kON && model->spin_dim != 2 && model->spin_dim != 3)
+    return fail(SCAMPER_EUNSUPPORTED, "O(N) spins run with N = 2 or 3 (got N = %d)", model->spin_dim);
   auto st = std::unique_ptr<scamper_state>(new (std::nothrow) scamper_state);
   if (!st) return fail(SCAMPER_ENOMEM, "state handle");
   st->model = *model;
@@ -470,6 +483,41 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
     st->geom.nh = l.n_sites / 2;
     set_divisors(st->geom);
     st->z = 2 * l.nd;
+  } else if (model->kind == kON) {
+    // Graph lattice with O(N) spins: any finite weights, self-loops kept
+    // aside (constant energy, never in a local field).
+    if (l.n_sites >= (1ull << 32) || l.neighbors.size() >= (1ull << 32))
+      return fail(SCAMPER_EUNSUPPORTED, "graph lattices are limited to 2^32 sites and incidences");
+    st->csr = true;
+    st->lat = l;
+    std::vector<uint32_t> off(l.n_sites + 1, 0), nb;
+    std::vector<double> w;
+    nb.reserve(l.neighbors.size());
+    w.reserve(l.neighbors.size());
+    for (uint64_t s = 0; s < l.n_sites; ++s) {
+      for (uint64_t i = l.offsets[s]; i < l.offsets[s + 1]; ++i) {
+        const Bond &b = l.edges[l.edge_ids[i]];
+        if (b.source == b.target) continue;
+        nb.push_back((uint32_t)l.neighbors[i]);
+        w.push_back(b.weight);
+      }
+      off[s + 1] = (uint32_t)nb.size();
+    }
+    for (const Bond &b : l.edges)
+      if (b.source == b.target) st->self_loop_weight += b.weight;
+    std::vector<uint32_t> cls(l.class_sites.begin(), l.class_sites.end());
+    st->class_start = l.class_start;
+    SC_HIP(hipMalloc(&st->d_offsets, off.size() * 4));
+    SC_HIP(hipMalloc(&st->d_neighbors, std::max<size_t>(nb.size(), 1) * 4));
+    SC_HIP(hipMalloc(&st->d_fweights, std::max<size_t>(w.size(), 1) * sizeof(double)));
+    SC_HIP(hipMalloc(&st->d_class_sites, std::max<size_t>(cls.size(), 1) * 4));
+    SC_HIP(hipMemcpy(st->d_offsets, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    if (!nb.empty()) {
+      SC_HIP(hipMemcpy(st->d_neighbors, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
+      SC_HIP(hipMemcpy(st->d_fweights, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (!cls.empty()) SC_HIP(hipMemcpy(st->d_class_sites, cls.data(), cls.size() * 4, hipMemcpyHostToDevice));
+    st->ogeom = OnCsrGeom{l.n_sites, st->d_offsets, st->d_neighbors, st->d_fweights, st->d_class_sites};
   } else {
     // Graph lattice: integer weights |w| <= 127, weighted degree <= 32,
     // self-loops kept aside (constant energy, never in a local field).
@@ -527,7 +575,8 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
   SC_HIP(hipMalloc(&st->d_params, sizeof(ReplicaParams) * n_replicas));
   SC_HIP(hipMalloc(&st->d_em, sizeof(long long) * kEmStride * n_replicas));
   for (uint32_t r = 0; r < n_replicas; ++r) {
-    hipError_t e = st->continuous ? launch_on3_init(st->geom, st->model.spin_dim, st->d_vec, st->keys[r], r, 0, st->stream)
+    hipError_t e = st->continuous ? (st->csr ? launch_on_csr_init(st->ogeom, st->model.spin_dim, st->d_vec, st->keys[r], r, 0, st->stream)
+                                             : launch_on3_init(st->geom, st->model.spin_dim, st->d_vec, st->keys[r], r, 0, st->stream))
                    : st->csr ? launch_csr_init(l.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream)
                              : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[r], r, 0, st->stream);
     if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
@@ -557,7 +606,8 @@ int scamper_state_get_beta(const scamper_state *st, uint32_t replica, double *be
 
 int scamper_state_init(scamper_state *st, uint32_t replica, int random) {
   if (int rc = check_state(st, replica)) return rc;
-  hipError_t e = st->continuous ? launch_on3_init(st->geom, st->model.spin_dim, st->d_vec, st->keys[replica], replica, random != 0, st->stream)
+  hipError_t e = st->continuous ? (st->csr ? launch_on_csr_init(st->ogeom, st->model.spin_dim, st->d_vec, st->keys[replica], replica, random != 0, st->stream)
+                                           : launch_on3_init(st->geom, st->model.spin_dim, st->d_vec, st->keys[replica], replica, random != 0, st->stream))
                  : st->csr ? launch_csr_init(st->lat.n_sites, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream)
                            : launch_stencil_init(st->geom, st->model.kind, st->model.q, st->d_spins, st->keys[replica], replica, random != 0, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "init: %s", hipGetErrorString(e));
@@ -568,6 +618,15 @@ int scamper_state_init(scamper_state *st, uint32_t replica, int random) {
 static int vec_upload(scamper_state *st, uint32_t replica, const float *xyz) {
   const uint64_t n = st->lat.n_sites, nh = st->geom.nh;
   const int d = st->model.spin_dim;
+  if (st->csr) {  // graph lattice: planes [component][site]
+    std::vector<float> planes((uint64_t)d * n);
+    for (uint64_t s = 0; s < n; ++s)
+      for (int k = 0; k < d; ++k) planes[(uint64_t)k * n + s] = xyz[(uint64_t)d * s + k];
+    SC_HIP(hipStreamSynchronize(st->stream));
+    SC_HIP(hipMemcpy(st->d_vec + (uint64_t)d * replica * n, planes.data(), planes.size() * sizeof(float),
+                     hipMemcpyHostToDevice));
+    return SCAMPER_OK;
+  }
   std::vector<float> planes(2ull * d * nh);
   for (uint64_t s = 0; s < n; ++s) {
     uint32_t c;
@@ -583,6 +642,15 @@ static int vec_upload(scamper_state *st, uint32_t replica, const float *xyz) {
 static int vec_download(scamper_state *st, uint32_t replica, float *xyz) {
   const uint64_t n = st->lat.n_sites, nh = st->geom.nh;
   const int d = st->model.spin_dim;
+  if (st->csr) {
+    std::vector<float> planes((uint64_t)d * n);
+    SC_HIP(hipStreamSynchronize(st->stream));
+    SC_HIP(hipMemcpy(planes.data(), st->d_vec + (uint64_t)d * replica * n, planes.size() * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    for (uint64_t s = 0; s < n; ++s)
+      for (int k = 0; k < d; ++k) xyz[(uint64_t)d * s + k] = planes[(uint64_t)k * n + s];
+    return SCAMPER_OK;
+  }
   std::vector<float> planes(2ull * d * nh);
   SC_HIP(hipStreamSynchronize(st->stream));
   SC_HIP(hipMemcpy(planes.data(), st->d_vec + 2ull * d * replica * nh, planes.size() * sizeof(float),
@@ -708,9 +776,16 @@ int ensure_cluster(scamper_state *st) {
     const std::vector<Bond> &e = st->lat.edges;
     std::vector<ClusterEdge> ce(e.size());
     std::vector<int8_t> cw(e.size());
+    std::vector<double> cwf(e.size());
     for (size_t k = 0; k < e.size(); ++k) {
       ce[k] = ClusterEdge{(uint32_t)e[k].source, (uint32_t)e[k].target};
-      cw[k] = (int8_t)e[k].weight;  // integer weights in [-127, 127] on the graph path
+      cw[k] = (int8_t)e[k].weight;  // integer weights in [-127, 127] on the discrete graph path
+      cwf[k] = e[k].weight;         // any weight for O(N)
+    }
+    if (st->continuous) {
+      SC_HIP(hipMalloc(&st->d_cweights_f64, std::max<size_t>(cwf.size(), 1) * sizeof(double)));
+      if (!cwf.empty())
+        SC_HIP(hipMemcpy(st->d_cweights_f64, cwf.data(), cwf.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     SC_HIP(hipMalloc(&st->d_cedges, std::max<size_t>(ce.size(), 1) * sizeof(ClusterEdge)));
     SC_HIP(hipMalloc(&st->d_cweights, std::max<size_t>(cw.size(), 1)));
@@ -766,7 +841,12 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
     const uint64_t step = st->sweeps + t;
     for (uint32_t r = 0; r < st->n_rep; ++r) {
       hipError_t e;
-      if (st->continuous) {
+      if (st->continuous && st->csr) {
+        const int d = st->model.spin_dim;
+        e = launch_cluster_on_csr(n, st->lat.edges.size(), st->d_cedges, st->d_cweights_f64, d,
+                                  st->d_vec + (uint64_t)d * r * n, st->d_parent, st->keys[r], step, st->beta[r],
+                                  st->model.j, wolff, st->d_cscratch, st->stream);
+      } else if (st->continuous) {
         e = launch_cluster_on(st->geom, st->model.spin_dim, st->d_vec + 2ull * st->model.spin_dim * r * st->geom.nh,
                               st->d_parent, st->keys[r], step, st->beta[r], st->model.j, wolff, st->d_cscratch,
                               st->stream);
@@ -790,8 +870,7 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
     if (measure) {
       long long *em = st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride;
       const hipError_t e =
-          st->continuous ? launch_on3_measure(st->geom, st->model.spin_dim, st->d_vec, st->n_rep, (float)st->model.j,
-                                              reinterpret_cast<double *>(em), st->stream)
+          st->continuous ? measure_on(st, reinterpret_cast<double *>(em))
           : st->csr      ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream)
                          : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream);
       if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure launch: %s", hipGetErrorString(e));
@@ -828,6 +907,19 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
       // HybridCore: heat bath then over-relaxation each sweep (hybrid.rs:46-59)
       const int parts[2] = {algorithm == kOverRelaxation ? kOverRelaxation : kContinuousHeatBath,
                             algorithm == kHybridHeatBathOverRelaxation ? kOverRelaxation : -1};
+      if (st->csr) {
+        // graph lattice: colour classes in order, then the measurement
+        hipError_t e = hipSuccess;
+        for (int pi = 0; pi < 2 && e == hipSuccess; ++pi)
+          if (parts[pi] >= 0)
+            e = launch_on_csr_sweep(st->ogeom, st->model.spin_dim, parts[pi], st->d_vec, st->d_params, st->n_rep,
+                                    st->class_start.data(), (uint32_t)(st->class_start.size() - 1), sweep,
+                                    st->model.j, st->stream);
+        if (e == hipSuccess && measure)
+          e = measure_on(st, reinterpret_cast<double *>(st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride));
+        if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+        continue;
+      }
       for (int pi = 0; pi < 2; ++pi) {
         if (parts[pi] < 0) continue;
         const bool last = pi == 1 || parts[1] < 0;
@@ -916,9 +1008,7 @@ int scamper_synchronize(scamper_state *st) {
 int scamper_state_observables(scamper_state *st, uint32_t replica, double *energy, double *magnetization) {
   if (int rc = check_state(st, replica)) return rc;
   SC_HIP(hipMemsetAsync(st->d_em, 0, sizeof(long long) * kEmStride * st->n_rep, st->stream));
-  hipError_t e = st->continuous
-                     ? launch_on3_measure(st->geom, st->model.spin_dim, st->d_vec, st->n_rep, (float)st->model.j,
-                                          reinterpret_cast<double *>(st->d_em), st->stream)
+  hipError_t e = st->continuous ? measure_on(st, reinterpret_cast<double *>(st->d_em))
                  : st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream)
                            : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure: %s", hipGetErrorString(e));

@@ -109,6 +109,26 @@ hipError_t launch_csr_init(uint64_t n_sites, int kind, int q, int8_t *spins, uin
 
 // O(N) spins, N = dim = 2 or 3 (FP32 component planes per colour, dim per
 // colour per replica); em gets double sums {s.h, Mx, My, Mz} per replica.
+// O(N) spins on a graph lattice: FP32 planes [component][site] per replica
+// (dim n_sites floats), updated colour class by colour class; `weights` are
+// the bond weights of the non-loop incidences, in CSR order.
+struct OnCsrGeom {
+  uint64_t n_sites;
+  const uint32_t *offsets;      // [n_sites+1]
+  const uint32_t *neighbors;    // non-loop incidences only
+  const double *weights;        // per incidence
+  const uint32_t *class_sites;  // sites of class 0, then class 1, ...
+};
+hipError_t launch_on_csr_sweep(const OnCsrGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
+                               uint32_t n_rep, const uint64_t *class_start_host, uint32_t n_colours, uint64_t sweep,
+                               double j, hipStream_t st);
+// em[0] = sum over sites of s.h (J w included: every bond twice), em[1..dim]
+// = the summed components, one kEmStride record per replica (doubles).
+hipError_t launch_on_csr_measure(const OnCsrGeom &g, int dim, const float *spins, uint32_t n_rep, double j,
+                                 double *em, hipStream_t st);
+hipError_t launch_on_csr_init(const OnCsrGeom &g, int dim, float *spins, uint64_t key, uint32_t rep, int random,
+                              hipStream_t st);
+
 hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
                             uint32_t n_rep, uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t stream);
 hipError_t launch_on3_measure(const StencilGeom &g, int dim, const float *spins, uint32_t n_rep, float j, double *em,
@@ -231,6 +251,11 @@ hipError_t launch_cluster_transform(const StencilGeom *g, uint64_t n_sites, int8
 // (dim doubles).
 hipError_t launch_cluster_on(const StencilGeom &g, int dim, float *planes, uint32_t *parent, uint64_t key,
                              uint64_t step, double beta, double j, int wolff, void *scratch, hipStream_t stream);
+// The same on a graph lattice (`planes`: one replica's [component][site]
+// planes; edges and their f64 weights in edge order, self-loops included).
+hipError_t launch_cluster_on_csr(uint64_t n_sites, uint64_t n_edges, const ClusterEdge *edges, const double *weights,
+                                 int dim, float *planes, uint32_t *parent, uint64_t key, uint64_t step, double beta,
+                                 double j, int wolff, void *scratch, hipStream_t stream);
 
 // Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
 bool ising_fast_path_ok(const StencilGeom &g);

@@ -581,6 +581,61 @@ uint32_t blocks_for(uint64_t n) {
   return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
+// Graph lattices: edge e = (source, target, weight) in edge order,
+// self-loops included (their union is a no-op, their word index counts);
+// spins are one replica's planes [component][site].
+template <int D>
+__global__ void k_bonds_on_csr(const float *__restrict__ v, uint64_t n, const ClusterEdge *__restrict__ edges,
+                               const double *__restrict__ weights, uint64_t n_edges, uint32_t *parent, uint64_t key,
+                               uint64_t step, double beta, double j, const double *__restrict__ r_) {
+  double r[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = r_[c];
+  for (uint64_t e = thread_id(); e < n_edges; e += grid_threads()) {
+    const ClusterEdge b = edges[e];
+    double a[D], t[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      a[k] = (double)v[(uint64_t)k * n + b.source];
+      t[k] = (double)v[(uint64_t)k * n + b.target];
+    }
+    const double coupling = j * weights[e];
+    const double product = dotn<D>(a, r) * dotn<D>(t, r);
+    if (coupling == 0.0 || product <= 0.0) continue;
+    double p = 1.0 - exp(-2.0 * beta * coupling * product);
+    p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);  // clamp_probability; p is finite here
+    if ((double)cluster_word(key, e, step, kStreamClusterBond) * 0x1p-32 < p) unite(parent, b.source, b.target);
+  }
+}
+
+template <int D>
+__global__ void k_transform_on_csr(float *__restrict__ v, uint64_t n, uint32_t *parent, int wolff, uint64_t key,
+                                   uint64_t step, const uint32_t *__restrict__ seed, const double *__restrict__ r_) {
+  const uint32_t seed_root = wolff ? seed[0] : 0u;
+  double r[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = r_[c];
+  for (uint64_t s = thread_id(); s < n; s += grid_threads()) {
+    const uint32_t root = find_root(parent, (uint32_t)s);
+    const bool flip = wolff ? root == seed_root
+                            : cluster_word(key, root, step, kStreamClusterTarget) < 0x80000000u;
+    if (!flip) continue;
+    double x[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = (double)v[(uint64_t)k * n + s];
+    const double proj = dotn<D>(x, r);
+    double u[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) u[k] = x[k] - 2.0 * proj * r[k];
+    const double norm = sqrt(dotn<D>(u, u));
+    if (norm > 1e-15)
+#pragma unroll
+      for (int k = 0; k < D; ++k) u[k] /= norm;
+#pragma unroll
+    for (int k = 0; k < D; ++k) v[(uint64_t)k * n + s] = (float)u[k];
+  }
+}
+
 template <int D>
 void launch_on_update(const StencilGeom &g, float *planes, uint32_t *parent, uint64_t key, uint64_t step,
                       double beta, double j, int wolff, uint32_t *seed, double *r, hipStream_t stream) {
@@ -655,6 +710,33 @@ hipError_t launch_cluster_on(const StencilGeom &g, int dim, float *planes, uint3
   if (dim == 2) launch_on_update<2>(g, planes, parent, key, step, beta, j, wolff, seed, r, stream);
   else if (dim == 3) launch_on_update<3>(g, planes, parent, key, step, beta, j, wolff, seed, r, stream);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_cluster_on_csr(uint64_t n_sites, uint64_t n_edges, const ClusterEdge *edges, const double *weights,
+                                 int dim, float *planes, uint32_t *parent, uint64_t key, uint64_t step, double beta,
+                                 double j, int wolff, void *scratch, hipStream_t stream) {
+  uint32_t *seed = static_cast<uint32_t *>(scratch);
+  double *r = reinterpret_cast<double *>(static_cast<char *>(scratch) + 8);
+  if (dim != 2 && dim != 3) return hipErrorInvalidValue;
+  if (dim == 2) hipLaunchKernelGGL(k_on_direction<2>, dim3(1), dim3(64), 0, stream, key, step, r);
+  else hipLaunchKernelGGL(k_on_direction<3>, dim3(1), dim3(64), 0, stream, key, step, r);
+  hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n_sites)), dim3(kThreads), 0, stream, parent, n_sites);
+  if (n_edges) {
+    if (dim == 2)
+      hipLaunchKernelGGL(k_bonds_on_csr<2>, dim3(blocks_for(n_edges)), dim3(kThreads), 0, stream, planes, n_sites,
+                         edges, weights, n_edges, parent, key, step, beta, j, r);
+    else
+      hipLaunchKernelGGL(k_bonds_on_csr<3>, dim3(blocks_for(n_edges)), dim3(kThreads), 0, stream, planes, n_sites,
+                         edges, weights, n_edges, parent, key, step, beta, j, r);
+  }
+  if (wolff) hipLaunchKernelGGL(k_on_seed, dim3(1), dim3(64), 0, stream, parent, n_sites, key, step, seed);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_transform_on_csr<2>, dim3(blocks_for(n_sites)), dim3(kThreads), 0, stream, planes, n_sites,
+                       parent, wolff, key, step, seed, r);
+  else
+    hipLaunchKernelGGL(k_transform_on_csr<3>, dim3(blocks_for(n_sites)), dim3(kThreads), 0, stream, planes, n_sites,
+                       parent, wolff, key, step, seed, r);
   return hipGetLastError();
 }
 

@@ -566,6 +566,147 @@ __global__ __launch_bounds__(kBlock) void k_on2_init(float *__restrict__ spins, 
   store2(spins + (2ull * rep + colour) * 2ull * g.nh, g.nh, j, v);
 }
 
+// ------------------------------------------------ O(N) on graph lattices
+// Any CsrLattice: one replica's spins are FP32 planes [component][site],
+// updated one colour class at a time in the lattice's class order; the rank
+// j of a site inside its class numbers its draws as on the stencil, and the
+// local field sums J w s over the non-loop incidences in CSR order
+// (local_field, models.rs:579-595).  O(3) sums in FP32 like the stencil
+// kernels; O(2) in f64 like k_on2.
+
+template <int D, int ALG>
+__global__ __launch_bounds__(kBlock) void k_on_csr(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
+                                                   OnCsrGeom g, uint64_t begin, uint64_t count, uint32_t colour,
+                                                   uint64_t sweep, double jd) {
+  const uint32_t rep = blockIdx.y;
+  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= count) return;
+  const uint64_t n = g.n_sites;
+  float *__restrict__ v = spins + (uint64_t)rep * D * n;
+  const uint32_t site = g.class_sites[begin + j];
+  const uint32_t e0 = g.offsets[site], e1 = g.offsets[site + 1];
+  const ReplicaParams &P = reps[rep];
+  if (D == 3) {
+    V3 h{0.f, 0.f, 0.f};
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t t = g.neighbors[e];
+      const float sc = (float)(jd * g.weights[e]);
+      h.x += sc * v[t];
+      h.y += sc * v[n + t];
+      h.z += sc * v[2 * n + t];
+    }
+    V3 ns;
+    if (ALG == kContinuousHeatBath) {
+      const U4 blk = philox4x32_10((uint32_t)(j >> 1), (uint32_t)sweep, ctr2(sweep, colour, kStreamCont),
+                                   (uint32_t)(j >> 33), (uint32_t)P.key, (uint32_t)(P.key >> 32));
+      ns = heat_bath_site(h, P, j, sweep, colour, (j & 1) ? blk.z : blk.x, (j & 1) ? blk.w : blk.y);
+    } else {
+      ns = reflect_site(V3{v[site], v[n + site], v[2 * n + site]}, h);
+    }
+    v[site] = ns.x;
+    v[n + site] = ns.y;
+    v[2 * n + site] = ns.z;
+  } else {
+    double hx = 0.0, hy = 0.0;
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t t = g.neighbors[e];
+      const double sc = jd * g.weights[e];
+      hx += sc * (double)v[t];
+      hy += sc * (double)v[n + t];
+    }
+    V2 ns{v[site], v[n + site]};
+    if (ALG == kContinuousHeatBath) {
+      const double norm = sqrt(hx * hx + hy * hy);
+      double out[2];
+      if (norm < 1e-15) {
+        const U4 fb = philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
+                                    (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32));
+        random_unit2(fb.x, fb.y, out);
+      } else {
+        double ca, sa;
+        von_mises_cs(P, j, sweep, colour, P.beta * norm, &ca, &sa);
+        const double ux = hx / norm, uy = hy / norm;
+        out[0] = ca * ux - sa * uy;
+        out[1] = sa * ux + ca * uy;
+      }
+      ns = V2{(float)out[0], (float)out[1]};
+    } else {
+      const float fx = (float)hx, fy = (float)hy;
+      const float nsq = fx * fx + fy * fy;
+      if (nsq >= 1e-28f) {
+        const float proj = (ns.x * fx + ns.y * fy) / nsq;
+        const float rx = 2.0f * proj * fx - ns.x, ry = 2.0f * proj * fy - ns.y;
+        const float inv = rsqrtf(rx * rx + ry * ry);
+        ns = V2{rx * inv, ry * inv};
+      }
+    }
+    v[site] = ns.x;
+    v[n + site] = ns.y;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_on_csr_measure(const float *__restrict__ spins, OnCsrGeom g, double jd,
+                                                           double *__restrict__ em) {
+  const uint32_t rep = blockIdx.y;
+  const uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t n = g.n_sites;
+  double acc[1 + D];
+#pragma unroll
+  for (int k = 0; k <= D; ++k) acc[k] = 0.0;
+  if (s < n) {
+    const float *__restrict__ v = spins + (uint64_t)rep * D * n;
+    double h[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) h[k] = 0.0;
+    for (uint32_t e = g.offsets[s]; e < g.offsets[s + 1]; ++e) {
+      const uint32_t t = g.neighbors[e];
+      const double sc = jd * g.weights[e];
+#pragma unroll
+      for (int k = 0; k < D; ++k) h[k] += sc * (double)v[(uint64_t)k * n + t];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double x = (double)v[(uint64_t)k * n + s];
+      acc[0] += x * h[k];
+      acc[1 + k] = x;
+    }
+  }
+  __shared__ double red[1 + D];
+  if (threadIdx.x <= D) red[threadIdx.x] = 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k <= D; ++k) {
+    double x = acc[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&red[k], x);
+  }
+  __syncthreads();
+  if (threadIdx.x <= D) atomicAdd(&em[(uint64_t)rep * kEmStride + threadIdx.x], red[threadIdx.x]);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_on_csr_init(float *__restrict__ spins, uint64_t n, uint64_t key,
+                                                        uint32_t rep, int random) {
+  const uint64_t site = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (site >= n) return;
+  float *__restrict__ v = spins + (uint64_t)rep * D * n;
+  const U4 w = random ? philox4x32_10((uint32_t)site, 0u, ctr2(0, 0, kStreamContInit), (uint32_t)(site >> 32),
+                                      (uint32_t)key, (uint32_t)(key >> 32))
+                      : U4{0u, 0u, 0u, 0u};
+  if (D == 3) {
+    const V3 u = random ? random_unit(w) : V3{1.f, 0.f, 0.f};  // ordered_spin (models.rs:479-484)
+    v[site] = u.x;
+    v[n + site] = u.y;
+    v[2 * n + site] = u.z;
+  } else {
+    double out[2] = {1.0, 0.0};
+    if (random) random_unit2(w.x, w.y, out);
+    v[site] = (float)out[0];
+    v[n + site] = (float)out[1];
+  }
+}
+
 }  // namespace
 
 hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
@@ -655,6 +796,49 @@ hipError_t launch_on3_init(const StencilGeom &g, int dim, float *spins, uint64_t
   }
   hipLaunchKernelGGL(k_on3_init, dim3((unsigned)((g.nh + kBlock - 1) / kBlock), 2), dim3(kBlock), 0, st, spins, g, key,
                      rep, random);
+  return hipGetLastError();
+}
+
+hipError_t launch_on_csr_sweep(const OnCsrGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
+                               uint32_t n_rep, const uint64_t *class_start_host, uint32_t n_colours, uint64_t sweep,
+                               double j, hipStream_t st) {
+  if (dim != 2 && dim != 3) return hipErrorInvalidValue;
+  const bool hb = alg == kContinuousHeatBath;
+  for (uint32_t c = 0; c < n_colours; ++c) {
+    const uint64_t begin = class_start_host[c], count = class_start_host[c + 1] - begin;
+    if (count == 0) continue;
+    const dim3 grid((unsigned)((count + kBlock - 1) / kBlock), n_rep);
+    if (dim == 3 && hb)
+      hipLaunchKernelGGL((k_on_csr<3, kContinuousHeatBath>), grid, dim3(kBlock), 0, st, spins, reps, g, begin, count,
+                         c, sweep, j);
+    else if (dim == 3)
+      hipLaunchKernelGGL((k_on_csr<3, kOverRelaxation>), grid, dim3(kBlock), 0, st, spins, reps, g, begin, count, c,
+                         sweep, j);
+    else if (hb)
+      hipLaunchKernelGGL((k_on_csr<2, kContinuousHeatBath>), grid, dim3(kBlock), 0, st, spins, reps, g, begin, count,
+                         c, sweep, j);
+    else
+      hipLaunchKernelGGL((k_on_csr<2, kOverRelaxation>), grid, dim3(kBlock), 0, st, spins, reps, g, begin, count, c,
+                         sweep, j);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_on_csr_measure(const OnCsrGeom &g, int dim, const float *spins, uint32_t n_rep, double j,
+                                 double *em, hipStream_t st) {
+  const dim3 grid((unsigned)((g.n_sites + kBlock - 1) / kBlock), n_rep);
+  if (dim == 3) hipLaunchKernelGGL(k_on_csr_measure<3>, grid, dim3(kBlock), 0, st, spins, g, j, em);
+  else if (dim == 2) hipLaunchKernelGGL(k_on_csr_measure<2>, grid, dim3(kBlock), 0, st, spins, g, j, em);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_on_csr_init(const OnCsrGeom &g, int dim, float *spins, uint64_t key, uint32_t rep, int random,
+                              hipStream_t st) {
+  const dim3 grid((unsigned)((g.n_sites + kBlock - 1) / kBlock));
+  if (dim == 3) hipLaunchKernelGGL(k_on_csr_init<3>, grid, dim3(kBlock), 0, st, spins, g.n_sites, key, rep, random);
+  else if (dim == 2) hipLaunchKernelGGL(k_on_csr_init<2>, grid, dim3(kBlock), 0, st, spins, g.n_sites, key, rep, random);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
