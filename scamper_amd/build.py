@@ -32,6 +32,7 @@ PER_FILE = {
     "carlo.cpp": ["-ffp-contract=off"],
     "pool.cpp": ["-ffp-contract=off"],
     "k_cluster.hip": ["-ffp-contract=off"],
+    "k_csr_real.hip": ["-ffp-contract=off"],
 }
 
 

@@ -76,6 +76,12 @@ struct scamper_state {
   double self_loop_weight = 0.0;  // sum of w over self-loops (constant energy)
   uint32_t *d_offsets = nullptr, *d_neighbors = nullptr, *d_class_sites = nullptr;
   int8_t *d_weights = nullptr;
+  // Ising / Potts on a graph lattice: every incidence with its f64 weight,
+  // for real weights (real_weights) and the Potts heat bath on weighted graphs
+  bool real_weights = false;
+  RealCsrGeom rgeom{};
+  uint32_t *d_roffsets = nullptr, *d_rneighbors = nullptr;
+  double *d_rweights = nullptr;
   // O(N) on a graph lattice: f64 weights of the non-loop incidences
   OnCsrGeom ogeom{};
   double *d_fweights = nullptr;
@@ -121,6 +127,9 @@ struct scamper_state {
     hipFree(d_cweights);
     hipFree(d_cweights_f64);
     hipFree(d_fweights);
+    hipFree(d_roffsets);
+    hipFree(d_rneighbors);
+    hipFree(d_rweights);
     hipFree(d_vec);
     hipFree(d_params);
     hipFree(d_hbexp);
@@ -152,19 +161,36 @@ bool uses_tiles(const scamper_state *st, int alg) {
   return st->geom.nh < (1ull << 31);
 }
 
+// Ising / Potts updates on a graph that run on the f64 kernels: real
+// weights, or the Potts heat bath on a weighted graph.
+bool uses_real_kernels(const scamper_state *st, int alg) {
+  if (!st->csr || st->continuous) return false;
+  return st->real_weights || (st->model.kind == kPotts && alg == kHeatBath && !st->unit_weights);
+}
+
+// Adds the Ising / Potts counters of every replica on a graph into em.
+hipError_t measure_graph(const scamper_state *st, long long *em) {
+  return st->real_weights ? launch_csr_real_measure(st->rgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em,
+                                                    st->stream)
+                          : launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em,
+                                               st->stream);
+}
+
 // Build and upload the per-replica decision tables for `alg`.
 int upload_tables(scamper_state *st, int alg) {
   std::vector<ReplicaParams> p(st->n_rep);
   std::vector<double> hb;
   const int z = st->csr ? st->w_max : st->z;
-  if (st->csr && st->model.kind == kPotts && alg == kHeatBath && !st->unit_weights)
-    return fail(SCAMPER_EUNSUPPORTED, "Potts heat bath on a graph lattice needs unit bond weights");
-  if (st->model.kind == kPotts && alg == kHeatBath) hb.assign((size_t)st->n_rep * (kMaxZ + 1) * (kMaxZ + 1), 0.0);
+  // the f64 graph kernels read only key and beta
+  const bool f64_graph = uses_real_kernels(st, alg);
+  if (st->model.kind == kPotts && alg == kHeatBath && !f64_graph)
+    hb.assign((size_t)st->n_rep * (kMaxZ + 1) * (kMaxZ + 1), 0.0);
   for (uint32_t r = 0; r < st->n_rep; ++r) {
     memset(&p[r], 0, sizeof(ReplicaParams));
     p[r].key = st->keys[r];
     p[r].beta = st->beta[r];
-    if (st->csr && st->model.kind == kIsing) {
+    if (f64_graph) {
+    } else if (st->csr && st->model.kind == kIsing) {
       if (alg == kMetropolis) csr_ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
       else csr_ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
     } else if (st->model.kind == kIsing) {
@@ -321,7 +347,11 @@ void observables_from_em(const scamper_state *st, const long long *em, double *e
     return;
   }
   if (e) {
-    if (st->csr)
+    if (st->csr && st->real_weights) {
+      double bonds;  // f64 weighted bond sum, every bond twice
+      std::memcpy(&bonds, em, sizeof bonds);
+      *e = -st->model.j * (0.5 * bonds + st->self_loop_weight);
+    } else if (st->csr)
       *e = -st->model.j * ((double)(em[0] / 2) + st->self_loop_weight);
     else
       *e = -st->model.j * (double)em[0];
@@ -519,48 +549,67 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
     if (!cls.empty()) SC_HIP(hipMemcpy(st->d_class_sites, cls.data(), cls.size() * 4, hipMemcpyHostToDevice));
     st->ogeom = OnCsrGeom{l.n_sites, st->d_offsets, st->d_neighbors, st->d_fweights, st->d_class_sites};
   } else {
-    // Graph lattice: integer weights |w| <= 127, weighted degree <= 32,
-    // self-loops kept aside (constant energy, never in a local field).
+    // Graph lattice.  Integer weights |w| <= 127 with weighted degree <= 32
+    // take the integer-threshold kernels (self-loops kept aside: constant
+    // energy, never in a local field); any other finite weights take the
+    // f64 kernels, which also serve the Potts heat bath on weighted graphs.
     if (l.n_sites >= (1ull << 32) || l.neighbors.size() >= (1ull << 32))
       return fail(SCAMPER_EUNSUPPORTED, "graph lattices are limited to 2^32 sites and incidences");
     st->csr = true;
     st->lat = l;
-    std::vector<uint32_t> off(l.n_sites + 1, 0), nb;
+    std::vector<uint32_t> off(l.n_sites + 1, 0), nb, roff(l.n_sites + 1, 0), rnb;
     std::vector<int8_t> w;
+    std::vector<double> rw;
     nb.reserve(l.neighbors.size());
     w.reserve(l.neighbors.size());
+    rnb.reserve(l.neighbors.size());
+    rw.reserve(l.neighbors.size());
     for (uint64_t s = 0; s < l.n_sites; ++s) {
       int wsum = 0;
       for (uint64_t i = l.offsets[s]; i < l.offsets[s + 1]; ++i) {
         const Bond &b = l.edges[l.edge_ids[i]];
-        if (b.weight != std::floor(b.weight) || std::fabs(b.weight) > 127.0)
-          return fail(SCAMPER_EUNSUPPORTED, "bond weights must be integers with |w| <= 127 on the GPU path");
+        if (!std::isfinite(b.weight)) return fail(SCAMPER_EINVAL, "bond weights must be finite");
+        rnb.push_back((uint32_t)l.neighbors[i]);
+        rw.push_back(b.weight);
+        if (b.weight != std::floor(b.weight) || std::fabs(b.weight) > 127.0) st->real_weights = true;
         if (b.weight != 1.0) st->unit_weights = false;
         if (b.source == b.target) continue;
         nb.push_back((uint32_t)l.neighbors[i]);
-        w.push_back((int8_t)b.weight);
-        wsum += (int)std::fabs(b.weight);
+        w.push_back(st->real_weights ? (int8_t)0 : (int8_t)b.weight);
+        wsum += (int)std::fmin(std::fabs(b.weight), 1e6);
       }
       off[s + 1] = (uint32_t)nb.size();
+      roff[s + 1] = (uint32_t)rnb.size();
       st->w_max = std::max(st->w_max, wsum);
     }
-    if (st->w_max > kMaxZ) return fail(SCAMPER_EUNSUPPORTED, "weighted degree %d exceeds %d", st->w_max, kMaxZ);
+    if (st->w_max > kMaxZ) st->real_weights = true;
     for (const Bond &b : l.edges)
       if (b.source == b.target) st->self_loop_weight += b.weight;
     std::vector<uint32_t> cls(l.class_sites.begin(), l.class_sites.end());
     st->class_start = l.class_start;
-    SC_HIP(hipMalloc(&st->d_offsets, off.size() * 4));
-    SC_HIP(hipMalloc(&st->d_neighbors, std::max<size_t>(nb.size(), 1) * 4));
-    SC_HIP(hipMalloc(&st->d_weights, std::max<size_t>(w.size(), 1)));
-    SC_HIP(hipMalloc(&st->d_class_sites, cls.size() * 4));
-    SC_HIP(hipMemcpy(st->d_offsets, off.data(), off.size() * 4, hipMemcpyHostToDevice));
-    if (!nb.empty()) {
-      SC_HIP(hipMemcpy(st->d_neighbors, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
-      SC_HIP(hipMemcpy(st->d_weights, w.data(), w.size(), hipMemcpyHostToDevice));
+    SC_HIP(hipMalloc(&st->d_class_sites, std::max<size_t>(cls.size(), 1) * 4));
+    if (!cls.empty()) SC_HIP(hipMemcpy(st->d_class_sites, cls.data(), cls.size() * 4, hipMemcpyHostToDevice));
+    SC_HIP(hipMalloc(&st->d_roffsets, roff.size() * 4));
+    SC_HIP(hipMalloc(&st->d_rneighbors, std::max<size_t>(rnb.size(), 1) * 4));
+    SC_HIP(hipMalloc(&st->d_rweights, std::max<size_t>(rw.size(), 1) * sizeof(double)));
+    SC_HIP(hipMemcpy(st->d_roffsets, roff.data(), roff.size() * 4, hipMemcpyHostToDevice));
+    if (!rnb.empty()) {
+      SC_HIP(hipMemcpy(st->d_rneighbors, rnb.data(), rnb.size() * 4, hipMemcpyHostToDevice));
+      SC_HIP(hipMemcpy(st->d_rweights, rw.data(), rw.size() * sizeof(double), hipMemcpyHostToDevice));
     }
-    SC_HIP(hipMemcpy(st->d_class_sites, cls.data(), cls.size() * 4, hipMemcpyHostToDevice));
-    st->cgeom = CsrGeom{l.n_sites, l.n_colours, st->w_max, st->d_offsets, st->d_neighbors, st->d_weights,
-                        st->d_class_sites};
+    st->rgeom = RealCsrGeom{l.n_sites, st->d_roffsets, st->d_rneighbors, st->d_rweights, st->d_class_sites};
+    if (!st->real_weights) {
+      SC_HIP(hipMalloc(&st->d_offsets, off.size() * 4));
+      SC_HIP(hipMalloc(&st->d_neighbors, std::max<size_t>(nb.size(), 1) * 4));
+      SC_HIP(hipMalloc(&st->d_weights, std::max<size_t>(w.size(), 1)));
+      SC_HIP(hipMemcpy(st->d_offsets, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+      if (!nb.empty()) {
+        SC_HIP(hipMemcpy(st->d_neighbors, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
+        SC_HIP(hipMemcpy(st->d_weights, w.data(), w.size(), hipMemcpyHostToDevice));
+      }
+      st->cgeom = CsrGeom{l.n_sites, l.n_colours, st->w_max, st->d_offsets, st->d_neighbors, st->d_weights,
+                          st->d_class_sites};
+    }
   }
   st->continuous = model->kind == kON;
   if (st->continuous) {
@@ -820,6 +869,8 @@ int prepare_cluster(scamper_state *st) {
 int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
   const uint64_t n = st->lat.n_sites;
   if (n >= (1ull << 32)) return fail(SCAMPER_EUNSUPPORTED, "cluster updates need fewer than 2^32 sites");
+  if (st->real_weights && !st->continuous)
+    return fail(SCAMPER_EUNSUPPORTED, "Ising / Potts cluster updates need integer bond weights |w| <= 127");
   // the reference asserts J * w >= 0 (models.rs:139-142, 321-324)
   if (st->csr) {
     for (const Bond &b : st->lat.edges)
@@ -871,7 +922,7 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
       long long *em = st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride;
       const hipError_t e =
           st->continuous ? measure_on(st, reinterpret_cast<double *>(em))
-          : st->csr      ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream)
+          : st->csr      ? measure_graph(st, em)
                          : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, em, st->stream);
       if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure launch: %s", hipGetErrorString(e));
     }
@@ -935,11 +986,15 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
       continue;
     }
     if (st->csr) {
-      hipError_t e = launch_csr_discrete(st->cgeom, st->class_start.data(), st->model.kind, st->model.q, algorithm,
-                                         st->d_spins, st->d_params, st->n_rep, sweep, st->d_hbexp, st->stream);
+      hipError_t e =
+          uses_real_kernels(st, algorithm)
+              ? launch_csr_real(st->rgeom, st->class_start.data(), (uint32_t)(st->class_start.size() - 1),
+                                st->model.kind, st->model.q, algorithm, st->d_spins, st->d_params, st->n_rep, sweep,
+                                st->model.j, st->stream)
+              : launch_csr_discrete(st->cgeom, st->class_start.data(), st->model.kind, st->model.q, algorithm,
+                                    st->d_spins, st->d_params, st->n_rep, sweep, st->d_hbexp, st->stream);
       if (e == hipSuccess && measure)
-        e = launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep,
-                               st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride, st->stream);
+        e = measure_graph(st, st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride);
       if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
       continue;
     }
@@ -1009,7 +1064,7 @@ int scamper_state_observables(scamper_state *st, uint32_t replica, double *energ
   if (int rc = check_state(st, replica)) return rc;
   SC_HIP(hipMemsetAsync(st->d_em, 0, sizeof(long long) * kEmStride * st->n_rep, st->stream));
   hipError_t e = st->continuous ? measure_on(st, reinterpret_cast<double *>(st->d_em))
-                 : st->csr ? launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream)
+                 : st->csr ? measure_graph(st, st->d_em)
                            : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "measure: %s", hipGetErrorString(e));
   long long em[kEmStride];

@@ -104,6 +104,22 @@ hipError_t launch_csr_discrete(const CsrGeom &g, const uint64_t *class_start_hos
                                uint64_t sweep, const double *hb_exp, hipStream_t stream);
 hipError_t launch_csr_measure(const CsrGeom &g, int kind, int q, const int8_t *spins, uint32_t n_rep,
                               long long *em, hipStream_t stream);
+// Ising / Potts on a graph lattice with real weights: every incidence in CSR
+// order, self-loops included (neighbour == site), with its f64 weight.
+struct RealCsrGeom {
+  uint64_t n_sites;
+  const uint32_t *offsets;      // [n_sites+1]
+  const uint32_t *neighbors;    // all incidences
+  const double *weights;        // per incidence
+  const uint32_t *class_sites;  // sites of class 0, then class 1, ...
+};
+hipError_t launch_csr_real(const RealCsrGeom &g, const uint64_t *class_start_host, uint32_t n_colours, int kind, int q,
+                           int algorithm, int8_t *spins, const ReplicaParams *reps, uint32_t n_rep, uint64_t sweep,
+                           double j, hipStream_t st);
+// em[0] holds a double (the weighted bond sum, every bond twice); the
+// magnetisation counters are integers as for launch_csr_measure.
+hipError_t launch_csr_real_measure(const RealCsrGeom &g, int kind, int q, const int8_t *spins, uint32_t n_rep,
+                                   long long *em, hipStream_t st);
 hipError_t launch_csr_init(uint64_t n_sites, int kind, int q, int8_t *spins, uint64_t key, uint32_t rep,
                            int random, hipStream_t stream);
 

@@ -1,7 +1,12 @@
 """GPU parity of the graph-lattice (CSR) path against the oracle's CSR
 restatement, bit for bit: non-bipartite lattices (greedy colouring with 3+
 classes), open boundaries, and a weighted multigraph with parallel bonds and
-self-loops (the reference's tests/physics/lattice_exact.rs:23-96 shapes)."""
+self-loops (the reference's tests/physics/lattice_exact.rs:23-96 shapes).
+
+Real weights (non-integer, |w| > 127 or weighted degree > 32) and the Potts
+heat bath on weighted graphs run on the f64 kernels, which restate the
+reference's f64 rules: spins equal the oracle's bit for bit, and E agrees to
+1e-9 relative (the device sums bonds per site, the oracle per edge)."""
 import numpy as np
 import pytest
 
@@ -34,8 +39,6 @@ LATTICES = [
 @pytest.mark.parametrize("name,args,weights", LATTICES)
 @pytest.mark.parametrize("kind,q,alg", [(0, 2, 0), (0, 2, 1), (1, 3, 0), (1, 4, 1)])
 def test_csr_bit_exact(sc, orc, name, args, weights, kind, q, alg):
-    if kind == 1 and alg == 1 and weights is not None:
-        pytest.skip("Potts heat bath on the GPU path needs unit weights")
     print(f"case {name}{args} w={weights is not None} kind={kind} q={q} alg={alg}", flush=True)
     lat, olat = _pair(sc, orc, name, *args, weights=weights)
     seed, beta, n = 77, 0.7, 6
@@ -56,14 +59,53 @@ def test_csr_bit_exact(sc, orc, name, args, weights, kind, q, alg):
     assert mag[-1] == chain.magnetization()
 
 
-def test_potts_heat_bath_rejects_weights(sc):
-    lat = sc.Lattice.from_edges(*MULTI, MULTI_W)
-    st = sc.State(lat, sc.model(1, 1.0, 3), 1, 1)
-    with pytest.raises(sc.ScamperError):
-        st.sweep(1, 1)
+def _glass(L, seed):
+    """Periodic L x L Edwards-Anderson couplings, Gaussian (edges +x, +y)."""
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for y in range(L):
+        for x in range(L):
+            s = x + L * y
+            src += [s, s]
+            dst += [(x + 1) % L + L * y, x + L * ((y + 1) % L)]
+    return L * L, src, dst, list(rng.normal(size=len(src)))
 
 
-def test_non_integer_weights_rejected(sc):
+REAL = [
+    ("multigraph", MULTI[0], MULTI[1], MULTI[2], [0.5, 1.25, -0.75, 2.5, 1.0, 0.3, -1.7, 1.1, 0.9, 2.2, 1.0, 5.5]),
+    ("glass", *_glass(8, 3)),
+    ("wide", 4, [0, 1, 2, 3, 0], [1, 2, 3, 0, 2], [200.0, -150.0, 3.0, 1.0, 0.25]),
+]
+
+
+@pytest.mark.parametrize("case", REAL, ids=[c[0] for c in REAL])
+@pytest.mark.parametrize("kind,q,alg", [(0, 2, 0), (0, 2, 1), (1, 3, 0), (1, 4, 1)])
+def test_real_weights_bit_exact(sc, orc, case, kind, q, alg):
+    name, n_sites, src, dst, w = case
+    print(f"case real {name} kind={kind} q={q} alg={alg}", flush=True)
+    lat, olat = sc.Lattice.from_edges(n_sites, src, dst, w), orc.Lattice.from_edges(n_sites, src, dst, w)
+    seed, beta, n = 78, 0.9, 8
+    st = sc.State(lat, sc.model(kind, 1.0, q), 1, seed)
+    st.init(True)
+    key = orc.replica_key(seed, 0)
+    m = orc.model(kind, 1.0, q)
+    start = orc.random_start(olat, m, key)
+    assert np.array_equal(st.download_f64(), start)
+    chain = orc.Chain(olat, m, start)
+    st.set_beta(beta)
+    st.sweep(alg, n, measure=True)
+    for t in range(n):
+        (chain.metropolis if alg == 0 else chain.heat_bath)(beta, key, t)
+    assert np.array_equal(st.download_f64(), chain.spins)
+    e, mag = st.series()
+    assert abs(e[-1] - chain.total_energy()) <= 1e-9 * (1 + abs(chain.total_energy()))
+    assert mag[-1] == chain.magnetization()
+    e1, m1 = st.observables()
+    assert e1 == e[-1] and m1 == mag[-1]
+
+
+def test_real_weight_clusters_refused(sc):
     lat = sc.Lattice.from_edges(3, [0, 1], [1, 2], [0.5, 1.0])
+    st = sc.State(lat, sc.model(0), 1, 1)
     with pytest.raises(sc.ScamperError):
-        sc.State(lat, sc.model(0), 1, 1)
+        st.sweep(5, 1)
