@@ -110,6 +110,7 @@ struct scamper_state {
   // graph lattice's edges with their integer weights
   uint32_t *d_parent = nullptr, *d_cscratch = nullptr;
   uint64_t *d_cthr = nullptr;
+  uint64_t *d_ethr = nullptr;  // real-weight graphs: thresholds per (replica, edge)
   ClusterEdge *d_cedges = nullptr;
   int8_t *d_cweights = nullptr;
   double *d_cweights_f64 = nullptr;  // O(N) on a graph lattice: f64 edge weights
@@ -123,6 +124,7 @@ struct scamper_state {
     hipFree(d_parent);
     hipFree(d_cscratch);
     hipFree(d_cthr);
+    hipFree(d_ethr);
     hipFree(d_cedges);
     hipFree(d_cweights);
     hipFree(d_cweights_f64);
@@ -859,6 +861,19 @@ int prepare_cluster(scamper_state *st) {
           cluster_bond_threshold(st->model.kind, st->beta[r], std::fabs(st->model.j), (double)w);
   SC_HIP(hipStreamSynchronize(st->stream));
   SC_HIP(hipMemcpy(st->d_cthr, st->cthr.data(), st->cthr.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  if (st->csr && st->real_weights) {
+    // one threshold per edge: K(beta, |J| |w|), the same rule as by integer |w|
+    const std::vector<Bond> &edges = st->lat.edges;
+    const size_t ne = edges.size();
+    if (!st->d_ethr && hipMalloc(&st->d_ethr, std::max<size_t>(ne * st->n_rep, 1) * sizeof(uint64_t)) != hipSuccess)
+      return fail(SCAMPER_ENOMEM, "edge thresholds: %zu edges x %u replicas", ne, st->n_rep);
+    std::vector<uint64_t> et(ne * st->n_rep);
+    for (uint32_t r = 0; r < st->n_rep; ++r)
+      for (size_t e = 0; e < ne; ++e)
+        et[(size_t)r * ne + e] = cluster_bond_threshold(st->model.kind, st->beta[r], std::fabs(st->model.j),
+                                                        std::fabs(edges[e].weight));
+    if (!et.empty()) SC_HIP(hipMemcpy(st->d_ethr, et.data(), et.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  }
   st->cthr_ready = true;
   return SCAMPER_OK;
 }
@@ -869,8 +884,6 @@ int prepare_cluster(scamper_state *st) {
 int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
   const uint64_t n = st->lat.n_sites;
   if (n >= (1ull << 32)) return fail(SCAMPER_EUNSUPPORTED, "cluster updates need fewer than 2^32 sites");
-  if (st->real_weights && !st->continuous)
-    return fail(SCAMPER_EUNSUPPORTED, "Ising / Potts cluster updates need integer bond weights |w| <= 127");
   // the reference asserts J * w >= 0 (models.rs:139-142, 321-324)
   if (st->csr) {
     for (const Bond &b : st->lat.edges)
@@ -904,7 +917,9 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
       } else if (st->csr) {
         int8_t *spins = st->d_spins + (uint64_t)r * n;
         e = launch_cluster_bonds_csr(n, st->lat.edges.size(), st->d_cedges, st->d_cweights, spins, st->d_parent,
-                                     st->keys[r], step, st->d_cthr + (size_t)r * kClusterWeights, st->stream);
+                                     st->keys[r], step, st->d_cthr + (size_t)r * kClusterWeights,
+                                     st->real_weights ? st->d_ethr + (size_t)r * st->lat.edges.size() : nullptr,
+                                     st->stream);
         if (e == hipSuccess)
           e = launch_cluster_transform(nullptr, n, spins, st->d_parent, st->model.kind, st->model.q, wolff,
                                        st->keys[r], step, st->d_cscratch, st->stream);

@@ -252,9 +252,12 @@ struct ClusterEdge {
 constexpr int kClusterWeights = 128;
 hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spins, uint32_t *parent,
                                         uint64_t key, uint64_t step, uint64_t thr, hipStream_t stream);
+// `edge_thr` (one threshold per edge, or null for thr by |w|) serves graphs
+// with real weights.
 hipError_t launch_cluster_bonds_csr(uint64_t n_sites, uint64_t n_edges, const ClusterEdge *edges,
                                     const int8_t *weights, const int8_t *spins, uint32_t *parent, uint64_t key,
-                                    uint64_t step, const uint64_t *thr, hipStream_t stream);
+                                    uint64_t step, const uint64_t *thr, const uint64_t *edge_thr,
+                                    hipStream_t stream);
 // SW: every cluster takes its smallest site's target; Wolff: the seed's
 // cluster takes -seed (Ising) or another state (Potts).  `scratch` holds 2
 // uint32 for the Wolff seed's root and target.

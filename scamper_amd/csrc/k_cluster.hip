@@ -115,14 +115,15 @@ __global__ void k_bonds_stencil(const int8_t *__restrict__ spins, uint32_t *pare
 
 __global__ void k_bonds_csr(uint64_t n_edges, const ClusterEdge *__restrict__ edges, const int8_t *__restrict__ w,
                             const int8_t *__restrict__ spins, uint32_t *parent, uint64_t key, uint64_t step,
-                            const uint64_t *__restrict__ thr) {
+                            const uint64_t *__restrict__ thr, const uint64_t *__restrict__ edge_thr) {
   for (uint64_t e = thread_id(); e < n_edges; e += grid_threads()) {
     const ClusterEdge b = edges[e];
     if (b.source == b.target || spins[b.source] != spins[b.target]) continue;  // a self-loop joins nothing
-    // J w >= 0 on every edge, so the coupling is |J| |w| (thr is indexed by |w|)
+    // J w >= 0 on every edge, so the coupling is |J| |w|: thr is indexed by
+    // |w| for integer weights, edge_thr holds one threshold per edge otherwise
     const int wi = w[e];
-    if ((uint64_t)cluster_word(key, e, step, kStreamClusterBond) < thr[wi < 0 ? -wi : wi])
-      unite(parent, b.source, b.target);
+    const uint64_t k = edge_thr ? edge_thr[e] : thr[wi < 0 ? -wi : wi];
+    if ((uint64_t)cluster_word(key, e, step, kStreamClusterBond) < k) unite(parent, b.source, b.target);
   }
 }
 
@@ -742,11 +743,12 @@ hipError_t launch_cluster_on_csr(uint64_t n_sites, uint64_t n_edges, const Clust
 
 hipError_t launch_cluster_bonds_csr(uint64_t n_sites, uint64_t n_edges, const ClusterEdge *edges,
                                     const int8_t *weights, const int8_t *spins, uint32_t *parent, uint64_t key,
-                                    uint64_t step, const uint64_t *thr, hipStream_t stream) {
+                                    uint64_t step, const uint64_t *thr, const uint64_t *edge_thr,
+                                    hipStream_t stream) {
   hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n_sites)), dim3(kThreads), 0, stream, parent, n_sites);
   if (n_edges)
     hipLaunchKernelGGL(k_bonds_csr, dim3(blocks_for(n_edges)), dim3(kThreads), 0, stream, n_edges, edges, weights,
-                       spins, parent, key, step, thr);
+                       spins, parent, key, step, thr, edge_thr);
   return hipGetLastError();
 }
 

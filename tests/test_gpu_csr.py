@@ -104,8 +104,32 @@ def test_real_weights_bit_exact(sc, orc, case, kind, q, alg):
     assert e1 == e[-1] and m1 == mag[-1]
 
 
-def test_real_weight_clusters_refused(sc):
-    lat = sc.Lattice.from_edges(3, [0, 1], [1, 2], [0.5, 1.0])
+@pytest.mark.parametrize("graph", ["multigraph", "glass"])
+@pytest.mark.parametrize("kind,q", [(0, 2), (1, 3)])
+@pytest.mark.parametrize("wolff", [False, True])
+def test_real_weight_clusters_bit_exact(sc, orc, graph, kind, q, wolff):
+    """Swendsen-Wang / Wolff on random-bond ferromagnets (|w| of the real
+    weights): one threshold per edge, spins equal the oracle's bit for bit."""
+    name, n_sites, src, dst, w = next(c for c in REAL if c[0] == graph)
+    w = [abs(x) for x in w]
+    print(f"case real clusters {graph} kind={kind} wolff={wolff}", flush=True)
+    lat, olat = sc.Lattice.from_edges(n_sites, src, dst, w), orc.Lattice.from_edges(n_sites, src, dst, w)
+    seed, steps, betas = 81, 5, [0.3, 0.8]
+    st = sc.State(lat, sc.model(kind, 1.0, q), 2, seed)
+    for r in range(2):
+        st.init(True, r)
+        st.set_beta(betas[r], r)
+    m = orc.model(kind, 1.0, q)
+    start = [st.download_f64(r) for r in range(2)]
+    st.sweep(6 if wolff else 5, steps)
+    for r in range(2):
+        s = start[r].copy()
+        orc.cluster_steps(olat, m, s, betas[r], orc.replica_key(seed, r), 0, steps, wolff)
+        assert np.array_equal(st.download_f64(r), s), f"replica {r}"
+
+
+def test_real_weight_clusters_refuse_negative_bonds(sc):
+    lat = sc.Lattice.from_edges(3, [0, 1], [1, 2], [0.5, -1.25])
     st = sc.State(lat, sc.model(0), 1, 1)
     with pytest.raises(sc.ScamperError):
         st.sweep(5, 1)
