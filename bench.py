@@ -302,7 +302,8 @@ def on3(sc, args) -> dict:
             "higher_is_better": True, "dtype": "f32 spins, f64 observables", "data": "synthetic",
             "config": {"workload": "on3_hybrid", "L": L, "replicas": R, "T": [1.40, 1.49], "seed": 42},
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+                         "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "traffic": traffic_from_profile(os.path.join(ROOT, "profiles", "on3_L96_pmc.txt"))},
             **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_on3()})}
 
 
