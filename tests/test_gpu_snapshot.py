@@ -30,6 +30,11 @@ CASES = [
     (lambda sc: sc.Lattice.cubic(4), lambda sc: sc.model(sc.ON, 1.0, 2, 3), 4, 0.7, 14, 10, 15),
     (lambda sc: sc.Lattice.square(8, 8), lambda sc: sc.model(sc.ON, 1.0, 2, 2), 4, 1.1, 15, 10, 15),
     (lambda sc: sc.Lattice.square(8, 8), lambda sc: sc.model(sc.ON, 1.0, 2, 2), 5, 1.1, 16, 10, 15),
+    # graph lattices: O(3) antiferromagnet, XY clusters, real-weight Ising
+    (lambda sc: sc.Lattice.triangular(4, 4), lambda sc: sc.model(sc.ON, -1.0, 2, 3), 4, 0.9, 17, 10, 15),
+    (lambda sc: sc.Lattice.honeycomb(4, 4), lambda sc: sc.model(sc.ON, 1.0, 2, 2), 6, 1.0, 18, 10, 15),
+    (lambda sc: sc.Lattice.from_edges(5, [0, 1, 2, 3, 4, 1], [1, 2, 3, 4, 0, 1], [0.5, 1.75, -0.25, 2.0, 1.0, 0.5]),
+     lambda sc: sc.model(sc.ISING, 1.0), 0, 0.6, 19, 30, 40),
 ]
 
 
