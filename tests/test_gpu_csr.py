@@ -133,3 +133,20 @@ def test_real_weight_clusters_refuse_negative_bonds(sc):
     st = sc.State(lat, sc.model(0), 1, 1)
     with pytest.raises(sc.ScamperError):
         st.sweep(5, 1)
+
+
+def test_run_one_real_weight_chain(sc):
+    """Scheduler::run_one, Ising Metropolis on an open chain with real
+    weights (a tree): <E> = -sum w tanh(beta w)."""
+    import math
+    w = [0.35, 1.7, 0.8, 2.25, 0.6, 1.1, 0.45, 1.9]
+    n = len(w) + 1
+    beta, meas = 0.7, 40000
+    want = -sum(x * math.tanh(beta * x) for x in w)
+    res, e, _ = sc.run_one(sc.Lattice.from_edges(n, list(range(n - 1)), list(range(1, n)), w), sc.model(sc.ISING, 1.0),
+                           sc.METROPOLIS, beta, True, 500, meas, 100, 17, series=True)
+    assert res["Energy"].mean == pytest.approx(e.mean(), rel=1e-12)
+    est = sc.binned_estimate(e, meas // 40)
+    z = (est.mean - want) / est.stderr
+    print(f"run_one real chain: {est.mean:.5f} exact {want:.5f} z={z:.2f}")
+    assert abs(z) < 4.5

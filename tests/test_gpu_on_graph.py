@@ -127,3 +127,19 @@ def test_weighted_tree_energy(sc, n_comp):
     z = (means.mean() - want) / (means.std(ddof=1) / math.sqrt(n_rep))
     print(f"O({n_comp}) tree: <E> {means.mean():.5f} exact {want:.5f} z={z:.2f}")
     assert abs(z) < 4.5
+
+
+def test_run_one_xy_tree(sc):
+    """Scheduler::run_one over an XY state on a weighted tree (hybrid heat
+    bath + over-relaxation): the Energy estimate matches -sum w I1/I0(beta w)."""
+    from scipy.special import iv
+    src, dst, w = [0, 1, 1, 3], [1, 2, 3, 4], [1.0, 2.5, 0.5, 1.25]
+    beta, meas = 0.8, 40000
+    want = -sum(x * iv(1, beta * x) / iv(0, beta * x) for x in w)
+    res, e, _ = sc.run_one(sc.Lattice.from_edges(5, src, dst, w), sc.model(sc.ON, 1.0, 2, 2), sc.HYBRID_HB_OR,
+                           beta, True, 500, meas, 100, 13, series=True)
+    assert res["Energy"].mean == pytest.approx(e.mean(), rel=1e-12)
+    est = sc.binned_estimate(e, meas // 40)
+    z = (est.mean - want) / est.stderr
+    print(f"run_one XY tree: {est.mean:.5f} exact {want:.5f} z={z:.2f}")
+    assert abs(z) < 4.5
