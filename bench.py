@@ -16,6 +16,8 @@ Other workloads, one JSON line each (not the headline):
     --workload pt3d   64-replica multi-spin parallel tempering, 3-D L=128
     --workload on3    32-replica O(3) heat bath + over-relaxation, 3-D L=96
     --workload xy     32-replica O(2) (XY) von Mises heat bath + over-relaxation, 3-D L=96
+    --workload glass  64-replica 3-D Edwards-Anderson spin glass (Gaussian couplings, L=32),
+                      Metropolis through the f64 graph kernels
     --workload potts3d  the tiled Potts kernel alone: q=3, 3-D L=256, measured sweeps
     --workload cluster  Swendsen-Wang and Wolff updates of the 2-D Ising model at
                         beta_c, L=4096, and of the 3-D O(3) model near T_c, L=96
@@ -143,6 +145,36 @@ def cpu_xy() -> dict:
         ch.over_relaxation()
     return _cpu_timed(step, 2 * lat.n_sites, "site-updates/ns",
                       "XY heat-bath + over-relaxation sweeps of a 24^3 periodic lattice (f64, CSR)")
+
+
+def _glass_edges(L: int, seed: int = 7):
+    """Periodic L^3 Edwards-Anderson couplings: edges +x, +y, +z of every
+    site (build_hypercubic order), Gaussian weights."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for z in range(L):
+        for y in range(L):
+            for x in range(L):
+                s = x + L * (y + L * z)
+                src += [s, s, s]
+                dst += [(x + 1) % L + L * (y + L * z), x + L * ((y + 1) % L + L * z), x + L * (y + L * ((z + 1) % L))]
+    return L ** 3, src, dst, list(rng.normal(size=len(src)))
+
+
+def cpu_glass() -> dict:
+    """The spin-glass Metropolis sweep in the reference's layout (f64 spins,
+    CSR lattice with real weights, orc_sweep_metropolis), 3-D L=16."""
+    import oracle
+
+    n, src, dst, w = _glass_edges(16)
+    lat = oracle.Lattice.from_edges(n, src, dst, w)
+    m = oracle.model(oracle.ISING, 1.0, 2)
+    key = oracle.replica_key(42, 0)
+    ch = oracle.Chain(lat, m, oracle.random_start(lat, m, key))
+    return _cpu_timed(lambda i: ch.metropolis(1.0, key, i), n, "spin-flips/ns",
+                      "Metropolis sweeps of a 16^3 Gaussian spin glass (f64, CSR)")
 
 
 def cpu_potts3d() -> dict:
@@ -334,6 +366,36 @@ def xy(sc, args) -> dict:
             **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_xy()})}
 
 
+def glass(sc, args) -> dict:
+    """64 replicas of the 3-D Edwards-Anderson spin glass, L=32, Gaussian
+    couplings, T = 1.0; Metropolis sweeps through the f64 graph kernels (the
+    same ΔE and acceptance arithmetic as the reference)."""
+    L, R = 32, 64
+    n, src, dst, w = _glass_edges(L)
+    lat = sc.Lattice.from_edges(n, src, dst, w)
+    st = sc.State(lat, sc.model(sc.ISING, 1.0), R, 42)
+    for r in range(R):
+        st.init(True, r)
+        st.set_beta(1.0, r)
+    st.sweep(sc.METROPOLIS, args.warmup)
+    st.synchronize()
+    ms = st.sweep_timed(sc.METROPOLIS, args.glass_steps)
+    updates = R * n * args.glass_steps
+    # per update: own spin read + write (2 B), 6 neighbour spins (6 B), 6
+    # incidences of neighbour index + f64 weight (72 B), offsets (8 B) and
+    # class list (4 B): 92 B, most of it the replica-shared graph
+    gbs = 92 * updates / (ms * 1e-3) / 1e9
+    return {"metric": "spin-flips/ns (3-D Gaussian spin glass L=32, 64 replicas)",
+            "value": round(updates / (ms * 1e6), 3), "unit": "spin-flips/ns", "n_gpus": 1,
+            "steps": args.glass_steps, "warmup": args.warmup, "ms_per_step": ms / args.glass_steps,
+            "higher_is_better": True, "dtype": "int8 spins, f64 energies", "data": "synthetic",
+            "config": {"workload": "spin_glass_metropolis", "L": L, "replicas": R, "T": 1.0, "couplings": "gaussian",
+                       "seed": 42},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_glass()})}
+
+
 def potts3d(sc, args) -> dict:
     """The serving pool's kernel on one large lattice through the plain state
     path (no graphs, so a profiler sees every launch): 3-state Potts, 3-D
@@ -431,7 +493,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=16384)
-    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "xy", "potts3d", "cluster", "serve", "calibrate"],
+    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "xy", "glass", "potts3d", "cluster", "serve", "calibrate"],
                     default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -442,6 +504,7 @@ def main() -> int:
     ap.add_argument("--on3-steps", type=int, default=200)
     ap.add_argument("--potts-steps", type=int, default=1000)
     ap.add_argument("--cluster-steps", type=int, default=200)
+    ap.add_argument("--glass-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU "
@@ -458,6 +521,8 @@ def main() -> int:
         line = on3(sc, args)
     elif args.workload == "xy":
         line = xy(sc, args)
+    elif args.workload == "glass":
+        line = glass(sc, args)
     elif args.workload == "potts3d":
         line = potts3d(sc, args)
     elif args.workload == "cluster":
