@@ -28,15 +28,25 @@ __device__ __forceinline__ double bond(int kind, double j, double w, int l, int 
   return kind == kIsing ? -j * w * (double)l * (double)r : (l == r ? -j * w : 0.0);
 }
 
-__device__ double local_energy(const RealCsrGeom &g, int kind, double j, const int8_t *s, uint32_t site, int value) {
-  double energy = 0.0;
+// local(a) and local(b) in one pass over the incidences: each sum is still
+// taken from 0.0 in CSR order, so both equal separate passes bit for bit.
+__device__ void local_energies(const RealCsrGeom &g, int kind, double j, const int8_t *s, uint32_t site, int a, int b,
+                               double *ea, double *eb) {
+  double sa = 0.0, sb = 0.0;
   for (uint32_t i = g.offsets[site]; i < g.offsets[site + 1]; ++i) {
     const uint32_t t = g.neighbors[i];
     const double w = g.weights[i];
-    if (t == site) energy += 0.5 * bond(kind, j, w, value, value);
-    else energy += bond(kind, j, w, value, (int)s[t]);
+    if (t == site) {
+      sa += 0.5 * bond(kind, j, w, a, a);
+      sb += 0.5 * bond(kind, j, w, b, b);
+    } else {
+      const int n = s[t];
+      sa += bond(kind, j, w, a, n);
+      sb += bond(kind, j, w, b, n);
+    }
   }
-  return energy;
+  *ea = sa;
+  *eb = sb;
 }
 
 __device__ __forceinline__ bool accept_log(double log_a, double u) {
@@ -68,7 +78,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_real(int8_t *__restrict__ spins,
       proposed = (int)(((uint64_t)pw * (uint64_t)(q - 1)) >> 32);
       if (proposed >= old) proposed += 1;
     }
-    const double de = local_energy(g, KIND, j, s, site, proposed) - local_energy(g, KIND, j, s, site, old);
+    double e_new, e_old;
+    local_energies(g, KIND, j, s, site, proposed, old, &e_new, &e_old);
+    const double de = e_new - e_old;
     const double term = de == INFINITY ? -INFINITY : (de == -INFINITY ? INFINITY : -P.beta * de);
     if (accept_log((term + 0.0) + 0.0, u)) value = proposed;
   } else if (KIND == kIsing) {
