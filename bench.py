@@ -18,6 +18,8 @@ Other workloads, one JSON line each (not the headline):
     --workload xy     32-replica O(2) (XY) von Mises heat bath + over-relaxation, 3-D L=96
     --workload glass  64-replica 3-D Edwards-Anderson spin glass (Gaussian couplings, L=32),
                       Metropolis through the f64 graph kernels
+    --workload tri    32-replica O(3) antiferromagnet on the triangular lattice (L=256),
+                      heat bath + over-relaxation through the graph-lattice O(N) kernels
     --workload potts3d  the tiled Potts kernel alone: q=3, 3-D L=256, measured sweeps
     --workload cluster  Swendsen-Wang and Wolff updates of the 2-D Ising model at
                         beta_c, L=4096, and of the 3-D O(3) model near T_c, L=96
@@ -396,6 +398,51 @@ def glass(sc, args) -> dict:
             **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_glass()})}
 
 
+def cpu_tri() -> dict:
+    """O(3) heat bath + over-relaxation on the triangular lattice in the
+    reference's layout (f64, CSR), L=32, J = -1, T = 0.5."""
+    import oracle
+
+    lat = oracle.Lattice.triangular(32, 32)
+    m = oracle.model(oracle.ON, -1.0, 2, 3)
+    key = oracle.replica_key(42, 0)
+    ch = oracle.Chain(lat, m, oracle.random_start(lat, m, key))
+
+    def step(i):
+        ch.on3_heat_bath(2.0, key, i)
+        ch.over_relaxation()
+    return _cpu_timed(step, 2 * lat.n_sites, "site-updates/ns",
+                      "heat-bath + over-relaxation sweeps of a 32x32 triangular antiferromagnet (f64, CSR)")
+
+
+def tri(sc, args) -> dict:
+    """32 replicas of the classical Heisenberg antiferromagnet (J = -1) on the
+    256 x 256 triangular lattice at T = 0.5; each step one heat-bath sweep and
+    one over-relaxation sweep over the lattice's colour classes."""
+    L, R = 256, 32
+    lat = sc.Lattice.triangular(L, L)
+    st = sc.State(lat, sc.model(sc.ON, -1.0, 2, 3), R, 42)
+    for r in range(R):
+        st.init(True, r)
+        st.set_beta(2.0, r)
+    st.sweep(sc.HYBRID_HB_OR, args.warmup)
+    st.synchronize()
+    ms = st.sweep_timed(sc.HYBRID_HB_OR, args.on3_steps)
+    updates = 2 * R * lat.n_sites * args.on3_steps
+    # per update: own 12 B read + write, 6 neighbours' 12 B (with reuse), 6
+    # incidences of neighbour index + f64 weight (72 B, replica-shared), 8 B
+    # offsets, 4 B class list: 120 B
+    gbs = 120 * updates / (ms * 1e-3) / 1e9
+    return {"metric": "site-updates/ns (O(3) triangular antiferromagnet L=256, 32 replicas, HB+OR)",
+            "value": round(updates / (ms * 1e6), 3), "unit": "site-updates/ns", "n_gpus": 1,
+            "steps": args.on3_steps, "warmup": args.warmup, "ms_per_step": ms / args.on3_steps,
+            "higher_is_better": True, "dtype": "f32 spins, f64 observables", "data": "synthetic",
+            "config": {"workload": "on3_triangular_af", "L": L, "replicas": R, "J": -1.0, "T": 0.5, "seed": 42},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_tri()})}
+
+
 def potts3d(sc, args) -> dict:
     """The serving pool's kernel on one large lattice through the plain state
     path (no graphs, so a profiler sees every launch): 3-state Potts, 3-D
@@ -493,7 +540,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=16384)
-    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "xy", "glass", "potts3d", "cluster", "serve", "calibrate"],
+    ap.add_argument("--workload", choices=["ising2d", "pt3d", "on3", "xy", "glass", "tri", "potts3d", "cluster", "serve", "calibrate"],
                     default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -523,6 +570,8 @@ def main() -> int:
         line = xy(sc, args)
     elif args.workload == "glass":
         line = glass(sc, args)
+    elif args.workload == "tri":
+        line = tri(sc, args)
     elif args.workload == "potts3d":
         line = potts3d(sc, args)
     elif args.workload == "cluster":
