@@ -3,22 +3,24 @@
 //
 // One thread owns 16 consecutive bytes (16 sites) of a row and walks R rows
 // of one plane, keeping the other colour's rows y-1, y, y+1 in registers and
-// issuing row r+1's loads before row r is computed.  Per row it needs two
-// Philox blocks: their 16 bytes are the top and second bytes of the 16
-// sites' draws, laid out exactly like the spins, so every decision is made
-// four sites at a time with byte-wise integer arithmetic (SWAR):
+// issuing row r+1's loads before row r is computed.  Per row it needs one
+// Philox block: its 16 bytes are the top bytes of the 16 sites' draws, laid
+// out exactly like the spins, so every decision is made four sites at a time
+// with byte-wise integer arithmetic (SWAR):
 //
 //   anti  = number of anti-aligned neighbours per byte (spins are 0x01/0xFF,
 //           so (s ^ n) has bit 1 set exactly when the two differ)
-//   t8,m8 = top and second byte of the threshold K[anti]   (v_perm lookups)
-//   flip  = (top, mid) < (t8, m8), or K[anti] = 2^32 (every word accepts);
-//           the pairs are interleaved into 16-bit lanes by v_perm and
-//           compared two sites at a time with packed u16 max/min
-//   tie   = (top, mid) == (t8, m8)   (one uphill site in ~65536)
+//   t8    = top byte of the threshold K[anti]            (v_perm lookup)
+//   flip  = top < t8, or K[anti] = 2^32 (every word accepts)
+//   tie   = top == t8 and K[anti] < 2^32  (one uphill site in 256)
 //
-// A tie needs the low 16 bits of the draw from a third block.  The row loop
-// only records where ties are (one LDS word per row, no branch); after the R
-// rows each tied site is settled from its neighbours.
+// A tie is decided by the lower 24 bits of its draw against those of K: the
+// mid byte (a second block, shared by the row's 16 sites) and, when that
+// ties too, the low half (a third block).  The row loop parks a tied row in
+// a small per-thread LDS list together with the anti-aligned counts and spins
+// of its 16 sites, and the ties are settled after the R rows, so the second
+// block is drawn for ~2% of rows instead of all of them.  A thread whose list
+// is full settles the row at once.
 //
 // The per-site rule is MetropolisCore's (CMC.rs/src/algorithms/metropolis.rs:
 // 67-106) with accept_log_probability (Carlo.rs/src/acceptance.rs:11-22)
@@ -33,6 +35,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kMsb = 0x80808080u;
 constexpr uint32_t kMaxRows = 16;
+constexpr uint32_t kSlots = 4;  // tied rows a thread can park before settling at once
 
 __device__ __forceinline__ uint32_t wrap_dec(uint32_t v, uint32_t n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ uint32_t wrap_inc(uint32_t v, uint32_t n) { return v + 1 == n ? 0 : v + 1; }
@@ -52,56 +55,71 @@ __device__ __forceinline__ void store16(int8_t *p, const Chunk &c) {
 // 0x02 in every byte where a and b differ (spins 0x01 / 0xFF).
 __device__ __forceinline__ uint32_t differ2(uint32_t a, uint32_t b) { return (a ^ b) & 0x02020202u; }
 
-// Packed unsigned 16-bit max / min of two lanes (v_pk_max_u16, v_pk_min_u16).
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
-  return __builtin_bit_cast(uint32_t,
-                            __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+// Bit 7 of each byte set where byte x < byte y (unsigned).  d's bytes are
+// (x | 0x80) - (y & 0x7f) >= 1, so no borrow crosses a byte, and bit 7 of d
+// is set exactly when x mod 128 >= y mod 128; x < y when the top bits say so,
+// or they agree and the low seven bits are smaller.
+__device__ __forceinline__ uint32_t bytes_lt(uint32_t x, uint32_t y) {
+  const uint32_t d = (x | kMsb) - (y & ~kMsb);
+  return ((~x & y) | (~(x ^ y) & ~d)) & kMsb;
 }
-__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
-  return __builtin_bit_cast(uint32_t,
-                            __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+// Bit 7 of each byte set where byte x == byte y: (e & 0x7f) + 0x7f reaches
+// bit 7 exactly when the low seven bits of e = x ^ y are not all zero.
+__device__ __forceinline__ uint32_t bytes_eq(uint32_t x, uint32_t y) {
+  const uint32_t e = x ^ y;
+  return ~(((e & ~kMsb) + ~kMsb) | e) & kMsb;
 }
 
-// v_perm selectors: the 16-bit values (top << 8 | mid) of sites 0, 1 (kPair01)
-// or 2, 3 (kPair23) of a word, from perm(top word, mid word); and the bytes 0
-// and 2 of two such lane words gathered back into site order (kGather).
-constexpr uint32_t kPair01 = 0x05010400u, kPair23 = 0x07030602u, kGather = 0x06040200u;
+// A parked row: which of its sites tie on the top byte, and per site the
+// anti-aligned count (3 bits) and whether the spin is -1 (1 bit).
+struct TiedRow {
+  uint32_t meta;  // bit 8b + 7 - k: site 4k + b ties; bits 0..3: row r
+  uint32_t a01;   // byte b: site b in bits 0..3, site 4 + b in bits 4..7
+  uint32_t a23;   // byte b: site 8 + b in bits 0..3, site 12 + b in bits 4..7
+};
 
-// Settle one tied site: its spin is still the pre-sweep value; recount its
-// anti-aligned neighbours and compare the low 24 bits of its draw.
+// Settle the ties of one row.  Their spins still hold the pre-sweep values;
+// a tied site flips when the lower 24 bits of its draw, mid byte then low
+// half, are below those of K.
 template <int ND, bool MEAS>
-__device__ void settle_tie(int8_t *__restrict__ own, const int8_t *__restrict__ oth, const StencilGeom &g,
-                           uint32_t y, uint32_t zz, uint32_t i, uint32_t colour, uint64_t sweep,
-                           const ReplicaParams &P, long long &acc0, long long &acc1) {
+__device__ __forceinline__ void settle_row(int8_t *__restrict__ own, uint64_t j0, const TiedRow &t, uint64_t sweep,
+                                           uint32_t colour, const ReplicaParams &P, long long &acc0,
+                                           long long &acc1) {
   constexpr int z = 2 * ND;
-  const uint64_t plane = (uint64_t)g.ly * zz;
-  const uint64_t j = (plane + y) * g.lx2 + i;
-  const int8_t s = own[j];
-  const uint32_t p = (colour + y + zz) & 1u;
-  const uint64_t rb = (plane + y) * g.lx2;
-  int anti = (oth[rb + (p ? i : wrap_dec(i, g.lx2))] != s) + (oth[rb + (p ? wrap_inc(i, g.lx2) : i)] != s) +
-             (oth[(plane + wrap_dec(y, g.ly)) * g.lx2 + i] != s) + (oth[(plane + wrap_inc(y, g.ly)) * g.lx2 + i] != s);
-  if (ND == 3)
-    anti += (oth[((uint64_t)g.ly * wrap_dec(zz, g.lz) + y) * g.lx2 + i] != s) +
-            (oth[((uint64_t)g.ly * wrap_inc(zz, g.lz) + y) * g.lx2 + i] != s);
-  if (site_low(P.key, j, sweep, colour, kStreamAccept) < P.thr[anti].k16) {
-    own[j] = -s;
-    if (MEAS) {
-      acc0 += 4ll * anti - 2ll * z;  // bond sum of the site changes sign
-      acc1 += s < 0 ? 2 : -2;
+  const U4 M = byte_block(P.key, j0 >> 4, sweep, colour, kStreamAccept, 1);
+  uint32_t tp = t.meta & 0xF0F0F0F0u;
+  while (tp) {
+    const uint32_t bit = __builtin_ctz(tp);
+    tp &= tp - 1;
+    const uint32_t b = bit >> 3, k = 7u - (bit & 7u);
+    const uint32_t s16 = 4 * k + b;
+    const uint32_t nib = ((k < 2 ? t.a01 : t.a23) >> (8 * b + 4 * (k & 1u))) & 0xFu;
+    const uint32_t anti = nib & 7u;
+    const Threshold th = P.thr[anti];
+    const uint32_t mid = (word_of(M, k) >> (8 * b)) & 0xFFu;
+    bool flip = mid < th.m8;
+    if (mid == th.m8) flip = site_low(P.key, j0 + s16, sweep, colour, kStreamAccept) < th.k16;
+    if (flip) {
+      const bool neg = (nib >> 3) != 0;
+      own[j0 + s16] = neg ? 1 : -1;
+      if (MEAS) {
+        acc0 += 4ll * anti - 2ll * z;  // bond sum of the site changes sign
+        acc1 += neg ? 2 : -2;
+      }
     }
   }
 }
 
-template <int ND, bool MEAS>
+// R rows per thread, a compile-time constant so the row loop unrolls fully
+// and the prefetched rows stay in their own registers (a rolled loop copies
+// them into the current row's registers, which waits for the loads).
+template <int ND, bool MEAS, uint32_t R>
 __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spins,
                                                        const ReplicaParams *__restrict__ reps,
                                                        StencilGeom g, uint32_t colour,
-                                                       uint64_t sweep, uint32_t R,
-                                                       long long *__restrict__ em) {
+                                                       uint64_t sweep, long long *__restrict__ em) {
   constexpr uint32_t z = 2 * ND;
-  __shared__ uint32_t tie_rows[kMaxRows * kBlock];
+  __shared__ TiedRow parked[kSlots][kBlock];
   const uint32_t rep = blockIdx.y;
   // XCD-aware order: the hardware deals consecutive blocks round-robin over
   // 8 XCDs; remap so each XCD walks one contiguous band of rows and the
@@ -111,29 +129,21 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
   if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
   const ReplicaParams &P = reps[rep];
 
-  // byte LUTs indexed by the anti-aligned count (<= 6 < 8): top and second
-  // byte of K, and 1 where K = 2^32
-  uint32_t t_lo = 0, t_hi = 0, m_lo = 0, m_hi = 0, al_lo = 0, al_hi = 0;
+  // byte LUTs indexed by the anti-aligned count (<= 6 < 8): the top byte of
+  // K, and 0x80 where K = 2^32
+  uint32_t t_lo = 0, t_hi = 0, al_lo = 0, al_hi = 0;
 #pragma unroll
   for (uint32_t a = 0; a <= z; ++a) {
     const uint32_t k8 = P.thr[a].k8;
-    const uint32_t tb = k8 > 0xFFu ? 0xFFu : k8, ab = k8 > 0xFFu ? 1u : 0u;
-    const uint32_t mb = P.thr[a].m8;
+    const uint32_t tb = k8 > 0xFFu ? 0xFFu : k8, ab = k8 > 0xFFu ? 0x80u : 0u;
     if (a < 4) {
       t_lo |= tb << (8 * a);
-      m_lo |= mb << (8 * a);
       al_lo |= ab << (8 * a);
     } else {
       t_hi |= tb << (8 * (a - 4));
-      m_hi |= mb << (8 * (a - 4));
       al_hi |= ab << (8 * (a - 4));
     }
   }
-
-  // 1 in both 16-bit lanes, opaque to the optimiser, which would otherwise
-  // turn min(x, 1) into per-lane compares and selects
-  uint32_t lanes1 = 0x00010001u;
-  asm volatile("" : "+v"(lanes1));
 
   const uint32_t cpr = g.lx2 >> 4;
   const uint64_t tid = (uint64_t)b * kBlock + threadIdx.x;
@@ -152,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
     const uint64_t key = P.key;
     const uint32_t left_i = chunk == 0 ? g.lx2 - 4 : i0 - 4;  // dword holding byte i0-1
     const uint32_t right_i = i0 + 16 == g.lx2 ? 0 : i0 + 16;   // dword holding byte i0+16
-    uint32_t tied_rows = 0;
+    uint32_t n_parked = 0;
 
     // Row r+1's loads are issued before row r is computed (the rows are
     // distinct and the other colour is read-only during a half-sweep), so
@@ -172,6 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
       zm = load16(oth + (zm_plane + y0) * g.lx2 + i0);
       zp = load16(oth + (zp_plane + y0) * g.lx2 + i0);
     }
+#pragma unroll
     for (uint32_t r = 0; r < R; ++r) {
       const uint32_t y = y0 + r;
       const uint64_t row = plane + y;
@@ -202,31 +213,23 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
       }
       const uint64_t j0 = row * g.lx2 + i0;
       const U4 T = byte_block(key, j0 >> 4, sweep, colour, kStreamAccept, 0);
-      const U4 M = byte_block(key, j0 >> 4, sweep, colour, kStreamAccept, 1);
-      uint32_t tpos = 0;  // bit 8b+k: site 4k+b is tied
+      uint32_t tpos = 0;        // bit 8b + 7 - k: site 4k + b ties
+      uint32_t anti[2] = {0, 0};  // packed counts, as TiedRow::a01 / a23
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         uint32_t a2 = differ2(o.w[k], up.w[k]) + differ2(o.w[k], down.w[k]) +
                       differ2(o.w[k], mid.w[k]) + differ2(o.w[k], sh.w[k]);
         if (ND == 3) a2 += differ2(o.w[k], zm.w[k]) + differ2(o.w[k], zp.w[k]);
         const uint32_t a = a2 >> 1;  // anti-aligned count per byte, 0..z
+        anti[k >> 1] |= a << (4 * (k & 1));
         const uint32_t t8 = __builtin_amdgcn_perm(t_hi, t_lo, a);
-        const uint32_t m8 = __builtin_amdgcn_perm(m_hi, m_lo, a);
-        const uint32_t always = __builtin_amdgcn_perm(al_hi, al_lo, a);  // 1 where K = 2^32
-        const uint32_t ht = word_of(T, k), hm = word_of(M, k);
-        // word < K from its top two bytes: (top, mid) < (t8, m8) as 16-bit
-        // lanes, two sites per packed compare; u < k is max(u, k) != u
-        const uint32_t u01 = __builtin_amdgcn_perm(ht, hm, kPair01), u23 = __builtin_amdgcn_perm(ht, hm, kPair23);
-        const uint32_t k01 = __builtin_amdgcn_perm(t8, m8, kPair01), k23 = __builtin_amdgcn_perm(t8, m8, kPair23);
-        const uint32_t lt01 = pk_min_u16(pk_max_u16(u01, k01) ^ u01, lanes1);
-        const uint32_t lt23 = pk_min_u16(pk_max_u16(u23, k23) ^ u23, lanes1);
-        const uint32_t m = __builtin_amdgcn_perm(lt23, lt01, kGather) | always;  // 1 per accepted site
-        // (top, mid) == (t8, m8): a tie, settled from the low bytes unless K = 2^32
-        const uint32_t ne01 = pk_min_u16(u01 ^ k01, lanes1), ne23 = pk_min_u16(u23 ^ k23, lanes1);
-        const uint32_t tie = ~__builtin_amdgcn_perm(ne23, ne01, kGather) & ~always & 0x01010101u;
-        tpos |= tie << k;
-        // bytes 0x00 / 0xFF from selectors 0 / 1 into 0x0000FF00
-        const uint32_t flip = __builtin_amdgcn_perm(0u, 0x0000FF00u, m) & 0xFEFEFEFEu;
+        const uint32_t always = __builtin_amdgcn_perm(al_hi, al_lo, a);  // 0x80 where K = 2^32
+        const uint32_t top = word_of(T, k);
+        const uint32_t acc = bytes_lt(top, t8) | always;  // 0x80 per accepted site
+        tpos |= (bytes_eq(top, t8) & ~always) >> k;
+        const uint32_t m = acc >> 7;  // 1 per accepted site
+        // bytes 0x00 / 0xFE from selectors 0 / 1 into 0x0000FE00
+        const uint32_t flip = __builtin_amdgcn_perm(0u, 0x0000FE00u, m);
         if (MEAS) {
           // bond sum over the chunk's sites with their new spins:
           // sum (z - 2 * anti_new), anti_new = z - anti where flipped
@@ -237,8 +240,17 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
         o.w[k] ^= flip;
       }
       store16(own + row * g.lx2 + i0, o);
-      tie_rows[r * kBlock + threadIdx.x] = tpos;
-      tied_rows |= (tpos != 0 ? 1u : 0u) << r;
+      if (tpos) {
+        // tied sites kept their spins: record the spins' signs beside the counts
+        TiedRow t;
+        t.meta = tpos | r;
+        t.a01 = anti[0] | ((o.w[0] & kMsb) >> 4) | (o.w[1] & kMsb);
+        t.a23 = anti[1] | ((o.w[2] & kMsb) >> 4) | (o.w[3] & kMsb);
+        if (n_parked < kSlots)
+          parked[n_parked++][threadIdx.x] = t;
+        else
+          settle_row<ND, MEAS>(own, j0, t, sweep, colour, P, acc0, acc1);
+      }
       up = mid;
       mid = down;
       down = down_n;
@@ -249,17 +261,10 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
         zp = zp_n;
       }
     }
-    // settle the tied sites (tied sites were left unflipped above)
-    while (tied_rows) {
-      const uint32_t r = __builtin_ctz(tied_rows);
-      tied_rows &= tied_rows - 1;
-      uint32_t tp = tie_rows[r * kBlock + threadIdx.x];
-      while (tp) {
-        const uint32_t bit = __builtin_ctz(tp);
-        tp &= tp - 1;
-        const uint32_t s16 = 4 * (bit & 7u) + (bit >> 3);
-        settle_tie<ND, MEAS>(own, oth, g, y0 + r, zz, i0 + s16, colour, sweep, P, acc0, acc1);
-      }
+    for (uint32_t s = 0; s < n_parked; ++s) {
+      const TiedRow t = parked[s][threadIdx.x];
+      const uint64_t j0 = (plane + y0 + (t.meta & 0xFu)) * g.lx2 + i0;
+      settle_row<ND, MEAS>(own, j0, t, sweep, colour, P, acc0, acc1);
     }
   }
   if (MEAS) {
@@ -287,6 +292,21 @@ uint32_t fast_rows(const StencilGeom &g) {
   return 1;
 }
 
+template <int ND, bool MEAS>
+void launch_rows(uint32_t R, dim3 grid, hipStream_t st, int8_t *spins, const ReplicaParams *reps,
+                 const StencilGeom &g, uint32_t colour, uint64_t sweep, long long *em) {
+#define SC_ROWS(RR) \
+  hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, em)
+  switch (R) {
+    case 16: SC_ROWS(16); break;
+    case 8: SC_ROWS(8); break;
+    case 4: SC_ROWS(4); break;
+    case 2: SC_ROWS(2); break;
+    default: SC_ROWS(1); break;
+  }
+#undef SC_ROWS
+}
+
 }  // namespace
 
 bool ising_fast_path_ok(const StencilGeom &g) {
@@ -300,11 +320,11 @@ hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaP
   const uint64_t threads = (uint64_t)(g.lx2 / 16) * (g.ly / R) * g.lz;
   dim3 grid((unsigned)((threads + kBlock - 1) / kBlock), n_rep);
   if (g.nd == 2) {
-    if (em) hipLaunchKernelGGL((k_ising_fast<2, true>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, R, em);
-    else hipLaunchKernelGGL((k_ising_fast<2, false>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, R, em);
+    if (em) launch_rows<2, true>(R, grid, st, spins, reps, g, colour, sweep, em);
+    else launch_rows<2, false>(R, grid, st, spins, reps, g, colour, sweep, em);
   } else {
-    if (em) hipLaunchKernelGGL((k_ising_fast<3, true>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, R, em);
-    else hipLaunchKernelGGL((k_ising_fast<3, false>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, R, em);
+    if (em) launch_rows<3, true>(R, grid, st, spins, reps, g, colour, sweep, em);
+    else launch_rows<3, false>(R, grid, st, spins, reps, g, colour, sweep, em);
   }
   return hipGetLastError();
 }

@@ -61,9 +61,9 @@ def test_run_one_c1(sc):
         gb = np.array(g["block_means"][name])
         z = (mu - gb.mean()) / math.sqrt(se**2 + (gb.std(ddof=1) / math.sqrt(gb.size)) ** 2)
         print(f"{name}: device {mu:.6g} vs faithful CPU {gb.mean():.6g}, z = {z:.2f}")
-        assert abs(z) < 3.5
+        assert abs(z) < 3
     u_dev, su_dev = _binder(blocks["M2"], blocks["M4"])
     u_cpu, su_cpu = _binder(np.array(g["block_means"]["M2"]), np.array(g["block_means"]["M4"]))
     z = (u_dev - u_cpu) / math.sqrt(su_dev**2 + su_cpu**2)
     print(f"Binder: device {u_dev:.4f}+-{su_dev:.4f} CPU {u_cpu:.4f}+-{su_cpu:.4f} z = {z:.2f}")
-    assert abs(z) < 3.5
+    assert abs(z) < 3
