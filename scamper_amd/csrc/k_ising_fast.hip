@@ -35,7 +35,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kMsb = 0x80808080u;
 constexpr uint32_t kMaxRows = 16;
-constexpr uint32_t kSlots = 4;  // tied rows a thread can park before settling at once
+constexpr uint32_t kSlots = 4;
+constexpr uint32_t kPrefetch = 1;  // rows whose loads are in flight ahead of the row computed
+// DPP controls: every lane takes the value of lane - 1 (shr) or lane + 1 (shl)
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;  // tied rows a thread can park before settling at once
 
 __device__ __forceinline__ uint32_t wrap_dec(uint32_t v, uint32_t n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ uint32_t wrap_inc(uint32_t v, uint32_t n) { return v + 1 == n ? 0 : v + 1; }
@@ -69,6 +72,12 @@ __device__ __forceinline__ uint32_t bytes_eq(uint32_t x, uint32_t y) {
   const uint32_t e = x ^ y;
   return ~(((e & ~kMsb) + ~kMsb) | e) & kMsb;
 }
+
+// One row's loads beyond the rows above it.
+struct RowIn {
+  Chunk down, o, zm, zp;
+  uint32_t edge;
+};
 
 // A parked row: which of its sites tie on the top byte, and per site the
 // anti-aligned count (3 bits) and whether the spin is -1 (1 bit).
@@ -113,7 +122,7 @@ __device__ __forceinline__ void settle_row(int8_t *__restrict__ own, uint64_t j0
 // R rows per thread, a compile-time constant so the row loop unrolls fully
 // and the prefetched rows stay in their own registers (a rolled loop copies
 // them into the current row's registers, which waits for the loads).
-template <int ND, bool MEAS, uint32_t R>
+template <int ND, bool MEAS, uint32_t R, typename Off, uint32_t PF, bool NBR>
 __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spins,
                                                        const ReplicaParams *__restrict__ reps,
                                                        StencilGeom g, uint32_t colour,
@@ -158,46 +167,67 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
     const uint32_t i0 = chunk * 16;
     int8_t *__restrict__ own = spins + (2ull * rep + colour) * g.nh;
     const int8_t *__restrict__ oth = spins + (2ull * rep + 1 - colour) * g.nh;
-    const uint64_t plane = (uint64_t)g.ly * zz;
+    // offsets inside one colour array: 32-bit when it has < 2^32 sites, so
+    // every load takes one offset register beside the array's scalar base
+    const Off lx2 = g.lx2;
+    const Off plane = (Off)g.ly * zz;
     const uint64_t key = P.key;
     const uint32_t left_i = chunk == 0 ? g.lx2 - 4 : i0 - 4;  // dword holding byte i0-1
     const uint32_t right_i = i0 + 16 == g.lx2 ? 0 : i0 + 16;   // dword holding byte i0+16
     uint32_t n_parked = 0;
 
-    // Row r+1's loads are issued before row r is computed (the rows are
+    // Row r+PF's loads are issued before row r is computed (the rows are
     // distinct and the other colour is read-only during a half-sweep), so
     // memory latency overlaps the draws and decisions.
+    // The horizontal partner of the chunk's first (p = 0) or last (p = 1)
+    // site is a byte of the next chunk to the left or right.  With NBR a
+    // wave's lanes hold 64 consecutive chunks of one row, so that byte is in
+    // the neighbouring lane's registers (a DPP wave shift); only the lane at
+    // the wave's edge loads it.
+    const uint32_t lane = threadIdx.x & 63u;
     auto edge_of = [&](uint32_t y) {
       const uint32_t pe = (colour + y + zz) & 1u;
-      return *reinterpret_cast<const uint32_t *>(oth + (plane + y) * g.lx2 + (pe ? right_i : left_i));
+      const uint32_t *src = reinterpret_cast<const uint32_t *>(oth + ((plane + y) * lx2 + (pe ? right_i : left_i)));
+      if (!NBR) return *src;
+      uint32_t v = 0;
+      if (lane == (pe ? 63u : 0u)) v = *src;
+      return v;
     };
-    const uint64_t zm_plane = (uint64_t)g.ly * wrap_dec(zz, g.lz), zp_plane = (uint64_t)g.ly * wrap_inc(zz, g.lz);
-    Chunk up = load16(oth + (plane + wrap_dec(y0, g.ly)) * g.lx2 + i0);
-    Chunk mid = load16(oth + (plane + y0) * g.lx2 + i0);
-    Chunk down = load16(oth + (plane + wrap_inc(y0, g.ly)) * g.lx2 + i0);
-    Chunk o = load16(own + (plane + y0) * g.lx2 + i0);
-    uint32_t edge = edge_of(y0);
-    Chunk zm, zp;
-    if (ND == 3) {
-      zm = load16(oth + (zm_plane + y0) * g.lx2 + i0);
-      zp = load16(oth + (zp_plane + y0) * g.lx2 + i0);
-    }
+    const Off zm_plane = (Off)g.ly * wrap_dec(zz, g.lz), zp_plane = (Off)g.ly * wrap_inc(zz, g.lz);
+    // what row y0 + rr needs beyond the rows above it: the other colour's row
+    // below, its own spins, the horizontal edge and (3-D) the planes beside
+    auto fetch = [&](uint32_t rr) {
+      RowIn v;
+      const uint32_t yy = y0 + rr;
+      v.down = load16(oth + ((plane + wrap_inc(yy, g.ly)) * lx2 + i0));
+      v.o = load16(own + ((plane + yy) * lx2 + i0));
+      v.edge = edge_of(yy);
+      if (ND == 3) {
+        v.zm = load16(oth + ((zm_plane + yy) * lx2 + i0));
+        v.zp = load16(oth + ((zp_plane + yy) * lx2 + i0));
+      }
+      return v;
+    };
+    Chunk up = load16(oth + ((plane + wrap_dec(y0, g.ly)) * lx2 + i0));
+    Chunk mid = load16(oth + ((plane + y0) * lx2 + i0));
+    RowIn cur = fetch(0), n1, n2;
+    if (PF == 2 && R > 1) n1 = fetch(1);
 #pragma unroll
     for (uint32_t r = 0; r < R; ++r) {
       const uint32_t y = y0 + r;
-      const uint64_t row = plane + y;
-      Chunk down_n, o_n, zm_n, zp_n;
-      uint32_t edge_n = 0;
-      if (r + 1 < R) {
-        down_n = load16(oth + (plane + wrap_inc(y + 1, g.ly)) * g.lx2 + i0);
-        o_n = load16(own + (row + 1) * g.lx2 + i0);
-        edge_n = edge_of(y + 1);
-        if (ND == 3) {
-          zm_n = load16(oth + (zm_plane + y + 1) * g.lx2 + i0);
-          zp_n = load16(oth + (zp_plane + y + 1) * g.lx2 + i0);
-        }
-      }
+      const Off row = plane + y;
+      if (PF == 1 && r + 1 < R) n1 = fetch(r + 1);
+      if (PF == 2 && r + 2 < R) n2 = fetch(r + 2);
+      const Chunk &down = cur.down, &zm = cur.zm, &zp = cur.zp;
       const uint32_t p = (colour + y + zz) & 1u;
+      uint32_t edge = cur.edge;
+      if (NBR) {
+        // lane - 1's last word (p = 0) or lane + 1's first word (p = 1)
+        const uint32_t from_nbr = p ? (uint32_t)__builtin_amdgcn_mov_dpp((int)mid.w[0], kDppWaveShl1, 0xF, 0xF, false)
+                                    : (uint32_t)__builtin_amdgcn_mov_dpp((int)mid.w[3], kDppWaveShr1, 0xF, 0xF, false);
+        if (lane != (p ? 63u : 0u)) edge = from_nbr;
+      }
+      Chunk &o = cur.o;
       // horizontal partner: bytes i0-1..i0+14 (p = 0) or i0+1..i0+16 (p = 1)
       Chunk sh;
       if (p == 0) {
@@ -211,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
         sh.w[2] = __builtin_amdgcn_alignbyte(mid.w[3], mid.w[2], 1);
         sh.w[3] = __builtin_amdgcn_alignbyte(edge, mid.w[3], 1);
       }
-      const uint64_t j0 = row * g.lx2 + i0;
+      const Off j0 = row * lx2 + i0;
       const U4 T = byte_block(key, j0 >> 4, sweep, colour, kStreamAccept, 0);
       uint32_t tpos = 0;        // bit 8b + 7 - k: site 4k + b ties
       uint32_t anti[2] = {0, 0};  // packed counts, as TiedRow::a01 / a23
@@ -239,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
         }
         o.w[k] ^= flip;
       }
-      store16(own + row * g.lx2 + i0, o);
+      store16(own + (row * lx2 + i0), o);
       if (tpos) {
         // tied sites kept their spins: record the spins' signs beside the counts
         TiedRow t;
@@ -253,17 +283,12 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
       }
       up = mid;
       mid = down;
-      down = down_n;
-      o = o_n;
-      edge = edge_n;
-      if (ND == 3) {
-        zm = zm_n;
-        zp = zp_n;
-      }
+      cur = n1;
+      if (PF == 2) n1 = n2;
     }
     for (uint32_t s = 0; s < n_parked; ++s) {
       const TiedRow t = parked[s][threadIdx.x];
-      const uint64_t j0 = (plane + y0 + (t.meta & 0xFu)) * g.lx2 + i0;
+      const Off j0 = (plane + y0 + (t.meta & 0xFu)) * lx2 + i0;
       settle_row<ND, MEAS>(own, j0, t, sweep, colour, P, acc0, acc1);
     }
   }
@@ -287,22 +312,32 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
 }
 
 uint32_t fast_rows(const StencilGeom &g) {
-  for (uint32_t r : {16u, 8u, 4u, 2u, 1u})
+  // ly is even (the checkerboard needs even sides), so 2 always divides it
+  for (uint32_t r : {16u, 8u, 4u})
     if (r <= kMaxRows && g.ly % r == 0) return r;
-  return 1;
+  return 2;
 }
 
 template <int ND, bool MEAS>
 void launch_rows(uint32_t R, dim3 grid, hipStream_t st, int8_t *spins, const ReplicaParams *reps,
                  const StencilGeom &g, uint32_t colour, uint64_t sweep, long long *em) {
-#define SC_ROWS(RR) \
-  hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, em)
+  // lanes hold consecutive chunks of one row when a row is whole waves wide
+  const bool nbr = (g.lx2 / 16) % 64 == 0;
+#define SC_ROWS(RR)                                                                                        \
+  if (g.nh >= (1ull << 32))                                                                                \
+    hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR, uint64_t, kPrefetch, false>), grid, dim3(kBlock), 0, st, \
+                       spins, reps, g, colour, sweep, em);                                                 \
+  else if (nbr)                                                                                            \
+    hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR, uint32_t, kPrefetch, true>), grid, dim3(kBlock), 0, st,  \
+                       spins, reps, g, colour, sweep, em);                                                 \
+  else                                                                                                     \
+    hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR, uint32_t, kPrefetch, false>), grid, dim3(kBlock), 0, st, \
+                       spins, reps, g, colour, sweep, em)
   switch (R) {
     case 16: SC_ROWS(16); break;
     case 8: SC_ROWS(8); break;
     case 4: SC_ROWS(4); break;
-    case 2: SC_ROWS(2); break;
-    default: SC_ROWS(1); break;
+    default: SC_ROWS(2); break;
   }
 #undef SC_ROWS
 }

@@ -94,11 +94,15 @@ def test_ising2d_l16384_bit_exact(sc, orc):
     st.close()
 
 
-@pytest.mark.parametrize("dims,n_rep,sweeps", [((2048, 2048), 16, 3), ((128, 128, 128), 4, 3)])
+@pytest.mark.parametrize("dims,n_rep,sweeps", [((2048, 2048), 16, 3), ((128, 128, 128), 4, 3),
+                                               ((2048, 32, 8), 3, 3), ((1024, 64), 2, 4)])
 def test_remapped_grid_many_replicas(sc, orc, dims, n_rep, sweeps):
-    """A grid of 32 (2-D) or 16 (3-D) blocks per replica, so the XCD-remapped
-    block order is in use, with replicas at temperatures from the disordered
-    phase to deep in the ordered one."""
+    """Replicas at temperatures from the disordered phase to deep in the
+    ordered one.  2048^2 (32 blocks per replica) and 128^3 (16) use the
+    XCD-remapped block order.  Rows of Lx = 2048 sites are 1024 colour bytes,
+    64 lanes of 16 bytes, so the horizontal edge comes from the neighbouring
+    lane (2048^2 and 2048 x 32 x 8); Lx = 1024 rows are half a wave wide and
+    load it."""
     seed = 77
     print(f"case dims={dims} replicas={n_rep} sweeps={sweeps}", flush=True)
     lat = sc.Lattice.hypercubic(list(dims), True)
