@@ -121,6 +121,32 @@ int scamper_state_upload_i8(scamper_state *st, uint32_t replica, const int8_t *s
 int scamper_state_download_i8(scamper_state *st, uint32_t replica, int8_t *spins);
 int scamper_state_upload_f32(scamper_state *st, uint32_t replica, const float *spins);
 int scamper_state_download_f32(scamper_state *st, uint32_t replica, float *spins);
+/* Parallel tempering over the state's replicas (ParallelTemperingCompatible
+ * for ClassicalMC, CMC.rs/src/classical_mc.rs:203-229, driven by
+ * PtExchange::try_exchange, Carlo.rs/src/parallel_tempering.rs:362-419).
+ * scamper_state_set_ladder gives temperature label l the inverse
+ * temperature beta_by_label[l] (n_replicas values, at most 1024 replicas)
+ * and replica r label r.  scamper_state_exchange queues one exchange round
+ * on the state's stream: the exact energy of every replica, then pairs of
+ * adjacent labels with offset (step-1)&1, w = (beta_self - beta_partner)
+ * E_self on each side, the lower label deciding with
+ * accept_log_probability(w_lower + w_upper, u) on a counter-based draw of
+ * (seed, lower label, step); accepted pairs swap labels and each replica
+ * takes its new label's beta (change_parameter).  Labels stay on the device
+ * between rounds, so rounds and sweeps queue without host synchronisation.
+ * scamper_state_set_beta ends tempering (every replica keeps the beta of
+ * the label it holds). */
+int scamper_state_set_ladder(scamper_state *st, const double *beta_by_label);
+int scamper_state_exchange(scamper_state *st, uint64_t step);
+/* Label held by each replica (n_replicas values). */
+int scamper_state_labels(scamper_state *st, int32_t *labels);
+/* Energies of every replica used by the last exchange round. */
+int scamper_state_exchange_energies(scamper_state *st, double *energy);
+/* Accepted exchanges, decisions whose ln(u) fell within 1e-13 relative of
+ * the log ratio (where the device and host libm could disagree), and
+ * exchange rounds so far (since the ladder was set). */
+int scamper_state_exchange_stats(scamper_state *st, uint64_t *accepted, uint64_t *near_ties, uint64_t *rounds);
+
 /* Sweeps done so far (the counter in every Philox draw). */
 int scamper_state_sweeps(const scamper_state *st, uint64_t *sweeps);
 /* Set that counter: the whole random-number state of a replica is (seed,

@@ -95,6 +95,11 @@ SIGNATURES = {
     "scamper_state_download_f32": (i32, [vp, u32, vp]),
     "scamper_state_sweeps": (i32, [vp, vp]),
     "scamper_state_set_sweeps": (i32, [vp, u64]),
+    "scamper_state_set_ladder": (i32, [vp, vp]),
+    "scamper_state_exchange": (i32, [vp, u64]),
+    "scamper_state_labels": (i32, [vp, vp]),
+    "scamper_state_exchange_energies": (i32, [vp, vp]),
+    "scamper_state_exchange_stats": (i32, [vp, vp, vp, vp]),
     "scamper_sweep": (i32, [vp, i32, u64, i32]),
     "scamper_synchronize": (i32, [vp]),
     "scamper_sweep_timed": (i32, [vp, i32, u64, i32, vp]),
@@ -316,6 +321,34 @@ class State:
 
     def set_sweeps(self, sweeps: int):
         check(lib().scamper_state_set_sweeps(self.h, int(sweeps)))
+
+    def set_ladder(self, betas):
+        """Temperature label l gets beta betas[l]; replica r holds label r
+        (the parameter ladder of parallel tempering)."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        if b.size != self.n_replicas:
+            raise ValueError(f"expected {self.n_replicas} betas, got {b.size}")
+        check(lib().scamper_state_set_ladder(self.h, _p(b)))
+
+    def exchange(self, step: int):
+        """One replica-exchange round (PtExchange::try_exchange,
+        Carlo.rs/src/parallel_tempering.rs:362-419), queued on the device."""
+        check(lib().scamper_state_exchange(self.h, int(step)))
+
+    def labels(self) -> np.ndarray:
+        out = np.zeros(self.n_replicas, np.int32)
+        check(lib().scamper_state_labels(self.h, _p(out)))
+        return out
+
+    def exchange_energies(self) -> np.ndarray:
+        out = np.zeros(self.n_replicas, np.float64)
+        check(lib().scamper_state_exchange_energies(self.h, _p(out)))
+        return out
+
+    def exchange_stats(self):
+        a, t, r = u64(), u64(), u64()
+        check(lib().scamper_state_exchange_stats(self.h, C.byref(a), C.byref(t), C.byref(r)))
+        return a.value, t.value, r.value
 
     def get_beta(self, replica: int = 0) -> float:
         b = dbl()

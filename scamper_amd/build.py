@@ -3,9 +3,9 @@
     python -m scamper_amd.build        (or scamper_amd.build.build())
 
 Every HIP source is compiled with packed-FP32 instructions disabled; the
-threshold tables and the cluster kernels (f64 O(3) bond probabilities and
-reflections) are compiled with -ffp-contract=off so their f64 arithmetic
-rounds like the reference's.
+threshold tables, the cluster kernels (f64 O(3) bond probabilities and
+reflections) and the exchange energies are compiled with -ffp-contract=off
+so their f64 arithmetic rounds like the reference's.
 """
 from __future__ import annotations
 
@@ -33,6 +33,7 @@ PER_FILE = {
     "pool.cpp": ["-ffp-contract=off"],
     "k_cluster.hip": ["-ffp-contract=off"],
     "k_csr_real.hip": ["-ffp-contract=off"],
+    "k_exchange.hip": ["-ffp-contract=off"],
 }
 
 

@@ -117,6 +117,20 @@ struct scamper_state {
   std::vector<uint64_t> cthr;  // host copy of d_cthr, valid while cthr_ready
   bool cthr_ready = false;
   int device = 0;
+  // parallel tempering: beta by temperature label (empty: no ladder), the
+  // label each replica holds (on the device; the host copy and beta[] follow
+  // it when labels_dirty is false), per-label decision tables, the energies
+  // of the last exchange round and the counters {accepted, near ties, rounds}
+  std::vector<double> ladder;
+  std::vector<int32_t> labels;
+  bool labels_dirty = false;
+  int32_t *d_labels = nullptr;
+  double *d_ladder = nullptr;
+  double *d_xenergy = nullptr;
+  unsigned long long *d_xstats = nullptr;
+  ReplicaParams *d_lab_params = nullptr;
+  TileJob *d_lab_rows = nullptr;
+  double *d_lab_hbexp = nullptr;
 
   ~scamper_state() {
     if (stream) hipStreamSynchronize(stream);
@@ -143,6 +157,13 @@ struct scamper_state {
     hipFree(d_neighbors);
     hipFree(d_class_sites);
     hipFree(d_weights);
+    hipFree(d_labels);
+    hipFree(d_ladder);
+    hipFree(d_xenergy);
+    hipFree(d_xstats);
+    hipFree(d_lab_params);
+    hipFree(d_lab_rows);
+    hipFree(d_lab_hbexp);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -178,40 +199,75 @@ hipError_t measure_graph(const scamper_state *st, long long *em) {
                                                st->stream);
 }
 
-// Build and upload the per-replica decision tables for `alg`.
+// Copy each replica's decision tables from the label it holds (on the
+// device, in stream order), for the tables uploaded last.
+hipError_t apply_labels(scamper_state *st) {
+  if (st->ladder.empty() || st->tables_for < 0) return hipSuccess;
+  TileJob *rows = nullptr;
+  if (uses_tiles(st, st->tables_for) && st->d_tiles && st->d_lab_rows)
+    rows = reinterpret_cast<TileJob *>(static_cast<TileHeader *>(st->d_tiles) + 1);
+  const bool hb = st->d_lab_hbexp && st->d_hbexp && st->model.kind == kPotts && st->tables_for == kHeatBath &&
+                  !uses_real_kernels(st, kHeatBath);
+  return launch_apply_labels(st->d_labels, st->n_rep, st->d_params, st->d_lab_params, rows,
+                             rows ? st->d_lab_rows : nullptr, hb ? st->d_hbexp : nullptr,
+                             hb ? st->d_lab_hbexp : nullptr, (uint32_t)((kMaxZ + 1) * (kMaxZ + 1)), st->stream);
+}
+
+// Make beta[] and the host copy of the labels follow the device's labels
+// (after exchange rounds).
+int sync_labels(scamper_state *st) {
+  if (st->ladder.empty() || !st->labels_dirty) return SCAMPER_OK;
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(st->labels.data(), st->d_labels, st->n_rep * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (uint32_t r = 0; r < st->n_rep; ++r) st->beta[r] = st->ladder[st->labels[r]];
+  st->labels_dirty = false;
+  return SCAMPER_OK;
+}
+
+// Build and upload the per-replica decision tables for `alg`.  With a
+// temperature ladder the tables are built per label, and the device copies
+// each replica's from the label it holds.
 int upload_tables(scamper_state *st, int alg) {
+  const bool ladder = !st->ladder.empty();
   std::vector<ReplicaParams> p(st->n_rep);
   std::vector<double> hb;
   const int z = st->csr ? st->w_max : st->z;
+  const size_t hb_per = (size_t)(kMaxZ + 1) * (kMaxZ + 1);
   // the f64 graph kernels read only key and beta
   const bool f64_graph = uses_real_kernels(st, alg);
-  if (st->model.kind == kPotts && alg == kHeatBath && !f64_graph)
-    hb.assign((size_t)st->n_rep * (kMaxZ + 1) * (kMaxZ + 1), 0.0);
+  if (st->model.kind == kPotts && alg == kHeatBath && !f64_graph) hb.assign((size_t)st->n_rep * hb_per, 0.0);
+  // tables of replica r, or of label r with a ladder
   for (uint32_t r = 0; r < st->n_rep; ++r) {
+    const double beta = ladder ? st->ladder[r] : st->beta[r];
     memset(&p[r], 0, sizeof(ReplicaParams));
     p[r].key = st->keys[r];
-    p[r].beta = st->beta[r];
+    p[r].beta = beta;
     if (f64_graph) {
     } else if (st->csr && st->model.kind == kIsing) {
-      if (alg == kMetropolis) csr_ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
-      else csr_ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
+      if (alg == kMetropolis) csr_ising_metropolis_table(beta, st->model.j, z, p[r].thr);
+      else csr_ising_heat_bath_table(beta, st->model.j, z, p[r].thr);
     } else if (st->model.kind == kIsing) {
-      if (alg == kMetropolis) ising_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
-      else ising_heat_bath_table(st->beta[r], st->model.j, z, p[r].thr);
+      if (alg == kMetropolis) ising_metropolis_table(beta, st->model.j, z, p[r].thr);
+      else ising_heat_bath_table(beta, st->model.j, z, p[r].thr);
     } else if (st->continuous) {
       // the O(N) kernels read only key and beta
     } else if (st->model.kind == kPotts) {
-      if (alg == kMetropolis) potts_metropolis_table(st->beta[r], st->model.j, z, p[r].thr);
-      else potts_heat_bath_exp(st->beta[r], st->model.j, z, hb.data() + (size_t)r * (kMaxZ + 1) * (kMaxZ + 1));
+      if (alg == kMetropolis) potts_metropolis_table(beta, st->model.j, z, p[r].thr);
+      else potts_heat_bath_exp(beta, st->model.j, z, hb.data() + (size_t)r * hb_per);
     }
   }
   SC_HIP(hipStreamSynchronize(st->stream));
   SC_HIP(hipMemcpy(st->d_params, p.data(), p.size() * sizeof(ReplicaParams), hipMemcpyHostToDevice));
+  if (ladder) {
+    if (!st->d_lab_params) SC_HIP(hipMalloc(&st->d_lab_params, p.size() * sizeof(ReplicaParams)));
+    SC_HIP(hipMemcpy(st->d_lab_params, p.data(), p.size() * sizeof(ReplicaParams), hipMemcpyHostToDevice));
+  }
+  std::vector<TileJob> rows;
   if (uses_tiles(st, alg)) {
     // one row per replica: every sweep of every replica, measured from 0
     st->tile_lane = tile_lane_sites((uint64_t)st->n_rep * st->geom.nh);
     const uint32_t per = (uint32_t)((st->geom.nh + tile_sites(st->tile_lane) - 1) / tile_sites(st->tile_lane));
-    std::vector<TileJob> rows(st->n_rep);
+    rows.resize(st->n_rep);
     for (uint32_t r = 0; r < st->n_rep; ++r) {
       TileJob &t = rows[r];
       memset(&t, 0, sizeof t);
@@ -241,12 +297,21 @@ int upload_tables(scamper_state *st, int alg) {
     SC_HIP(hipMemcpy(static_cast<char *>(st->d_tiles) + sizeof(TileHeader), rows.data(),
                      rows.size() * sizeof(TileJob), hipMemcpyHostToDevice));
     st->tile_grid = std::min<uint32_t>((h.n_tiles + 7u) & ~7u, 2048u);
+    if (ladder) {
+      if (!st->d_lab_rows) SC_HIP(hipMalloc(&st->d_lab_rows, rows.size() * sizeof(TileJob)));
+      SC_HIP(hipMemcpy(st->d_lab_rows, rows.data(), rows.size() * sizeof(TileJob), hipMemcpyHostToDevice));
+    }
   }
   if (!hb.empty()) {
     if (!st->d_hbexp) SC_HIP(hipMalloc(&st->d_hbexp, hb.size() * sizeof(double)));
     SC_HIP(hipMemcpy(st->d_hbexp, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (ladder) {
+      if (!st->d_lab_hbexp) SC_HIP(hipMalloc(&st->d_lab_hbexp, hb.size() * sizeof(double)));
+      SC_HIP(hipMemcpy(st->d_lab_hbexp, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
   }
   st->tables_for = alg;
+  if (ladder) SC_HIP(apply_labels(st));
   return SCAMPER_OK;
 }
 
@@ -642,6 +707,10 @@ void scamper_state_free(scamper_state *st) { delete st; }
 int scamper_state_set_beta(scamper_state *st, uint32_t replica, double beta) {
   if (int rc = check_state(st, replica)) return rc;
   if (!std::isfinite(beta) || beta < 0.0) return fail(SCAMPER_EINVAL, "beta must be finite and non-negative");
+  // a beta set by hand ends tempering: every replica keeps the beta of the
+  // label it holds now
+  if (int rc = sync_labels(st)) return rc;
+  st->ladder.clear();
   st->beta[replica] = beta;
   st->tables_for = -1;
   st->cthr_ready = false;
@@ -651,7 +720,91 @@ int scamper_state_set_beta(scamper_state *st, uint32_t replica, double beta) {
 int scamper_state_get_beta(const scamper_state *st, uint32_t replica, double *beta) {
   if (int rc = check_state(st, replica)) return rc;
   if (!beta) return fail(SCAMPER_EINVAL, "null argument");
+  if (int rc = sync_labels(const_cast<scamper_state *>(st))) return rc;
   *beta = st->beta[replica];
+  return SCAMPER_OK;
+}
+
+int scamper_state_set_ladder(scamper_state *st, const double *beta_by_label) {
+  if (!st || !beta_by_label) return fail(SCAMPER_EINVAL, "null argument");
+  if (st->n_rep > 1024) return fail(SCAMPER_EUNSUPPORTED, "replica exchange runs over at most 1024 replicas");
+  for (uint32_t l = 0; l < st->n_rep; ++l)
+    if (!std::isfinite(beta_by_label[l]) || beta_by_label[l] < 0.0)
+      return fail(SCAMPER_EINVAL, "beta of label %u must be finite and non-negative", l);
+  if (!st->d_labels) {
+    SC_HIP(hipMalloc(&st->d_labels, st->n_rep * sizeof(int32_t)));
+    SC_HIP(hipMalloc(&st->d_ladder, st->n_rep * sizeof(double)));
+    SC_HIP(hipMalloc(&st->d_xenergy, st->n_rep * sizeof(double)));
+    SC_HIP(hipMalloc(&st->d_xstats, 4 * sizeof(unsigned long long)));
+  }
+  st->ladder.assign(beta_by_label, beta_by_label + st->n_rep);
+  st->labels.resize(st->n_rep);
+  for (uint32_t r = 0; r < st->n_rep; ++r) {
+    st->labels[r] = (int32_t)r;
+    st->beta[r] = st->ladder[r];
+  }
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(st->d_labels, st->labels.data(), st->n_rep * sizeof(int32_t), hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(st->d_ladder, st->ladder.data(), st->n_rep * sizeof(double), hipMemcpyHostToDevice));
+  SC_HIP(hipMemset(st->d_xenergy, 0, st->n_rep * sizeof(double)));
+  SC_HIP(hipMemset(st->d_xstats, 0, 4 * sizeof(unsigned long long)));
+  st->labels_dirty = false;
+  st->tables_for = -1;
+  st->cthr_ready = false;
+  return SCAMPER_OK;
+}
+
+int scamper_state_exchange(scamper_state *st, uint64_t step) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  if (st->ladder.empty()) return fail(SCAMPER_EINVAL, "no temperature ladder: call scamper_state_set_ladder first");
+  // exact energies of the current spins (change_parameter's recompute)
+  hipError_t e = launch_em_zero(st->d_em, (uint64_t)kEmStride * st->n_rep, st->stream);
+  if (e == hipSuccess)
+    e = st->continuous ? measure_on(st, reinterpret_cast<double *>(st->d_em))
+        : st->csr      ? measure_graph(st, st->d_em)
+                       : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep,
+                                                st->d_em, st->stream);
+  EnergyForm form{};
+  form.j = st->model.j;
+  form.self_loops = st->self_loop_weight;
+  form.kind = st->continuous ? (st->csr ? kEnergyVecGraph : kEnergyVecStencil)
+              : st->csr      ? (st->real_weights ? kEnergyGraphReal : kEnergyGraph)
+                             : kEnergyStencil;
+  if (e == hipSuccess)
+    e = launch_state_exchange(st->d_labels, st->d_em, st->n_rep, form, st->d_ladder, stream_seed(st->seed, 0, 3), step,
+                              st->d_xenergy, st->d_xstats, st->stream);
+  if (e == hipSuccess) e = apply_labels(st);
+  if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "exchange: %s", hipGetErrorString(e));
+  st->labels_dirty = true;
+  st->cthr_ready = false;
+  return SCAMPER_OK;
+}
+
+int scamper_state_labels(scamper_state *st, int32_t *labels) {
+  if (!st || !labels) return fail(SCAMPER_EINVAL, "null argument");
+  if (st->ladder.empty()) return fail(SCAMPER_EINVAL, "no temperature ladder");
+  if (int rc = sync_labels(st)) return rc;
+  std::copy(st->labels.begin(), st->labels.end(), labels);
+  return SCAMPER_OK;
+}
+
+int scamper_state_exchange_energies(scamper_state *st, double *energy) {
+  if (!st || !energy) return fail(SCAMPER_EINVAL, "null argument");
+  if (st->ladder.empty()) return fail(SCAMPER_EINVAL, "no temperature ladder");
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(energy, st->d_xenergy, st->n_rep * sizeof(double), hipMemcpyDeviceToHost));
+  return SCAMPER_OK;
+}
+
+int scamper_state_exchange_stats(scamper_state *st, uint64_t *accepted, uint64_t *near_ties, uint64_t *rounds) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  if (st->ladder.empty()) return fail(SCAMPER_EINVAL, "no temperature ladder");
+  unsigned long long v[4];
+  SC_HIP(hipStreamSynchronize(st->stream));
+  SC_HIP(hipMemcpy(v, st->d_xstats, sizeof v, hipMemcpyDeviceToHost));
+  if (accepted) *accepted = v[0];
+  if (near_ties) *near_ties = v[1];
+  if (rounds) *rounds = v[2];
   return SCAMPER_OK;
 }
 
@@ -852,6 +1005,7 @@ int ensure_cluster(scamper_state *st) {
 // cluster updates; the thresholds are rebuilt after a beta changes.
 int prepare_cluster(scamper_state *st) {
   if (int rc = ensure_cluster(st)) return rc;
+  if (int rc = sync_labels(st)) return rc;
   if (st->cthr_ready) return SCAMPER_OK;
   st->cthr.assign((size_t)st->n_rep * kClusterWeights, 0);
   for (uint32_t r = 0; r < st->n_rep; ++r)
@@ -892,8 +1046,10 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
   } else if (st->model.j < 0.0) {
     return fail(SCAMPER_EUNSUPPORTED, "cluster updates require J * bond.weight >= 0");
   }
-  // O(N) bonds depend on the spins: no threshold tables
+  // O(N) bonds depend on the spins: no threshold tables (their beta is
+  // passed per launch)
   if (int rc = st->continuous ? ensure_cluster(st) : prepare_cluster(st)) return rc;
+  if (int rc = sync_labels(st)) return rc;
   const std::vector<uint64_t> &thr = st->cthr;
   if (measure) {
     if (int rc = ensure_series(st, n_sweeps)) return rc;

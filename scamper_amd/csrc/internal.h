@@ -276,6 +276,35 @@ hipError_t launch_cluster_on_csr(uint64_t n_sites, uint64_t n_edges, const Clust
                                  int dim, float *planes, uint32_t *parent, uint64_t key, uint64_t step, double beta,
                                  double j, int wolff, void *scratch, hipStream_t stream);
 
+// ------------------------------------------------------------ replica exchange
+// How a replica's energy follows from its recount counters (the expressions
+// of the host's observables): stencil and graph Ising / Potts (integer bond
+// sums), real-weight graphs (an f64 bond sum, every bond twice), O(N) on
+// stencils and graphs (f64 sums of s.h).
+enum EnergyKind : int32_t {
+  kEnergyStencil = 0,
+  kEnergyGraph = 1,
+  kEnergyGraphReal = 2,
+  kEnergyVecStencil = 3,
+  kEnergyVecGraph = 4,
+};
+struct EnergyForm {
+  int32_t kind;
+  int32_t pad;
+  double j;
+  double self_loops;  // sum of the self-loop weights
+};
+// One exchange round over the labels of n <= 1024 replicas from the
+// counters em[r * kEmStride ...]; stats = {accepted, near ties, rounds}.
+hipError_t launch_state_exchange(int32_t *labels, const long long *em, uint32_t n, const EnergyForm &form,
+                                 const double *ladder, uint64_t ex_key, uint64_t step, double *energy_out,
+                                 unsigned long long *stats, hipStream_t stream);
+// Copy the decision tables of the label each replica holds into its own
+// parameters (and tiled rows / heat-bath weights when given).
+hipError_t launch_apply_labels(const int32_t *labels, uint32_t n, ReplicaParams *params,
+                               const ReplicaParams *label_params, TileJob *rows, const TileJob *label_rows,
+                               double *hbexp, const double *label_hbexp, uint32_t hb_per, hipStream_t stream);
+
 // Register-blocked Ising Metropolis for 2-D/3-D with Lx/2 a multiple of 16.
 bool ising_fast_path_ok(const StencilGeom &g);
 hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,
