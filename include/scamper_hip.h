@@ -347,6 +347,26 @@ int scamper_pool_result(const scamper_pool *pool, uint64_t job_id, scamper_estim
 int scamper_pool_release(scamper_pool *pool, uint64_t job_id);
 int scamper_pool_stats_get(const scamper_pool *pool, scamper_pool_stats *out);
 
+/* ------------------------------------------------- no device allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): scamper_sweep's Ising Metropolis sweep (periodic 2-D or
+ * 3-D lattice, Lx a multiple of 32, even sides) of n_rep <= 64 replicas held
+ * in a caller-owned device buffer, d_spins: replica r's colour 0 then colour
+ * 1 at d_spins + r * N, each colour N/2 bytes of +1/-1 in rows of Lx/2
+ * (row y + Ly z, element i at x = 2i + ((c + y + z) & 1)), the layout a
+ * state keeps.  Replica r draws from the key of replica replica0 + r of
+ * `seed` and sweeps number sweep0 .. sweep0 + n_sweeps - 1, so the result
+ * equals scamper_sweep on a state of that seed, bit for bit.  When energy or
+ * magnetization is not NULL the last sweep is measured (n_rep values each).
+ * The decision tables and counters live in one fixed library-owned device
+ * array (about 44 KB per GPU, kept until the process ends) instead of a
+ * state's buffers: calls that overlap on one GPU race.  Runs on `stream` (a
+ * hipStream_t, NULL for the default stream) and returns when it is done. */
+int scamper_ising_sweep_noalloc(int8_t *d_spins, const uint64_t *dims, int n_dims, double j, const double *beta,
+                                uint32_t n_rep, uint64_t seed, uint32_t replica0, uint64_t sweep0,
+                                uint64_t n_sweeps, double *energy, double *magnetization, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

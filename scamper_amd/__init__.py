@@ -96,6 +96,7 @@ SIGNATURES = {
     "scamper_state_sweeps": (i32, [vp, vp]),
     "scamper_state_set_sweeps": (i32, [vp, u64]),
     "scamper_state_set_ladder": (i32, [vp, vp]),
+    "scamper_ising_sweep_noalloc": (i32, [vp, vp, i32, dbl, vp, u32, u64, u32, u64, u64, vp, vp, vp]),
     "scamper_state_exchange": (i32, [vp, u64]),
     "scamper_state_labels": (i32, [vp, vp]),
     "scamper_state_exchange_energies": (i32, [vp, vp]),
