@@ -149,6 +149,14 @@ def lib() -> C.CDLL:
     return _lib
 
 
+def _sized(a: np.ndarray, n: int, what: str) -> np.ndarray:
+    """a, after checking it holds exactly n values: the library reads n values
+    from the pointer, so a shorter array must never reach it."""
+    if a.size != n:
+        raise ValueError(f"expected {n} {what}, got {a.size}")
+    return a
+
+
 def last_error() -> str:
     buf = C.create_string_buffer(1024)
     lib().scamper_last_error(buf, 1024)
@@ -280,7 +288,7 @@ class State:
         check(lib().scamper_state_init(self.h, replica, int(random)))
 
     def upload_i8(self, spins, replica: int = 0):
-        a = np.ascontiguousarray(spins, dtype=np.int8)
+        a = _sized(np.ascontiguousarray(spins, dtype=np.int8).ravel(), self.lattice.n_sites, "spins")
         check(lib().scamper_state_upload_i8(self.h, replica, _p(a)))
 
     def download_i8(self, replica: int = 0) -> np.ndarray:
@@ -442,15 +450,15 @@ class MultiSpin:
     __del__ = close
 
     def set_betas(self, betas):
-        b = np.ascontiguousarray(betas, dtype=np.float64)
-        assert b.size == self.n
+        b = _sized(np.ascontiguousarray(betas, dtype=np.float64).ravel(), self.n, "betas")
         check(lib().scamper_multispin_set_betas(self.h, _p(b)))
 
     def init(self, random: bool):
         check(lib().scamper_multispin_init(self.h, int(random)))
 
     def upload(self, words):
-        w = np.ascontiguousarray(words, dtype=np.uint64)
+        w = _sized(np.ascontiguousarray(words, dtype=np.uint64).ravel(), self.n_bundles * self.lattice.n_sites,
+                   "site words")
         check(lib().scamper_multispin_upload(self.h, _p(w)))
 
     def download(self) -> np.ndarray:

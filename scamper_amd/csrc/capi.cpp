@@ -983,7 +983,9 @@ int ensure_cluster(scamper_state *st) {
     std::vector<double> cwf(e.size());
     for (size_t k = 0; k < e.size(); ++k) {
       ce[k] = ClusterEdge{(uint32_t)e[k].source, (uint32_t)e[k].target};
-      cw[k] = (int8_t)e[k].weight;  // integer weights in [-127, 127] on the discrete graph path
+      // integer weights in [-127, 127] on the discrete graph path; graphs
+      // with real weights read per-edge thresholds instead
+      cw[k] = st->real_weights ? (int8_t)0 : (int8_t)e[k].weight;
       cwf[k] = e[k].weight;         // any weight for O(N)
     }
     if (st->continuous) {
@@ -1163,7 +1165,8 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
                                 st->model.kind, st->model.q, algorithm, st->d_spins, st->d_params, st->n_rep, sweep,
                                 st->model.j, st->stream)
               : launch_csr_discrete(st->cgeom, st->class_start.data(), st->model.kind, st->model.q, algorithm,
-                                    st->d_spins, st->d_params, st->n_rep, sweep, st->d_hbexp, st->stream);
+                                    st->d_spins, st->d_params, st->n_rep, sweep, st->d_hbexp, st->model.j < 0.0,
+                                    st->stream);
       if (e == hipSuccess && measure)
         e = measure_graph(st, st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride);
       if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
@@ -1189,7 +1192,8 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
     for (uint32_t c = 0; c < 2; ++c) {
       long long *em = (measure && c == 1) ? st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride : nullptr;
       hipError_t e = launch_stencil_discrete(st->geom, st->model.kind, st->model.q, algorithm, st->d_spins,
-                                             st->d_params, st->n_rep, c, sweep, st->d_hbexp, em, st->stream);
+                                             st->d_params, st->n_rep, c, sweep, st->d_hbexp, st->model.j < 0.0, em,
+                                             st->stream);
       if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
     }
   }

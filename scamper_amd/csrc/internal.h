@@ -28,7 +28,9 @@ constexpr int kMaxQ = 16;          // largest Potts q on the GPU path
 
 // Per-replica device parameters.  `thr` is indexed by the kernel's local
 // energy class (see each kernel); `hb_exp` holds the heat-bath Potts weights
-// exp(lw(n) - lw(nmax)) indexed [n * (z+1) + nmax].
+// exp(lw(n) - lw(m)) indexed [n * (z+1) + m], where the kernels take m as
+// the count of largest log weight: the largest count when J > 0, the
+// smallest when J < 0 (hb_min).
 struct alignas(16) ReplicaParams {
   uint64_t key;
   double beta;
@@ -92,7 +94,7 @@ constexpr int kEmStride = 2 + kMaxQ;
 // ------------------------------------------------------------ launchers
 hipError_t launch_stencil_discrete(const StencilGeom &g, int model_kind, int q, int algorithm,
                                    int8_t *spins, const ReplicaParams *reps, uint32_t n_rep,
-                                   uint32_t colour, uint64_t sweep, const double *hb_exp,
+                                   uint32_t colour, uint64_t sweep, const double *hb_exp, int hb_min,
                                    long long *em, hipStream_t stream);
 hipError_t launch_stencil_measure(const StencilGeom &g, int model_kind, int q,
                                   const int8_t *spins, uint32_t n_rep, long long *em,
@@ -101,7 +103,7 @@ hipError_t launch_stencil_init(const StencilGeom &g, int model_kind, int q, int8
                                uint64_t key, uint32_t rep, int random, hipStream_t stream);
 hipError_t launch_csr_discrete(const CsrGeom &g, const uint64_t *class_start_host, int kind, int q,
                                int alg, int8_t *spins, const ReplicaParams *reps, uint32_t n_rep,
-                               uint64_t sweep, const double *hb_exp, hipStream_t stream);
+                               uint64_t sweep, const double *hb_exp, int hb_min, hipStream_t stream);
 hipError_t launch_csr_measure(const CsrGeom &g, int kind, int q, const int8_t *spins, uint32_t n_rep,
                               long long *em, hipStream_t stream);
 // Ising / Potts on a graph lattice with real weights: every incidence in CSR

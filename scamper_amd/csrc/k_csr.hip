@@ -36,7 +36,7 @@ template <int KIND, int ALG>
 __global__ __launch_bounds__(kBlock) void k_csr_site(int8_t *__restrict__ spins, const ReplicaParams *__restrict__ reps,
                                                      CsrGeom g, uint32_t colour, uint64_t class_start,
                                                      uint64_t class_size, uint64_t sweep, int q,
-                                                     const double *__restrict__ hb_exp) {
+                                                     const double *__restrict__ hb_exp, int hb_min) {
   const uint32_t rep = blockIdx.y;
   const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const ReplicaParams &P = reps[rep];
@@ -98,8 +98,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_site(int8_t *__restrict__ spins,
       int cnt[kMaxQ];
       for (int v = 0; v < q; ++v) cnt[v] = 0;
       for (uint32_t b = b0; b < b1; ++b) cnt[(uint8_t)s_all[g.neighbors[b]]] += g.weights[b];
-      int nmax = 0;
-      for (int v = 0; v < q; ++v) nmax = cnt[v] > nmax ? cnt[v] : nmax;
+      // the log weight of the largest count is max_log when beta J > 0, of
+      // the smallest when J < 0 (hb_min); unit weights only on this path
+      int nmax = hb_min ? (int)(b1 - b0) : 0;
+      for (int v = 0; v < q; ++v) nmax = (hb_min ? cnt[v] < nmax : cnt[v] > nmax) ? cnt[v] : nmax;
       const double *E = hb_exp + (uint64_t)rep * (kMaxZ + 1) * (kMaxZ + 1);
       double total = 0.0;
       for (int v = 0; v < q; ++v) total = __dadd_rn(total, E[cnt[v] * (W + 1) + nmax]);
@@ -174,13 +176,13 @@ __global__ __launch_bounds__(kBlock) void k_csr_init(int8_t *__restrict__ spins,
 
 hipError_t launch_csr_discrete(const CsrGeom &g, const uint64_t *class_start_host, int kind, int q,
                                int alg, int8_t *spins, const ReplicaParams *reps, uint32_t n_rep,
-                               uint64_t sweep, const double *hb_exp, hipStream_t st) {
+                               uint64_t sweep, const double *hb_exp, int hb_min, hipStream_t st) {
   for (uint32_t c = 0; c < g.n_colours; ++c) {
     const uint64_t start = class_start_host[c], size = class_start_host[c + 1] - start;
     if (size == 0) continue;
     dim3 grid((unsigned)((size + kBlock - 1) / kBlock), n_rep);
 #define SC_CSR(K, A) \
-  hipLaunchKernelGGL((k_csr_site<K, A>), grid, dim3(kBlock), 0, st, spins, reps, g, c, start, size, sweep, q, hb_exp)
+  hipLaunchKernelGGL((k_csr_site<K, A>), grid, dim3(kBlock), 0, st, spins, reps, g, c, start, size, sweep, q, hb_exp, hb_min)
     if (kind == kIsing) {
       if (alg == kMetropolis) SC_CSR(kIsing, kMetropolis); else SC_CSR(kIsing, kHeatBath);
     } else {

@@ -100,7 +100,7 @@ template <int ND, int KIND, int ALG, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_site(int8_t *__restrict__ spins,
                                                  const ReplicaParams *__restrict__ reps,
                                                  StencilGeom g, uint32_t colour, uint64_t sweep,
-                                                 int q, const double *__restrict__ hb_exp,
+                                                 int q, const double *__restrict__ hb_exp, int hb_min,
                                                  long long *__restrict__ em) {
   constexpr int z = 2 * ND;
   const uint32_t rep = blockIdx.y;
@@ -162,14 +162,16 @@ __global__ __launch_bounds__(kBlock) void k_site(int8_t *__restrict__ spins,
         if (accept) ns = (int8_t)r;
       } else {
 #pragma clang fp contract(off)
-        // heat bath (models.rs:340-380) in f64: weights from the host's exp
-        // table, then the reference's sequential threshold subtraction.
+        // heat bath (models.rs:340-380) in f64: weights exp(lw(n) - max_log)
+        // from the host's exp table, then the reference's sequential
+        // threshold subtraction.  max_log is the log weight of the largest
+        // count (beta J > 0) or of the smallest (hb_min: J < 0).
         int cnt[kMaxQ];
         for (int v = 0; v < q; ++v) cnt[v] = 0;
 #pragma unroll
         for (int k = 0; k < z; ++k) cnt[(uint8_t)n.v[k]] += 1;
-        int nmax = 0;
-        for (int v = 0; v < q; ++v) nmax = cnt[v] > nmax ? cnt[v] : nmax;
+        int nmax = hb_min ? z : 0;
+        for (int v = 0; v < q; ++v) nmax = (hb_min ? cnt[v] < nmax : cnt[v] > nmax) ? cnt[v] : nmax;
         double total = 0.0;
         for (int v = 0; v < q; ++v) total = __dadd_rn(total, hb_exp[rep * (kMaxZ + 1) * (kMaxZ + 1) + cnt[v] * (z + 1) + nmax]);
         const uint32_t w = site_word(key, j, sweep, colour, kStreamAccept);
@@ -265,11 +267,11 @@ __global__ __launch_bounds__(kBlock) void k_init(int8_t *__restrict__ spins, Ste
 template <int ND>
 hipError_t dispatch_site(const StencilGeom &g, int kind, int q, int alg, int8_t *spins,
                          const ReplicaParams *reps, uint32_t n_rep, uint32_t colour, uint64_t sweep,
-                         const double *hb_exp, long long *em, hipStream_t st) {
+                         const double *hb_exp, int hb_min, long long *em, hipStream_t st) {
   dim3 grid(capped_blocks(g.nh), n_rep);
   const bool meas = em != nullptr;
 #define SC_LAUNCH(K, A, M) \
-  hipLaunchKernelGGL((k_site<ND, K, A, M>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, q, hb_exp, em)
+  hipLaunchKernelGGL((k_site<ND, K, A, M>), grid, dim3(kBlock), 0, st, spins, reps, g, colour, sweep, q, hb_exp, hb_min, em)
   if (kind == kIsing) {
     if (alg == kMetropolis) {
       if (meas) SC_LAUNCH(kIsing, kMetropolis, true); else SC_LAUNCH(kIsing, kMetropolis, false);
@@ -291,14 +293,14 @@ hipError_t dispatch_site(const StencilGeom &g, int kind, int q, int alg, int8_t 
 
 hipError_t launch_stencil_discrete(const StencilGeom &g, int kind, int q, int alg, int8_t *spins,
                                    const ReplicaParams *reps, uint32_t n_rep, uint32_t colour,
-                                   uint64_t sweep, const double *hb_exp, long long *em,
+                                   uint64_t sweep, const double *hb_exp, int hb_min, long long *em,
                                    hipStream_t st) {
   if (kind == kIsing && alg == kMetropolis && ising_fast_path_ok(g))
     return launch_ising_fast(g, spins, reps, n_rep, colour, sweep, em, st);
   switch (g.nd) {
-    case 1: return dispatch_site<1>(g, kind, q, alg, spins, reps, n_rep, colour, sweep, hb_exp, em, st);
-    case 2: return dispatch_site<2>(g, kind, q, alg, spins, reps, n_rep, colour, sweep, hb_exp, em, st);
-    case 3: return dispatch_site<3>(g, kind, q, alg, spins, reps, n_rep, colour, sweep, hb_exp, em, st);
+    case 1: return dispatch_site<1>(g, kind, q, alg, spins, reps, n_rep, colour, sweep, hb_exp, hb_min, em, st);
+    case 2: return dispatch_site<2>(g, kind, q, alg, spins, reps, n_rep, colour, sweep, hb_exp, hb_min, em, st);
+    case 3: return dispatch_site<3>(g, kind, q, alg, spins, reps, n_rep, colour, sweep, hb_exp, hb_min, em, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -24,10 +24,13 @@ read from the document: the device state computes it from the spins.
 The random-number state of a device replica is (seed, replica, sweeps), so
 the document carries it under one extra key, "scamper": {"seed",
 "replica", "sweeps"}.  The reference's loader reads only its own keys.
-Loading restores the sweep counter.  A state created with the same seed
-and loaded into the same replica index then continues the original
-trajectory bit for bit, as the reference's split-run checkpoint test
-requires (CMC.rs/tests/integration/checkpoint.rs:27-73, 154-193).
+Loading into a state of the same seed and the same replica index restores
+the sweep counter, and the replica then continues the original trajectory
+bit for bit, as the reference's split-run checkpoint test requires
+(CMC.rs/tests/integration/checkpoint.rs:27-73, 154-193).  Under another
+seed or replica index the counter is left alone (with a warning).  The
+replicas of one state share the counter, so a load that would move it for
+the other replicas of a state that has swept is refused.
 
 O(N) spins live on the device as FP32 unit vectors, with norms within ~1e-7
 of 1.  The reference accepts an O(N) spin only when |norm^2 - 1| <= 1e-10 D
@@ -41,6 +44,7 @@ from __future__ import annotations
 
 import math
 import struct
+import warnings
 
 import numpy as np
 
@@ -59,6 +63,32 @@ class CheckpointCorrupted(ScamperError):
 
 def _spin_dim(state) -> int:
     return max(1, int(state.model.spin_dim))
+
+
+def _sweep_counter(state, ext, replica: int):
+    """The sweep counter to restore from the document's "scamper" record, or
+    None to leave the state's counter alone.
+
+    The counter is restored only for the same (seed, replica): under another
+    seed or replica index the document's counter names no position in this
+    replica's stream, and the spins and beta load as the reference's loader
+    loads them.  A state of several replicas shares one counter, so changing
+    it would rewind or advance the other replicas onto draws they have
+    already used: that is refused unless no replica has swept yet (a fresh
+    state) or the counter already has the document's value."""
+    if not isinstance(ext, dict) or _as_u64(ext.get("sweeps")) is None:
+        return None
+    sweeps = _as_u64(ext["sweeps"])
+    seed, rep = _as_u64(ext.get("seed")), _as_u64(ext.get("replica"))
+    if (seed is not None and seed != int(state.seed)) or (rep is not None and rep != int(replica)):
+        warnings.warn("snapshot from another seed or replica index: its sweep counter is not restored, "
+                      "so the continuation draws from this replica's own stream")
+        return None
+    if getattr(state, "n_replicas", 1) > 1 and state.sweeps not in (0, sweeps):
+        raise ValueError(f"the state's {state.n_replicas} replicas share one sweep counter ({state.sweeps}); "
+                         f"setting it to the snapshot's {sweeps} would move the other replicas onto draws "
+                         "they have used: load into a fresh state, or from snapshots with the same sweep count")
+    return sweeps
 
 
 def save_snapshot(state, replica: int = 0, topology: bool = True) -> dict:
@@ -197,6 +227,7 @@ def load_snapshot(state, snapshot: dict, replica: int = 0) -> None:
             exact = np.asarray(f32, dtype=np.float64)
             if not np.all(np.isfinite(exact)) or np.max(np.abs(exact - spins), initial=0.0) > 1e-6:
                 raise CheckpointCorrupted("snapshot.scamper.spins_f32", "does not match snapshot.spins")
+    sweeps = _sweep_counter(state, ext, replica)
     try:  # validate_configuration on upload; nothing is written when it fails
         if exact is not None:
             state.upload_f32(exact.astype(np.float32), replica)
@@ -205,5 +236,5 @@ def load_snapshot(state, snapshot: dict, replica: int = 0) -> None:
     except ScamperError as err:
         raise CheckpointCorrupted("snapshot.spins", str(err)) from None
     state.set_beta(beta, replica)
-    if isinstance(ext, dict) and _as_u64(ext.get("sweeps")) is not None:
-        state.set_sweeps(ext["sweeps"])
+    if sweeps is not None:
+        state.set_sweeps(sweeps)

@@ -102,3 +102,13 @@ def test_invalid_edges_rejected(sc):
         sc.Lattice.from_edges(2, [0], [1], [float("inf")])
     with pytest.raises(sc.ScamperError):
         sc.Lattice.from_edges(0, [], [])
+
+
+def test_short_arrays_never_reach_the_library(sc):
+    """Uploads hand the library a pointer it reads a fixed count from; the
+    bindings refuse arrays of any other length first."""
+    a = np.zeros(7, np.int8)
+    assert sc._sized(a, 7, "spins") is a
+    for n in (6, 8):
+        with pytest.raises(ValueError, match="expected"):
+            sc._sized(a, n, "spins")

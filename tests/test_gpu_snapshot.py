@@ -107,3 +107,39 @@ def test_o3_snapshot_meets_the_reference_norm_rule(sc, n_comp):
     fresh = _state(sc, lat, m, 1.0, 21)
     fresh.load_snapshot(plain)
     assert np.max(np.abs(fresh.download_f32() - f)) < 1e-6 and fresh.get_beta() == 0.7
+
+
+def test_two_replica_state_keeps_its_shared_counter(sc):
+    """The replicas of one state share a sweep counter: a snapshot loads into
+    a fresh two-replica state (the counter moves, no replica has swept), and
+    each replica then continues its own original trajectory; a load that
+    would move the counter of a state that has swept is refused and changes
+    nothing; a document from another seed leaves the counter alone."""
+    lat = sc.Lattice.square(16, 16)
+    m = sc.model(sc.ISING, 1.0)
+    ref = sc.State(lat, m, 2, 7)
+    for r in range(2):
+        ref.init(True, r)
+        ref.set_beta(0.45 + 0.1 * r, r)
+    ref.sweep(sc.METROPOLIS, 20)
+    docs = [ref.save_snapshot(r) for r in range(2)]
+    ref.sweep(sc.METROPOLIS, 15)
+    fresh = sc.State(lat, m, 2, 7)
+    for r in range(2):
+        fresh.load_snapshot(json.loads(json.dumps(docs[r])), r)
+    assert fresh.sweeps == 20
+    fresh.sweep(sc.METROPOLIS, 15)
+    for r in range(2):
+        assert np.array_equal(fresh.download_i8(r), ref.download_i8(r)), f"replica {r}"
+    # a state that has swept: moving its shared counter is refused
+    before = fresh.download_i8(1)
+    with pytest.raises(ValueError, match="share one sweep counter"):
+        fresh.load_snapshot(json.loads(json.dumps(docs[1])), 1)
+    assert np.array_equal(fresh.download_i8(1), before) and fresh.sweeps == 35
+    # another seed: spins and beta load, the counter stays
+    other = sc.State(lat, m, 1, 8)
+    other.sweep(sc.METROPOLIS, 3)
+    with pytest.warns(UserWarning, match="another seed"):
+        other.load_snapshot(json.loads(json.dumps(docs[0])), 0)
+    assert other.sweeps == 3
+    assert np.array_equal(other.download_f64(0), np.asarray(docs[0]["spins"]))

@@ -205,13 +205,15 @@ def test_colour_ordered_sweep_samples_boltzmann(orc, alg):
     assert abs(zs.mean()) < 2 * 4 / math.sqrt(len(zs)) + 0.5
 
 
-def test_potts_heat_bath_samples_boltzmann(orc):
-    # potts_exact.rs: q=3 on a 3x3 torus vs q^N enumeration
+@pytest.mark.parametrize("j", [1.0, -1.0])
+def test_potts_heat_bath_samples_boltzmann(orc, j):
+    # potts_exact.rs: q=3 on a 3x3 torus vs q^N enumeration; J = -1 pins the
+    # antiferromagnetic case, whose max_log is the smallest count's
     L, q, beta = 3, 3, 0.8
     src, dst = _small_square(L)
-    ex = exact.enumerate_potts(src, dst, L * L, q, beta)
+    ex = exact.enumerate_potts(src, dst, L * L, q, beta, j)
     lat = orc.Lattice.square(L, L)
-    m = orc.model(orc.POTTS, 1.0, q)
+    m = orc.model(orc.POTTS, j, q)
     zs = []
     for seed in range(4):
         key = orc.replica_key(100 + seed, 0)
