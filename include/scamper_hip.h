@@ -178,6 +178,11 @@ enum {
  * appended to the device-resident series (ClassicalMC::measure,
  * classical_mc.rs:187-190).  Asynchronous on the state's stream. */
 int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure);
+/* Algorithm::sweep_with_phase with its phase argument (SimulationPhase,
+ * algorithms/common.rs:9-55): 0 = Thermalization, 1 = Measurement.  Only
+ * adaptive proposals act on the phase (allows_adaptation); every algorithm
+ * above is a fixed kernel, so both phases sweep as scamper_sweep does. */
+int scamper_sweep_with_phase(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure, int phase);
 /* Block until the state's queued work is done. */
 int scamper_synchronize(scamper_state *st);
 /* scamper_sweep bracketed by HIP events on the state's stream: *ms is the

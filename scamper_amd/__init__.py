@@ -96,6 +96,7 @@ SIGNATURES = {
     "scamper_state_sweeps": (i32, [vp, vp]),
     "scamper_state_set_sweeps": (i32, [vp, u64]),
     "scamper_state_set_ladder": (i32, [vp, vp]),
+    "scamper_sweep_with_phase": (i32, [vp, i32, u64, i32, i32]),
     "scamper_ising_sweep_noalloc": (i32, [vp, vp, i32, dbl, vp, u32, u64, u32, u64, u64, vp, vp, vp]),
     "scamper_state_exchange": (i32, [vp, u64]),
     "scamper_state_labels": (i32, [vp, vp]),
@@ -378,6 +379,10 @@ class State:
 
     def sweep(self, algorithm: int = METROPOLIS, n_sweeps: int = 1, measure: bool = False):
         check(lib().scamper_sweep(self.h, algorithm, n_sweeps, int(measure)))
+
+    def sweep_with_phase(self, algorithm: int, n_sweeps: int, measure: bool, phase: int):
+        """Algorithm::sweep_with_phase; phase 0 = Thermalization, 1 = Measurement."""
+        check(lib().scamper_sweep_with_phase(self.h, algorithm, n_sweeps, int(measure), phase))
 
     def sweep_timed(self, algorithm: int = METROPOLIS, n_sweeps: int = 1,
                     measure: bool = False) -> float:

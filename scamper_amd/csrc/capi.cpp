@@ -1202,6 +1202,13 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
   return SCAMPER_OK;
 }
 
+int scamper_sweep_with_phase(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure, int phase) {
+  // SimulationPhase (algorithms/common.rs:9-12): only adaptive proposals act
+  // on it (allows_adaptation); every device algorithm is a fixed kernel
+  if (phase != 0 && phase != 1) return fail(SCAMPER_EINVAL, "phase must be 0 (thermalization) or 1 (measurement)");
+  return scamper_sweep(st, algorithm, n_sweeps, measure);
+}
+
 int scamper_sweep_timed(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure, float *ms) {
   if (!st || !ms) return fail(SCAMPER_EINVAL, "null argument");
   // table uploads and series growth synchronise; do them before the events

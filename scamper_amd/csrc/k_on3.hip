@@ -9,10 +9,12 @@
 // in FP64).  Spins are stored as three component planes per colour class
 // (x, y, z of rank j at [c][comp][j]) so every load is a coalesced dword.
 //
-// The inverse-CDF draw of cos(theta) is written in the form
-//   cos = 1 + log1p((1 - u) * expm1(-2 kappa)) / kappa,
+// The inverse-CDF draw of cos(theta) is written in the forms
+//   1 - cos = -log1p((1 - u) * expm1(-2 kappa)) / kappa,
+//   1 + cos =  log1p(u * expm1(2 kappa)) / kappa,
 // algebraically the reference's 1 + ln(u + (1 - u) e^{-2 kappa}) / kappa but
-// without its FP32 cancellation at small kappa.
+// without its FP32 cancellation at small kappa and in the tails, and
+// sin(theta) = sqrt((1 - cos)(1 + cos)) without that of 1 - cos^2.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -111,9 +113,23 @@ __device__ __forceinline__ V3 heat_bath_site(V3 h, const ReplicaParams &P, uint6
     return random_unit(philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
                                      (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32)));
   const float u = fmaxf((float)w0 * 2.3283064365386963e-10f, 1.17549435e-38f);
-  float c = 1.0f + log1pf((1.0f - u) * expm1f(-2.0f * kappa)) / kappa;
-  c = fminf(fmaxf(c, -1.0f), 1.0f);
-  const float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+  // a = 1 - cos(theta) and b = 1 + cos(theta), each formed where it is the
+  // small one: a = -log1p((1 - u) expm1(-2 kappa)) / kappa near the field,
+  // b = log1p(u expm1(2 kappa)) / kappa opposite it (the same inverse CDF;
+  // the first form's argument approaches -1 there and cancels in FP32), so
+  // cos(theta) and sin(theta) = sqrt(a b) keep their relative precision at
+  // both ends
+  float a = fminf(fmaxf(-log1pf((1.0f - u) * expm1f(-2.0f * kappa)) / kappa, 0.0f), 2.0f), b;
+  if (a > 1.0f) {
+    b = kappa < 40.0f ? log1pf(u * expm1f(2.0f * kappa)) / kappa
+                      : 2.0f + logf(u + (1.0f - u) * expf(-2.0f * kappa)) / kappa;
+    b = fminf(fmaxf(b, 0.0f), 2.0f);
+    a = 2.0f - b;
+  } else {
+    b = 2.0f - a;
+  }
+  const float c = a <= 1.0f ? 1.0f - a : b - 1.0f;
+  const float sn = sqrtf(a * b);
   float sphi, cphi;
   sincos_turns((float)w1 * 2.3283064365386963e-10f, &sphi, &cphi);
   const float lx = sn * cphi, ly = sn * sphi;

@@ -54,7 +54,11 @@ struct Job {
   std::unique_ptr<DefaultObservables> obs;
   std::vector<double> e_series, m_series;
   std::vector<int8_t> split_spins;  // colour-split copy of the final spins
+  int8_t *h_spins = nullptr;        // pinned staging of the final spins' copy
   scamper_estimate est[5]{};
+  ~Job() {
+    if (h_spins) hipHostFree(h_spins);
+  }
 };
 
 struct RowInfo {
@@ -241,6 +245,11 @@ void unsplit(const Job &J, const int8_t *split, int8_t *out) {
 }
 
 void finish(scamper_pool *P, Job &J) {
+  if (J.h_spins) {  // its copy completed before the chunk's end event
+    J.split_spins.assign(J.h_spins, J.h_spins + J.n_sites);
+    hipHostFree(J.h_spins);
+    J.h_spins = nullptr;
+  }
   J.obs->finalize(J.est);
   J.state = SCAMPER_JOB_DONE;
   P->completed.push_back(J.id);
@@ -304,8 +313,8 @@ int admit(scamper_pool *P, Job &J) {
   if (J.total == 0) {  // nothing to run: done at admission
     if (J.spec.keep_spins) {
       J.split_spins.resize(J.n_sites);
-      SC_HIP(hipMemcpyAsync(J.split_spins.data(), P->spins + J.off, J.n_sites, hipMemcpyDeviceToHost, P->stream));
       SC_HIP(hipStreamSynchronize(P->stream));
+      SC_HIP(hipMemcpy(J.split_spins.data(), P->spins + J.off, J.n_sites, hipMemcpyDeviceToHost));
     }
     P->arena.release(J.off, J.bytes);
     P->in_use -= J.bytes;
@@ -444,8 +453,10 @@ int step(scamper_pool *P, uint32_t *n_completed) {
     Job &J = *P->jobs.at(ri.id);
     J.enqueued += ri.steps;
     if (J.enqueued == J.total && J.spec.keep_spins) {
-      J.split_spins.resize(J.n_sites);
-      SC_HIP(hipMemcpyAsync(J.split_spins.data(), P->spins + J.off, J.n_sites, hipMemcpyDeviceToHost, P->stream));
+      // asynchronous copies go to pinned memory only; finish() moves it
+      if (hipHostMalloc(&J.h_spins, J.n_sites, hipHostMallocDefault) != hipSuccess)
+        return fail(SCAMPER_ENOMEM, "pinned staging of %llu final spins", (unsigned long long)J.n_sites);
+      SC_HIP(hipMemcpyAsync(J.h_spins, P->spins + J.off, J.n_sites, hipMemcpyDeviceToHost, P->stream));
     }
   }
   SC_HIP(hipEventRecord(P->ev_end[p], P->stream));

@@ -227,3 +227,70 @@ def test_xy_rejects_other_components(sc):
     for n_comp in (1, 4):
         with pytest.raises(sc.ScamperError):
             sc.State(lat, sc.model(sc.ON, 1.0, 2, n_comp), 1, 1)
+
+
+def _fields(spins, L):
+    """Local field h = sum of the six neighbours of every site of a periodic
+    L^3 lattice, from site-major f64 spins (x fastest)."""
+    a = spins.reshape(L, L, L, 3)
+    return sum(np.roll(a, sh, axis=ax) for ax in range(3) for sh in (1, -1)).reshape(-1, 3)
+
+
+def test_c4_shape_against_fp64_oracle(sc, orc):
+    """The 3-D Heisenberg workload at its own shape: 96^3 periodic, 32
+    replicas at T = 1.40 + 0.09 r / 31, FP32 spins from the seeded isotropic
+    start.  Lx/2 = 48 is even, so the heat-bath half runs the two-ranks-per-
+    thread kernel on its full XCD-remapped grid.  Three replicas (first,
+    middle, last) are replayed from the device's spins by the FP64 oracle
+    (models.rs:617-662 heat bath, microcanonical.rs:42-80 over-relaxation)
+    for one hybrid sweep.
+
+    Error model: both halves build the new spin from the direction h/|h| of
+    the local field (the heat bath's frame, the reflection), so FP32
+    rounding of order 1e-7 reaches a spin as about 1e-7 / |h|, and a
+    neighbour's error enters h.  Among 884,736 sites a few see small fields,
+    so the tail of the per-component error is wider than its bulk: the
+    bars are the maximum (2e-4), the 99.99% quantile (5e-5) and the mean
+    (1e-6) per component, and the replica's energy and |M| (1e-6)."""
+    L, R, seed = 96, 32, 42
+    print(f"case on3 C4 shape {L}^3 x {R} replicas", flush=True)
+    lat = sc.Lattice.hypercubic([L, L, L])
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 3), R, seed)
+    betas = [1.0 / (1.40 + 0.09 * r / 31) for r in range(R)]
+    for r in range(R):
+        st.init(True, r)
+        st.set_beta(betas[r], r)
+    st.sweep(sc.HYBRID_HB_OR, 2)  # leave the exact random start
+    replicas = (0, 13, 31)
+    starts = {r: st.download_f64(r) for r in replicas}
+    sweep = st.sweeps
+    st.sweep(sc.HYBRID_HB_OR, 1, measure=True)
+    olat = orc.Lattice.hypercubic([L, L, L])
+    m = orc.model(orc.ON, 1.0, 2, 3)
+    idx = np.arange(L ** 3)
+    colour = ((idx % L) + (idx // L % L) + (idx // (L * L))) & 1
+    for r in replicas:
+        chain = orc.Chain(olat, m, starts[r].copy())
+        chain.on3_heat_bath(betas[r], orc.replica_key(seed, r), sweep)
+        after_hb = chain.spins.copy()
+        chain.over_relaxation()
+        # the smallest |h| each site saw: heat bath (colour 0 from the start,
+        # colour 1 from the heat-bathed colour 0), over-relaxation (colour 0
+        # from the heat-bathed colour 1, colour 1 from the final colour 0)
+        hb = np.where(colour == 0, np.linalg.norm(_fields(starts[r], L), axis=1),
+                      np.linalg.norm(_fields(after_hb, L), axis=1))
+        orl = np.where(colour == 0, np.linalg.norm(_fields(after_hb, L), axis=1),
+                       np.linalg.norm(_fields(chain.spins, L), axis=1))
+        hmin = np.minimum(hb, orl)
+        got = st.download_f64(r)
+        err = np.abs(got - chain.spins).reshape(-1, 3).max(axis=1)
+        e, mag = st.series(r)
+        e_ref, m_ref = chain.total_energy(), chain.magnetization()
+        rel_e = abs(e[-1] - e_ref) / abs(e_ref)
+        worst = int(np.argmax(err))
+        print(f"replica {r}: per-site max {err.max():.2e} (|h|min {hmin[worst]:.2e} there), mean {err.mean():.2e}, "
+              f"99.99% {np.quantile(err, 0.9999):.2e}; "
+              f"E rel {rel_e:.2e}, |M| abs {abs(mag[-1] - m_ref):.2e}")
+        assert err.max() < 2e-4 and np.quantile(err, 0.9999) < 5e-5 and err.mean() < 1e-6
+        assert rel_e < 1e-6 and abs(mag[-1] - m_ref) < 1e-6
+    st.close()
