@@ -138,6 +138,18 @@ int scamper_state_download_f32(scamper_state *st, uint32_t replica, float *spins
  * the label it holds). */
 int scamper_state_set_ladder(scamper_state *st, const double *beta_by_label);
 int scamper_state_exchange(scamper_state *st, uint64_t step);
+/* n_steps tempering steps: one sweep of every replica with `algorithm`,
+ * then one exchange round numbered by the sweep count (the reference's
+ * ParallelTemperingMC::step with tempering_interval 1,
+ * Carlo.rs/src/parallel_tempering.rs:307-355).  With measure != 0 each step's exact
+ * energy and magnetisation of every replica join the series.  The steps'
+ * counters live on the device, so for O(N) on stencils and the Ising
+ * Metropolis kernel on Lx % 32 == 0 lattices chunks of 16 steps replay one
+ * captured HIP graph; other states launch step by step with identical
+ * results. */
+int scamper_state_pt_step(scamper_state *st, int algorithm, uint64_t n_steps, int measure);
+/* Captured-graph launches made by scamper_state_pt_step so far. */
+int scamper_state_pt_graph_launches(const scamper_state *st, uint64_t *n);
 /* Label held by each replica (n_replicas values). */
 int scamper_state_labels(scamper_state *st, int32_t *labels);
 /* Energies of every replica used by the last exchange round. */

@@ -99,6 +99,8 @@ SIGNATURES = {
     "scamper_sweep_with_phase": (i32, [vp, i32, u64, i32, i32]),
     "scamper_ising_sweep_noalloc": (i32, [vp, vp, i32, dbl, vp, u32, u64, u32, u64, u64, vp, vp, vp]),
     "scamper_state_exchange": (i32, [vp, u64]),
+    "scamper_state_pt_step": (i32, [vp, i32, u64, i32]),
+    "scamper_state_pt_graph_launches": (i32, [vp, vp]),
     "scamper_state_labels": (i32, [vp, vp]),
     "scamper_state_exchange_energies": (i32, [vp, vp]),
     "scamper_state_exchange_stats": (i32, [vp, vp, vp, vp]),
@@ -344,6 +346,15 @@ class State:
         """One replica-exchange round (PtExchange::try_exchange,
         Carlo.rs/src/parallel_tempering.rs:362-419), queued on the device."""
         check(lib().scamper_state_exchange(self.h, int(step)))
+
+    def pt_step(self, algorithm: int, n_steps: int, measure: bool = False):
+        """n_steps of (sweep, exchange round numbered by the sweep count)."""
+        check(lib().scamper_state_pt_step(self.h, algorithm, int(n_steps), int(measure)))
+
+    def pt_graph_launches(self) -> int:
+        n = u64()
+        check(lib().scamper_state_pt_graph_launches(self.h, C.byref(n)))
+        return n.value
 
     def labels(self) -> np.ndarray:
         out = np.zeros(self.n_replicas, np.int32)

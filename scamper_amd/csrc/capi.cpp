@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <string>
@@ -131,6 +132,12 @@ struct scamper_state {
   ReplicaParams *d_lab_params = nullptr;
   TileJob *d_lab_rows = nullptr;
   double *d_lab_hbexp = nullptr;
+  // tempering steps: device step counters and captured chunks of steps by
+  // (algorithm, measure), captured on cap_stream and launched on stream
+  StepClock *d_clock = nullptr;
+  hipStream_t cap_stream = nullptr;
+  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> pt_graphs;
+  uint64_t pt_graph_launches = 0;
 
   ~scamper_state() {
     if (stream) hipStreamSynchronize(stream);
@@ -164,6 +171,12 @@ struct scamper_state {
     hipFree(d_lab_params);
     hipFree(d_lab_rows);
     hipFree(d_lab_hbexp);
+    hipFree(d_clock);
+    for (auto &kv : pt_graphs) {
+      hipGraphExecDestroy(kv.second.second);
+      hipGraphDestroy(kv.second.first);
+    }
+    if (cap_stream) hipStreamDestroy(cap_stream);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -221,6 +234,112 @@ int sync_labels(scamper_state *st) {
   SC_HIP(hipMemcpy(st->labels.data(), st->d_labels, st->n_rep * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (uint32_t r = 0; r < st->n_rep; ++r) st->beta[r] = st->ladder[st->labels[r]];
   st->labels_dirty = false;
+  return SCAMPER_OK;
+}
+
+// Exact energies of the current spins (change_parameter's recompute), one
+// exchange round over the labels, and each replica's tables from its new
+// label.  With a clock, the round number is the clock's.
+hipError_t enqueue_exchange(scamper_state *st, uint64_t step, const StepClock *clk, hipStream_t s) {
+  hipError_t e = launch_em_zero(st->d_em, (uint64_t)kEmStride * st->n_rep, s);
+  if (e == hipSuccess) {
+    if (st->continuous)
+      e = st->csr ? launch_on_csr_measure(st->ogeom, st->model.spin_dim, st->d_vec, st->n_rep, st->model.j,
+                                          reinterpret_cast<double *>(st->d_em), s)
+                  : launch_on3_measure(st->geom, st->model.spin_dim, st->d_vec, st->n_rep, (float)st->model.j,
+                                       reinterpret_cast<double *>(st->d_em), s);
+    else if (st->csr)
+      e = st->real_weights ? launch_csr_real_measure(st->rgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep,
+                                                     st->d_em, s)
+                           : launch_csr_measure(st->cgeom, st->model.kind, st->model.q, st->d_spins, st->n_rep,
+                                                st->d_em, s);
+    else
+      e = launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep, st->d_em, s);
+  }
+  EnergyForm form{};
+  form.j = st->model.j;
+  form.self_loops = st->self_loop_weight;
+  form.kind = st->continuous ? (st->csr ? kEnergyVecGraph : kEnergyVecStencil)
+              : st->csr      ? (st->real_weights ? kEnergyGraphReal : kEnergyGraph)
+                             : kEnergyStencil;
+  if (e == hipSuccess)
+    e = launch_state_exchange(st->d_labels, st->d_em, st->n_rep, form, st->d_ladder, stream_seed(st->seed, 0, 3), step,
+                              st->d_xenergy, st->d_xstats, s, clk);
+  if (e == hipSuccess && !st->ladder.empty() && st->tables_for >= 0) {
+    TileJob *rows = nullptr;
+    if (uses_tiles(st, st->tables_for) && st->d_tiles && st->d_lab_rows)
+      rows = reinterpret_cast<TileJob *>(static_cast<TileHeader *>(st->d_tiles) + 1);
+    const bool hb = st->d_lab_hbexp && st->d_hbexp && st->model.kind == kPotts && st->tables_for == kHeatBath &&
+                    !uses_real_kernels(st, kHeatBath);
+    e = launch_apply_labels(st->d_labels, st->n_rep, st->d_params, st->d_lab_params, rows,
+                            rows ? st->d_lab_rows : nullptr, hb ? st->d_hbexp : nullptr, hb ? st->d_lab_hbexp : nullptr,
+                            (uint32_t)((kMaxZ + 1) * (kMaxZ + 1)), s);
+  }
+  return e;
+}
+
+constexpr uint64_t kPtGraphSteps = 16;  // tempering steps per captured chunk
+
+// Tempering steps whose sweep kernels read the device clock: O(N) on
+// stencils (heat bath, over-relaxation, hybrid) and the register-blocked
+// Ising Metropolis.
+bool pt_graph_ok(const scamper_state *st, int alg) {
+  if (st->csr) return false;
+  if (st->continuous)
+    return alg == kContinuousHeatBath || alg == kOverRelaxation || alg == kHybridHeatBathOverRelaxation;
+  return st->model.kind == kIsing && alg == kMetropolis && ising_fast_path_ok(st->geom);
+}
+
+// One tempering step on `s` from the clock: the sweep, the exchange round
+// (its energies are the step's measurement), the series record, the tick.
+int enqueue_pt_step(scamper_state *st, int alg, int measure, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (st->continuous) {
+    const int parts[2] = {alg == kOverRelaxation ? kOverRelaxation : kContinuousHeatBath,
+                          alg == kHybridHeatBathOverRelaxation ? kOverRelaxation : -1};
+    for (int pi = 0; pi < 2 && e == hipSuccess; ++pi)
+      for (uint32_t c = 0; c < 2 && e == hipSuccess && parts[pi] >= 0; ++c)
+        e = launch_on3_sweep(st->geom, st->model.spin_dim, parts[pi], st->d_vec, st->d_params, st->n_rep, c, 0,
+                             (float)st->model.j, nullptr, s, st->d_clock);
+  } else {
+    for (uint32_t c = 0; c < 2 && e == hipSuccess; ++c)
+      e = launch_ising_fast(st->geom, st->d_spins, st->d_params, st->n_rep, c, 0, nullptr, s, st->d_clock);
+  }
+  if (e == hipSuccess) e = enqueue_exchange(st, 0, st->d_clock, s);
+  if (e == hipSuccess && measure)
+    e = launch_series_from_em(st->d_em, st->d_clock, (uint64_t)st->n_rep * kEmStride, s);
+  if (e == hipSuccess) e = launch_clock_tick(st->d_clock, 1, measure ? 1 : 0, s);
+  if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "tempering step: %s", hipGetErrorString(e));
+  return SCAMPER_OK;
+}
+
+// The captured chunk of kPtGraphSteps tempering steps for (algorithm, measure).
+int pt_graph(scamper_state *st, int alg, int measure, hipGraphExec_t *out) {
+  const int key = alg * 2 + (measure != 0);
+  auto it = st->pt_graphs.find(key);
+  if (it != st->pt_graphs.end()) {
+    *out = it->second.second;
+    return SCAMPER_OK;
+  }
+  if (!st->cap_stream) SC_HIP(hipStreamCreateWithFlags(&st->cap_stream, hipStreamNonBlocking));
+  SC_HIP(hipStreamBeginCapture(st->cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc = SCAMPER_OK;
+  for (uint64_t t = 0; t < kPtGraphSteps && rc == SCAMPER_OK; ++t) rc = enqueue_pt_step(st, alg, measure, st->cap_stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st->cap_stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    hipGraphDestroy(g);
+    return fail(SCAMPER_EDEVICE, "graph instantiation: %s", hipGetErrorString(ei));
+  }
+  st->pt_graphs[key] = {g, x};
+  *out = x;
   return SCAMPER_OK;
 }
 
@@ -757,26 +876,61 @@ int scamper_state_set_ladder(scamper_state *st, const double *beta_by_label) {
 int scamper_state_exchange(scamper_state *st, uint64_t step) {
   if (!st) return fail(SCAMPER_EINVAL, "null state");
   if (st->ladder.empty()) return fail(SCAMPER_EINVAL, "no temperature ladder: call scamper_state_set_ladder first");
-  // exact energies of the current spins (change_parameter's recompute)
-  hipError_t e = launch_em_zero(st->d_em, (uint64_t)kEmStride * st->n_rep, st->stream);
-  if (e == hipSuccess)
-    e = st->continuous ? measure_on(st, reinterpret_cast<double *>(st->d_em))
-        : st->csr      ? measure_graph(st, st->d_em)
-                       : launch_stencil_measure(st->geom, st->model.kind, st->model.q, st->d_spins, st->n_rep,
-                                                st->d_em, st->stream);
-  EnergyForm form{};
-  form.j = st->model.j;
-  form.self_loops = st->self_loop_weight;
-  form.kind = st->continuous ? (st->csr ? kEnergyVecGraph : kEnergyVecStencil)
-              : st->csr      ? (st->real_weights ? kEnergyGraphReal : kEnergyGraph)
-                             : kEnergyStencil;
-  if (e == hipSuccess)
-    e = launch_state_exchange(st->d_labels, st->d_em, st->n_rep, form, st->d_ladder, stream_seed(st->seed, 0, 3), step,
-                              st->d_xenergy, st->d_xstats, st->stream);
-  if (e == hipSuccess) e = apply_labels(st);
+  const hipError_t e = enqueue_exchange(st, step, nullptr, st->stream);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "exchange: %s", hipGetErrorString(e));
   st->labels_dirty = true;
   st->cthr_ready = false;
+  return SCAMPER_OK;
+}
+
+int scamper_state_pt_step(scamper_state *st, int algorithm, uint64_t n_steps, int measure) {
+  if (!st) return fail(SCAMPER_EINVAL, "null state");
+  if (st->ladder.empty()) return fail(SCAMPER_EINVAL, "no temperature ladder: call scamper_state_set_ladder first");
+  if (n_steps == 0) return SCAMPER_OK;
+  if (measure)
+    if (int rc = ensure_series(st, n_steps)) return rc;
+  if (!st->d_clock) SC_HIP(hipMalloc(&st->d_clock, sizeof(StepClock)));
+  const bool graphs = pt_graph_ok(st, algorithm);
+  if (graphs) {
+    const int table_alg = st->continuous ? kContinuousHeatBath : algorithm;
+    if (st->tables_for != table_alg)
+      if (int rc = upload_tables(st, table_alg)) return rc;
+  }
+  // the clock starts at the host's counters: sweep s, exchange round s + 1
+  SC_HIP(launch_clock_set(st->d_clock, st->sweeps, st->sweeps + 1, st->series_len, st->d_series, st->stream));
+  const uint64_t chunks = graphs ? n_steps / kPtGraphSteps : 0;
+  if (chunks) {
+    hipGraphExec_t x = nullptr;
+    if (int rc = pt_graph(st, algorithm, measure, &x)) return rc;
+    for (uint64_t k = 0; k < chunks; ++k) SC_HIP(hipGraphLaunch(x, st->stream));
+    st->pt_graph_launches += chunks;
+    st->sweeps += chunks * kPtGraphSteps;
+    if (measure) st->series_len += chunks * kPtGraphSteps;
+  }
+  for (uint64_t t = chunks * kPtGraphSteps; t < n_steps; ++t) {
+    if (graphs) {
+      if (int rc = enqueue_pt_step(st, algorithm, measure, st->stream)) return rc;
+      ++st->sweeps;
+      if (measure) ++st->series_len;
+      continue;
+    }
+    // kernels without the clock: the sweep by value, then the clock's round
+    if (int rc = scamper_sweep(st, algorithm, 1, 0)) return rc;
+    hipError_t e = enqueue_exchange(st, 0, st->d_clock, st->stream);
+    if (e == hipSuccess && measure)
+      e = launch_series_from_em(st->d_em, st->d_clock, (uint64_t)st->n_rep * kEmStride, st->stream);
+    if (e == hipSuccess) e = launch_clock_tick(st->d_clock, 1, measure ? 1 : 0, st->stream);
+    if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "tempering step: %s", hipGetErrorString(e));
+    if (measure) ++st->series_len;
+  }
+  st->labels_dirty = true;
+  st->cthr_ready = false;
+  return SCAMPER_OK;
+}
+
+int scamper_state_pt_graph_launches(const scamper_state *st, uint64_t *n) {
+  if (!st || !n) return fail(SCAMPER_EINVAL, "null argument");
+  *n = st->pt_graph_launches;
   return SCAMPER_OK;
 }
 

@@ -91,6 +91,22 @@ struct CsrGeom {
 //         em[1 .. q] (Potts).
 constexpr int kEmStride = 2 + kMaxQ;
 
+// Step counters kept on the device, so that a captured graph of several
+// sweeps or tempering steps replays for any starting point: kernels given a
+// clock take their sweep number (and exchange round, series slot) from it
+// instead of their by-value arguments, and a tick kernel advances it.
+struct alignas(16) StepClock {
+  uint64_t sweep;     // sweep number of the step being run
+  uint64_t exchange;  // exchange round of the step (as the exchange kernels number it)
+  uint64_t slot;      // series slot the step's measurement goes to
+  void *series;       // series buffer (its records per slot as the owner lays them out)
+};
+hipError_t launch_clock_set(StepClock *clk, uint64_t sweep, uint64_t exchange, uint64_t slot, void *series,
+                            hipStream_t stream);
+hipError_t launch_clock_tick(StepClock *clk, uint64_t d_exchange, uint64_t d_slot, hipStream_t stream);
+// Copy `count` counters from em into the clock's series at its slot.
+hipError_t launch_series_from_em(const long long *em, const StepClock *clk, uint64_t count, hipStream_t stream);
+
 // ------------------------------------------------------------ launchers
 hipError_t launch_stencil_discrete(const StencilGeom &g, int model_kind, int q, int algorithm,
                                    int8_t *spins, const ReplicaParams *reps, uint32_t n_rep,
@@ -148,7 +164,8 @@ hipError_t launch_on_csr_init(const OnCsrGeom &g, int dim, float *spins, uint64_
                               hipStream_t st);
 
 hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
-                            uint32_t n_rep, uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t stream);
+                            uint32_t n_rep, uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t stream,
+                            const StepClock *clk = nullptr);
 hipError_t launch_on3_measure(const StencilGeom &g, int dim, const float *spins, uint32_t n_rep, float j, double *em,
                               hipStream_t stream);
 hipError_t launch_on3_init(const StencilGeom &g, int dim, float *spins, uint64_t key, uint32_t rep, int random,
@@ -159,10 +176,13 @@ hipError_t launch_ms_init(uint64_t *words, const uint64_t *keys, const StencilGe
                           hipStream_t stream);
 hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const int32_t *labels, const uint64_t *keys,
                            const StencilGeom &g, uint32_t n_bundles, uint32_t colour, uint64_t sweep,
-                           unsigned long long *counts, hipStream_t stream);
+                           unsigned long long *counts, hipStream_t stream, const StepClock *clk = nullptr);
+// With a clock, the round is clk->exchange and, when measure_to_clock is
+// set, the energies and |m| go to the clock's series slot ([slot][2][n]).
 hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const double *beta_by_label, uint32_t n,
                               double n_bonds, double n_sites, double j, uint64_t ex_key, uint64_t step, int do_exchange,
-                              double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t stream);
+                              double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t stream,
+                              const StepClock *clk = nullptr, int measure_to_clock = 0);
 
 // ------------------------------------------------------------ tiled jobs
 // Checkerboard Metropolis of Ising and Potts spins for any number of
@@ -300,7 +320,7 @@ struct EnergyForm {
 // counters em[r * kEmStride ...]; stats = {accepted, near ties, rounds}.
 hipError_t launch_state_exchange(int32_t *labels, const long long *em, uint32_t n, const EnergyForm &form,
                                  const double *ladder, uint64_t ex_key, uint64_t step, double *energy_out,
-                                 unsigned long long *stats, hipStream_t stream);
+                                 unsigned long long *stats, hipStream_t stream, const StepClock *clk = nullptr);
 // Copy the decision tables of the label each replica holds into its own
 // parameters (and tiled rows / heat-bath weights when given).
 hipError_t launch_apply_labels(const int32_t *labels, uint32_t n, ReplicaParams *params,
@@ -311,6 +331,6 @@ hipError_t launch_apply_labels(const int32_t *labels, uint32_t n, ReplicaParams 
 bool ising_fast_path_ok(const StencilGeom &g);
 hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,
                              uint32_t n_rep, uint32_t colour, uint64_t sweep, long long *em,
-                             hipStream_t stream);
+                             hipStream_t stream, const StepClock *clk = nullptr);
 
 }  // namespace scamper

@@ -17,6 +17,8 @@
 // the host's observables are.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.h"
 
 namespace scamper {
@@ -46,7 +48,9 @@ __device__ double energy_of(const long long *em, const EnergyForm &f) {
 __global__ __launch_bounds__(1024) void k_state_exchange(int32_t *__restrict__ labels, const long long *__restrict__ em,
                                                          uint32_t n, EnergyForm form, const double *__restrict__ ladder,
                                                          uint64_t ex_key, uint64_t step, double *__restrict__ energy_out,
-                                                         unsigned long long *__restrict__ stats) {
+                                                         unsigned long long *__restrict__ stats,
+                                                         const StepClock *__restrict__ clk) {
+  if (clk) step = clk->exchange;
   __shared__ int32_t replica_of[1024];
   __shared__ double energy[1024];
   const uint32_t t = threadIdx.x;
@@ -114,13 +118,48 @@ __global__ __launch_bounds__(256) void k_apply_labels(const int32_t *__restrict_
     for (uint32_t k = t; k < hb_per; k += blockDim.x) hbexp[(uint64_t)r * hb_per + k] = label_hbexp[(uint64_t)l * hb_per + k];
 }
 
+__global__ void k_clock_set(StepClock *clk, uint64_t sweep, uint64_t exchange, uint64_t slot, void *series) {
+  if (threadIdx.x == 0) *clk = StepClock{sweep, exchange, slot, series};
+}
+
+__global__ void k_clock_tick(StepClock *clk, uint64_t d_exchange, uint64_t d_slot) {
+  if (threadIdx.x == 0) {
+    clk->sweep += 1;
+    clk->exchange += d_exchange;
+    clk->slot += d_slot;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_series_from_em(const long long *__restrict__ em, const StepClock *clk,
+                                                        uint64_t count) {
+  long long *out = static_cast<long long *>(clk->series) + clk->slot * count;
+  for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < count; k += (uint64_t)gridDim.x * 256) out[k] = em[k];
+}
+
 }  // namespace
+
+hipError_t launch_clock_set(StepClock *clk, uint64_t sweep, uint64_t exchange, uint64_t slot, void *series,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, st, clk, sweep, exchange, slot, series);
+  return hipGetLastError();
+}
+
+hipError_t launch_clock_tick(StepClock *clk, uint64_t d_exchange, uint64_t d_slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_clock_tick, dim3(1), dim3(64), 0, st, clk, d_exchange, d_slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_series_from_em(const long long *em, const StepClock *clk, uint64_t count, hipStream_t st) {
+  const unsigned blocks = (unsigned)std::min<uint64_t>((count + 255) / 256, 64);
+  hipLaunchKernelGGL(k_series_from_em, dim3(blocks), dim3(256), 0, st, em, clk, count);
+  return hipGetLastError();
+}
 
 hipError_t launch_state_exchange(int32_t *labels, const long long *em, uint32_t n, const EnergyForm &form,
                                  const double *ladder, uint64_t ex_key, uint64_t step, double *energy_out,
-                                 unsigned long long *stats, hipStream_t st) {
+                                 unsigned long long *stats, hipStream_t st, const StepClock *clk) {
   hipLaunchKernelGGL(k_state_exchange, dim3(1), dim3(1024), 0, st, labels, em, n, form, ladder, ex_key, step,
-                     energy_out, stats);
+                     energy_out, stats, clk);
   return hipGetLastError();
 }
 

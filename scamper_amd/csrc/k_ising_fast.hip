@@ -126,8 +126,10 @@ template <int ND, bool MEAS, uint32_t R, typename Off, uint32_t PF, bool NBR>
 __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spins,
                                                        const ReplicaParams *__restrict__ reps,
                                                        StencilGeom g, uint32_t colour,
-                                                       uint64_t sweep, long long *__restrict__ em) {
+                                                       uint64_t sweep, long long *__restrict__ em,
+                                                       const StepClock *__restrict__ clk) {
   constexpr uint32_t z = 2 * ND;
+  if (clk) sweep = clk->sweep;
   __shared__ TiedRow parked[kSlots][kBlock];
   const uint32_t rep = blockIdx.y;
   // XCD-aware order: the hardware deals consecutive blocks round-robin over
@@ -320,19 +322,19 @@ uint32_t fast_rows(const StencilGeom &g) {
 
 template <int ND, bool MEAS>
 void launch_rows(uint32_t R, dim3 grid, hipStream_t st, int8_t *spins, const ReplicaParams *reps,
-                 const StencilGeom &g, uint32_t colour, uint64_t sweep, long long *em) {
+                 const StencilGeom &g, uint32_t colour, uint64_t sweep, long long *em, const StepClock *clk) {
   // lanes hold consecutive chunks of one row when a row is whole waves wide
   const bool nbr = (g.lx2 / 16) % 64 == 0;
 #define SC_ROWS(RR)                                                                                        \
   if (g.nh >= (1ull << 32))                                                                                \
     hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR, uint64_t, kPrefetch, false>), grid, dim3(kBlock), 0, st, \
-                       spins, reps, g, colour, sweep, em);                                                 \
+                       spins, reps, g, colour, sweep, em, clk);                                            \
   else if (nbr)                                                                                            \
     hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR, uint32_t, kPrefetch, true>), grid, dim3(kBlock), 0, st,  \
-                       spins, reps, g, colour, sweep, em);                                                 \
+                       spins, reps, g, colour, sweep, em, clk);                                            \
   else                                                                                                     \
     hipLaunchKernelGGL((k_ising_fast<ND, MEAS, RR, uint32_t, kPrefetch, false>), grid, dim3(kBlock), 0, st, \
-                       spins, reps, g, colour, sweep, em)
+                       spins, reps, g, colour, sweep, em, clk)
   switch (R) {
     case 16: SC_ROWS(16); break;
     case 8: SC_ROWS(8); break;
@@ -350,16 +352,16 @@ bool ising_fast_path_ok(const StencilGeom &g) {
 
 hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,
                              uint32_t n_rep, uint32_t colour, uint64_t sweep, long long *em,
-                             hipStream_t st) {
+                             hipStream_t st, const StepClock *clk) {
   const uint32_t R = fast_rows(g);
   const uint64_t threads = (uint64_t)(g.lx2 / 16) * (g.ly / R) * g.lz;
   dim3 grid((unsigned)((threads + kBlock - 1) / kBlock), n_rep);
   if (g.nd == 2) {
-    if (em) launch_rows<2, true>(R, grid, st, spins, reps, g, colour, sweep, em);
-    else launch_rows<2, false>(R, grid, st, spins, reps, g, colour, sweep, em);
+    if (em) launch_rows<2, true>(R, grid, st, spins, reps, g, colour, sweep, em, clk);
+    else launch_rows<2, false>(R, grid, st, spins, reps, g, colour, sweep, em, clk);
   } else {
-    if (em) launch_rows<3, true>(R, grid, st, spins, reps, g, colour, sweep, em);
-    else launch_rows<3, false>(R, grid, st, spins, reps, g, colour, sweep, em);
+    if (em) launch_rows<3, true>(R, grid, st, spins, reps, g, colour, sweep, em, clk);
+    else launch_rows<3, false>(R, grid, st, spins, reps, g, colour, sweep, em, clk);
   }
   return hipGetLastError();
 }

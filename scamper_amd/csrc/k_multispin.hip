@@ -68,8 +68,10 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
                                                      const int32_t *__restrict__ labels,
                                                      const uint64_t *__restrict__ keys, StencilGeom g,
                                                      uint32_t colour, uint64_t sweep,
-                                                     unsigned long long *__restrict__ counts) {
+                                                     unsigned long long *__restrict__ counts,
+                                                     const StepClock *__restrict__ clk) {
   constexpr int z = 2 * ND;
+  if (clk) sweep = clk->sweep;
   const uint32_t bundle = blockIdx.y;
   __shared__ Threshold sthr[64 * (z + 1)];
   __shared__ uint64_t skey[64];
@@ -263,7 +265,15 @@ __global__ __launch_bounds__(1024) void k_ms_exchange(int32_t *__restrict__ labe
                                                       double n_bonds, double n_sites, double j, uint64_t ex_key,
                                                       uint64_t step, int do_exchange, double *__restrict__ energy_out,
                                                       double *__restrict__ mag_out,
-                                                      unsigned long long *__restrict__ stats) {
+                                                      unsigned long long *__restrict__ stats,
+                                                      const StepClock *__restrict__ clk, int measure_to_clock) {
+  if (clk) {
+    step = clk->exchange;
+    if (measure_to_clock) {
+      energy_out = static_cast<double *>(clk->series) + clk->slot * 2ull * n;
+      mag_out = energy_out + n;
+    }
+  }
   __shared__ int32_t cfg_of[1024];
   __shared__ double energy[1024];
   const uint32_t t = threadIdx.x;
@@ -322,16 +332,16 @@ hipError_t launch_ms_init(uint64_t *words, const uint64_t *keys, const StencilGe
 
 hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const int32_t *labels, const uint64_t *keys,
                            const StencilGeom &g, uint32_t n_bundles, uint32_t colour, uint64_t sweep,
-                           unsigned long long *counts, hipStream_t st) {
+                           unsigned long long *counts, hipStream_t st, const StepClock *clk) {
   const uint64_t threads = ((g.nh + 15) / 16) * 4;  // four quarters per group of 16 sites
   dim3 grid((unsigned)((threads + kBlock - 1) / kBlock), n_bundles);
 #define SC_MS(ND)                                                                                                     \
   if (counts)                                                                                                         \
     hipLaunchKernelGGL((k_ms_sweep<ND, true>), grid, dim3(kBlock), 0, st, words, thr_by_label, labels, keys, g, colour, \
-                       sweep, counts);                                                                                \
+                       sweep, counts, clk);                                                                           \
   else                                                                                                                \
     hipLaunchKernelGGL((k_ms_sweep<ND, false>), grid, dim3(kBlock), 0, st, words, thr_by_label, labels, keys, g,       \
-                       colour, sweep, counts);
+                       colour, sweep, counts, clk);
   switch (g.nd) {
     case 1: SC_MS(1) break;
     case 2: SC_MS(2) break;
@@ -344,9 +354,10 @@ hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const
 
 hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const double *beta_by_label, uint32_t n,
                               double n_bonds, double n_sites, double j, uint64_t ex_key, uint64_t step, int do_exchange,
-                              double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t st) {
+                              double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t st,
+                              const StepClock *clk, int measure_to_clock) {
   hipLaunchKernelGGL(k_ms_exchange, dim3(1), dim3(1024), 0, st, labels, counts, beta_by_label, n, n_bonds, n_sites, j,
-                     ex_key, step, do_exchange, energy_out, mag_out, stats);
+                     ex_key, step, do_exchange, energy_out, mag_out, stats, clk, measure_to_clock);
   return hipGetLastError();
 }
 

@@ -172,7 +172,8 @@ __device__ __forceinline__ void xcd_item(uint32_t nbx, uint32_t *rep, uint32_t *
 template <int ND, int ALG, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_on3(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
                                                 StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
-                                                float jc, double *__restrict__ em) {
+                                                float jc, double *__restrict__ em, const StepClock *__restrict__ clk) {
+  if (clk) sweep = clk->sweep;
   uint32_t rep, bx;
   xcd_item(nbx, &rep, &bx);
   const uint64_t j = (uint64_t)bx * kBlock + threadIdx.x;
@@ -226,7 +227,8 @@ __global__ __launch_bounds__(kBlock) void k_on3(float *__restrict__ spins, const
 template <int ND, int ALG, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_on3_pair(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
                                                      StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
-                                                     float jc, double *__restrict__ em) {
+                                                     float jc, double *__restrict__ em, const StepClock *__restrict__ clk) {
+  if (clk) sweep = clk->sweep;
   uint32_t rep, bx;
   xcd_item(nbx, &rep, &bx);
   const uint64_t t = (uint64_t)bx * kBlock + threadIdx.x;
@@ -430,7 +432,8 @@ __device__ __forceinline__ void random_unit2(uint32_t w0, uint32_t w1, double *o
 template <int ND, int ALG, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_on2(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
                                                 StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
-                                                float jc, double *__restrict__ em) {
+                                                float jc, double *__restrict__ em, const StepClock *__restrict__ clk) {
+  if (clk) sweep = clk->sweep;
   uint32_t rep, bx;
   xcd_item(nbx, &rep, &bx);
   const uint64_t j = (uint64_t)bx * kBlock + threadIdx.x;
@@ -726,16 +729,19 @@ __global__ __launch_bounds__(kBlock) void k_on_csr_init(float *__restrict__ spin
 }  // namespace
 
 hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins, const ReplicaParams *reps,
-                            uint32_t n_rep, uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t st) {
+                            uint32_t n_rep, uint32_t colour, uint64_t sweep, float j, double *em, hipStream_t st,
+                            const StepClock *clk) {
   if (dim == 2) {
     const uint32_t nbx = (uint32_t)((g.nh + kBlock - 1) / kBlock);
     const bool flat = (uint64_t)nbx * n_rep * kBlock < (1ull << 32);
     const dim3 grid = flat ? dim3(nbx * n_rep) : dim3(nbx, n_rep);
 #define SC_ON2(ND, A)                                                                                                 \
   if (em)                                                                                                             \
-    hipLaunchKernelGGL((k_on2<ND, A, true>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);   \
+    hipLaunchKernelGGL((k_on2<ND, A, true>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em,    \
+                       clk);                                                                                          \
   else                                                                                                                \
-    hipLaunchKernelGGL((k_on2<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);
+    hipLaunchKernelGGL((k_on2<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em,   \
+                       clk);
 #define SC_ON2_ND(ND)                                                  \
   if (alg == kContinuousHeatBath) { SC_ON2(ND, kContinuousHeatBath) } \
   else { SC_ON2(ND, kOverRelaxation) }
@@ -758,9 +764,9 @@ hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins
   const dim3 grid = flat ? dim3(nbx * n_rep) : dim3(nbx, n_rep);
 #define SC_ON3_K(K, ND, A)                                                                                             \
   if (em)                                                                                                              \
-    hipLaunchKernelGGL((K<ND, A, true>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);        \
+    hipLaunchKernelGGL((K<ND, A, true>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em, clk);   \
   else                                                                                                                 \
-    hipLaunchKernelGGL((K<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em);
+    hipLaunchKernelGGL((K<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em, clk);
 #define SC_ON3(ND, A)              \
   if (pairs) {                     \
     SC_ON3_K(k_on3_pair, ND, A)    \

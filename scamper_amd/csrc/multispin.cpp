@@ -32,6 +32,13 @@ struct scamper_multispin {
   double *d_series = nullptr;  // [step][2][n]: energies then |m|
   uint64_t series_cap = 0, series_len = 0;
   bool tables_ready = false;
+  // device step counters and the captured chunks of kGraphSteps steps, by
+  // (exchange, measure); captured on cap_stream, launched on stream
+  StepClock *d_clock = nullptr;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph[2][2] = {};
+  hipGraphExec_t exec[2][2] = {};
+  uint64_t graph_launches = 0;
 
   ~scamper_multispin() {
     if (stream) hipStreamSynchronize(stream);
@@ -43,6 +50,13 @@ struct scamper_multispin {
     hipFree(d_counts);
     hipFree(d_stats);
     hipFree(d_series);
+    hipFree(d_clock);
+    for (int x = 0; x < 2; ++x)
+      for (int m = 0; m < 2; ++m) {
+        if (exec[x][m]) hipGraphExecDestroy(exec[x][m]);
+        if (graph[x][m]) hipGraphDestroy(graph[x][m]);
+      }
+    if (cap_stream) hipStreamDestroy(cap_stream);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -185,33 +199,84 @@ int scamper_multispin_download(scamper_multispin *ms, uint64_t *words) {
   return SCAMPER_OK;
 }
 
+namespace {
+
+constexpr uint64_t kGraphSteps = 16;  // steps per captured chunk
+
+// One step on `st`, its sweep number, exchange round and series slot taken
+// from the device clock: the colour-0 and colour-1 sweeps, then the
+// exchange kernel (which also turns the counters into energies and records
+// them when measuring), then the clock's tick.
+int enqueue_ms_step(scamper_multispin *ms, int exchange, int measure, hipStream_t st) {
+  const bool record = measure || exchange;
+  const double n_bonds = (double)ms->z * (double)ms->n_sites / 2.0;
+  const uint64_t ex_key = stream_seed(ms->seed, 0, 3);
+  for (uint32_t c = 0; c < 2; ++c) {
+    hipError_t e = launch_ms_sweep(ms->d_words, ms->d_thr, ms->d_labels, ms->d_keys, ms->geom, ms->n_bundles, c, 0,
+                                   (record && c == 1) ? ms->d_counts : nullptr, st, ms->d_clock);
+    if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "multispin sweep: %s", hipGetErrorString(e));
+  }
+  if (record) {
+    hipError_t e = launch_ms_exchange(ms->d_labels, ms->d_counts, ms->d_beta, ms->n, n_bonds, (double)ms->n_sites, ms->j,
+                                      ex_key, 0, exchange != 0, nullptr, nullptr, ms->d_stats, st, ms->d_clock,
+                                      measure != 0);
+    if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "multispin exchange: %s", hipGetErrorString(e));
+  }
+  SC_HIP(launch_clock_tick(ms->d_clock, exchange ? 1 : 0, measure ? 1 : 0, st));
+  return SCAMPER_OK;
+}
+
+// The captured chunk of kGraphSteps steps for (exchange, measure).
+int ms_graph(scamper_multispin *ms, int exchange, int measure, hipGraphExec_t *out) {
+  hipGraphExec_t &x = ms->exec[exchange != 0][measure != 0];
+  if (!x) {
+    if (!ms->cap_stream) SC_HIP(hipStreamCreateWithFlags(&ms->cap_stream, hipStreamNonBlocking));
+    SC_HIP(hipStreamBeginCapture(ms->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = SCAMPER_OK;
+    for (uint64_t t = 0; t < kGraphSteps && rc == SCAMPER_OK; ++t) rc = enqueue_ms_step(ms, exchange, measure, ms->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ms->cap_stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      hipGraphDestroy(g);
+      return fail(SCAMPER_EDEVICE, "graph instantiation: %s", hipGetErrorString(ei));
+    }
+    ms->graph[exchange != 0][measure != 0] = g;
+  }
+  *out = x;
+  return SCAMPER_OK;
+}
+
+}  // namespace
+
 int scamper_multispin_step(scamper_multispin *ms, uint64_t n_steps, int exchange, int measure) {
   if (!ms) return fail(SCAMPER_EINVAL, "null argument");
   if (!ms->tables_ready)
     if (int rc = upload_thresholds(ms)) return rc;
-  const bool record = measure || exchange;
   if (measure)
     if (int rc = ensure_ms_series(ms, n_steps)) return rc;
-  const double n_bonds = (double)ms->z * (double)ms->n_sites / 2.0;
-  const uint64_t ex_key = stream_seed(ms->seed, 0, 3);
-  for (uint64_t t = 0; t < n_steps; ++t) {
-    const uint64_t sweep = ms->sweeps + t;
-    for (uint32_t c = 0; c < 2; ++c) {
-      hipError_t e = launch_ms_sweep(ms->d_words, ms->d_thr, ms->d_labels, ms->d_keys, ms->geom, ms->n_bundles, c, sweep,
-                                     (record && c == 1) ? ms->d_counts : nullptr, ms->stream);
-      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "multispin sweep: %s", hipGetErrorString(e));
-    }
-    if (record) {
-      double *slot = measure ? ms->d_series + (ms->series_len + t) * 2ull * ms->n : nullptr;
-      const uint64_t step = exchange ? ms->exchange_steps + 1 : 0;
-      hipError_t e = launch_ms_exchange(ms->d_labels, ms->d_counts, ms->d_beta, ms->n, n_bonds, (double)ms->n_sites, ms->j,
-                                        ex_key, step, exchange != 0, slot, slot ? slot + ms->n : nullptr, ms->d_stats,
-                                        ms->stream);
-      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "multispin exchange: %s", hipGetErrorString(e));
-      if (exchange) ms->exchange_steps++;
-    }
+  if (n_steps == 0) return SCAMPER_OK;
+  if (!ms->d_clock) SC_HIP(hipMalloc(&ms->d_clock, sizeof(StepClock)));
+  // the clock starts where the host's counters are; whole chunks replay the
+  // captured graph, the rest are launched one by one
+  SC_HIP(launch_clock_set(ms->d_clock, ms->sweeps, exchange ? ms->exchange_steps + 1 : 0, ms->series_len,
+                          ms->d_series, ms->stream));
+  const uint64_t chunks = n_steps / kGraphSteps;
+  if (chunks) {
+    hipGraphExec_t x = nullptr;
+    if (int rc = ms_graph(ms, exchange, measure, &x)) return rc;
+    for (uint64_t k = 0; k < chunks; ++k) SC_HIP(hipGraphLaunch(x, ms->stream));
+    ms->graph_launches += chunks;
   }
+  for (uint64_t t = chunks * kGraphSteps; t < n_steps; ++t)
+    if (int rc = enqueue_ms_step(ms, exchange, measure, ms->stream)) return rc;
   ms->sweeps += n_steps;
+  if (exchange) ms->exchange_steps += n_steps;
   if (measure) ms->series_len += n_steps;
   return SCAMPER_OK;
 }

@@ -118,3 +118,49 @@ def test_set_beta_ends_tempering(sc):
     assert st.get_beta(0) == 1.0
     for r in range(1, 4):
         assert st.get_beta(r) == ladder[labels[r]]
+
+
+@pytest.mark.parametrize("case", ["on3", "ising_fast", "potts_tiles"])
+def test_pt_step_equals_sweep_then_exchange(sc, case):
+    """scamper_state_pt_step (captured graphs of 16 steps where the kernels
+    read the device clock, step by step elsewhere) equals the same steps made
+    one call at a time: sweep, then exchange round t + 1; spins, labels,
+    exchange energies and the measured series agree exactly."""
+    n_rep, steps, seed = 8, 37, 61
+    if case == "on3":
+        lat, m, alg, ladder = sc.Lattice.hypercubic([8, 8, 8]), sc.model(sc.ON, 1.0, 2, 3), sc.HYBRID_HB_OR, \
+            np.linspace(0.62, 0.74, n_rep)
+    elif case == "ising_fast":
+        lat, m, alg, ladder = sc.Lattice.square(64, 32), sc.model(sc.ISING), sc.METROPOLIS, np.linspace(0.38, 0.5, n_rep)
+    else:
+        lat, m, alg, ladder = sc.Lattice.square(20, 12), sc.model(sc.POTTS, 1.0, 3), sc.METROPOLIS, \
+            np.linspace(0.9, 1.1, n_rep)
+    a, b = sc.State(lat, m, n_rep, seed), sc.State(lat, m, n_rep, seed)
+    for st in (a, b):
+        st.set_ladder(ladder)
+        for r in range(n_rep):
+            st.init(True, r)
+    a.pt_step(alg, steps, measure=True)
+    energies, mags = [], []
+    for t in range(steps):
+        b.sweep(alg, 1)
+        b.exchange(t + 1)
+        energies.append(b.exchange_energies())
+        mags.append([b.observables(r)[1] for r in range(n_rep)])
+    # O(N) recounts add FP64 partial sums with atomics, in no fixed order:
+    # equal to the last bits; integer counts are exact
+    same = (lambda x, y: np.allclose(x, y, rtol=1e-12, atol=0)) if case == "on3" else np.array_equal
+    assert (a.pt_graph_launches() > 0) == (case != "potts_tiles")
+    assert np.array_equal(a.labels(), b.labels())
+    assert same(a.exchange_energies(), b.exchange_energies())
+    for r in range(n_rep):
+        if case == "on3":
+            assert np.array_equal(a.download_f32(r), b.download_f32(r)), f"replica {r}"
+        else:
+            assert np.array_equal(a.download_i8(r), b.download_i8(r)), f"replica {r}"
+        e, mm = a.series(r)
+        assert e.size == steps
+        assert same(e, [energies[t][r] for t in range(steps)]), f"replica {r}: series energies"
+        assert same(mm, [mags[t][r] for t in range(steps)]), f"replica {r}: series |m|"
+    assert a.sweeps == b.sweeps == steps
+    assert a.exchange_stats() == b.exchange_stats()

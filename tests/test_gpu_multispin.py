@@ -78,3 +78,25 @@ def test_exchange_trajectory_matches_oracle(sc, orc):
     plus = np.array([[((words >> np.uint64(r)) & np.uint64(1)).sum() for r in range(64)]])
     m_last = np.abs(2.0 * plus[0] - words.size) / words.size
     assert np.array_equal(m_dev[-1], m_last)
+
+
+def test_captured_steps_equal_single_steps(sc):
+    """Chunks of 16 steps replay one captured graph whose kernels read the
+    step counters from the device; 40 steps in one call (two chunks and 8
+    single steps) equal 40 calls of one step each: spins, labels, series and
+    exchange counters."""
+    lat = sc.Lattice.hypercubic([16, 16, 16])
+    betas = np.linspace(0.18, 0.26, 64)
+    a, b = sc.MultiSpin(lat, 1.0, 1, 3), sc.MultiSpin(lat, 1.0, 1, 3)
+    for ms in (a, b):
+        ms.set_betas(betas)
+        ms.init(True)
+    a.step(40, exchange=True, measure=True)
+    for _ in range(40):
+        b.step(1, exchange=True, measure=True)
+    assert np.array_equal(a.download(), b.download())
+    assert np.array_equal(a.labels(), b.labels())
+    ea, ma = a.series()
+    eb, mb = b.series()
+    assert ea.shape == eb.shape and np.array_equal(ea, eb) and np.array_equal(ma, mb)
+    assert a.stats() == b.stats()
