@@ -260,7 +260,7 @@ def serving(sc, args) -> dict:
     for rate in args.serve_rates:
         reqs = sv.potts_mix(args.serve_requests, rate, seed=11)
         res = sv.serve(reqs)
-        row = {"lambda_per_s": rate, **res.summary(reqs), "grid": res.grid}
+        row = {"lambda_per_s": rate, **res.summary(reqs), "launch_shape": res.shape}
         out["rates"].append(row)
         print(f"# serving lambda={rate}: {json.dumps(row)}", file=sys.stderr, flush=True)
     # Same requests both ways, all arriving at once; budgets a tenth of the

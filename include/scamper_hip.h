@@ -310,7 +310,9 @@ typedef struct {
   uint64_t spin_budget_bytes; /* device bytes for spins (one byte per site) */
   uint32_t max_jobs;          /* running jobs at once (1..4096) */
   uint32_t chunk_sweeps;      /* steps per graph launch (1..1024); 0 = 32 */
-  uint32_t grid;              /* blocks per launch, multiple of 8; 0 = autotune */
+  uint32_t grid;              /* blocks per launch, multiple of 8; 0 = autotune the grid, then
+                                 the sites per lane (4, 16 or by size), then the chunk length
+                                 (chunk_sweeps / 4, / 2 or chunk_sweeps), each on real chunks */
   uint32_t use_graphs;        /* 1 = hipGraph per chunk, 0 = plain launches */
 } scamper_pool_config;
 
@@ -339,6 +341,8 @@ typedef struct {
   uint64_t chunks;
   uint32_t grid;          /* blocks per launch in use (after autotuning) */
   uint32_t tuned;         /* 1 once the autotuner has settled */
+  uint32_t lane_sites;    /* sites per lane it settled on (0: by each launch's size) */
+  uint32_t chunk_sweeps;  /* steps per chunk it settled on */
 } scamper_pool_stats;
 
 int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out);

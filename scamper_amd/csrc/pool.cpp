@@ -37,7 +37,7 @@ constexpr uint32_t kRec = 8;   // counters per (job, step) record: em[0], em[1],
 constexpr uint32_t kSubs = 8;  // per-XCD partial records folded after the chunk
 constexpr uint32_t kMaxPoolQ = kRec - 2;
 constexpr uint32_t kGridCandidates[] = {256, 512, 1024, 2048, 4096};
-constexpr int kNumCandidates = sizeof(kGridCandidates) / sizeof(kGridCandidates[0]);
+constexpr uint32_t kLaneCandidates[] = {0, 4, 16};  // 0: by the launch's size (tile_lane_sites)
 constexpr int kTuneRounds = 2;  // timed chunks per candidate
 constexpr uint64_t kAlign = 256;
 constexpr uint32_t kMaxBypass = 64;  // later jobs admitted past a waiting head
@@ -71,7 +71,14 @@ struct InFlight {
   bool valid = false;
   std::vector<RowInfo> rows;
   uint64_t updates = 0;
+  uint32_t steps = 0;  // chunk length of this chunk
   int candidate = -1;  // autotuner candidate timed by this chunk
+};
+
+// Launch shape of a chunk: blocks per launch, sites per lane (0: by the
+// launch's size) and steps per chunk.
+struct Shape {
+  uint32_t grid, lane, chunk;
 };
 
 // First-fit allocator over [0, size).
@@ -139,11 +146,15 @@ struct scamper_pool {
   std::vector<uint64_t> running;    // admitted jobs with sweeps left to enqueue
   std::vector<uint64_t> completed;  // finished, not yet reported
   uint64_t n_submitted = 0, n_done = 0, updates = 0;
-  // autotuner
-  uint32_t grid = 0;
+  // autotuner: launch shapes timed on real chunks, one coordinate at a time
+  // (stage 0 the grid, 1 the lane width at the best grid, 2 the chunk length
+  // at the best grid and lane width); results never depend on the shape
+  Shape shape{0, 0, 0};  // in use (settled, or the fallback while tuning)
   bool tuned = false;
-  double cand_best[kNumCandidates] = {};
-  int cand_samples[kNumCandidates] = {};
+  int stage = 0;
+  std::vector<Shape> cands;
+  std::vector<double> cand_best;
+  std::vector<int> cand_samples;
   int next_candidate = 0;
 
   ~scamper_pool() {
@@ -170,15 +181,16 @@ struct scamper_pool {
 
 namespace {
 
-uint64_t ring_row_counters(const scamper_pool *P) { return (uint64_t)P->cfg.chunk_sweeps * kSubs * kRec; }
+// Counters of one job row in the ring of a chunk of K steps (the fold reads
+// rows of this length, so the layout follows each chunk's own K).
+uint64_t ring_row_counters(uint32_t K) { return (uint64_t)K * kSubs * kRec; }
 
 // The chunk's launches: zero the ring, K steps of (colour 0, colour 1), fold.
-int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, hipStream_t st) {
+int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, uint32_t K, hipStream_t st) {
   const TileHeader *hdr = reinterpret_cast<const TileHeader *>(P->d_table[p]);
   const TileJob *rows = reinterpret_cast<const TileJob *>(hdr + 1);
-  const uint32_t K = P->cfg.chunk_sweeps;
-  const EmLayout em{P->d_ring[p], ring_row_counters(P), (uint64_t)kSubs * kRec, kSubs - 1, kRec};
-  hipError_t e = launch_tile_zero(hdr, P->d_ring[p], ring_row_counters(P), st);
+  const EmLayout em{P->d_ring[p], ring_row_counters(K), (uint64_t)kSubs * kRec, kSubs - 1, kRec};
+  hipError_t e = launch_tile_zero(hdr, P->d_ring[p], ring_row_counters(K), st);
   for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
     e = launch_tile_sweep(hdr, rows, grid, lane, k, 0, 0, em, st);
     if (e == hipSuccess) e = launch_tile_sweep(hdr, rows, grid, lane, k, 1, 1, em, st);
@@ -188,17 +200,17 @@ int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, hip
   return SCAMPER_OK;
 }
 
-// The captured chunk for launch shape (grid, lane) on table buffer p,
+// The captured chunk for launch shape (grid, lane, K) on table buffer p,
 // captured on first use.
-int chunk_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane, hipGraphExec_t *exec) {
-  const uint64_t key = ((uint64_t)grid << 8) | lane;
+int chunk_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane, uint32_t K, hipGraphExec_t *exec) {
+  const uint64_t key = ((uint64_t)grid << 32) | ((uint64_t)K << 8) | lane;
   auto it = P->graphs[p].find(key);
   if (it != P->graphs[p].end()) {
     *exec = it->second.second;
     return SCAMPER_OK;
   }
   SC_HIP(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_chunk_body(P, p, grid, lane, P->cap_stream);
+  const int rc = enqueue_chunk_body(P, p, grid, lane, K, P->cap_stream);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(P->cap_stream, &g);
   if (rc) {
@@ -256,6 +268,28 @@ void finish(scamper_pool *P, Job &J) {
   ++P->n_done;
 }
 
+// Candidates of tuning stage `stage` around the current shape; past the last
+// stage the shape is settled.
+void start_stage(scamper_pool *P, int stage) {
+  P->stage = stage;
+  P->cands.clear();
+  const Shape s = P->shape;
+  if (stage == 0)
+    for (uint32_t g : kGridCandidates) P->cands.push_back(Shape{g, s.lane, s.chunk});
+  else if (stage == 1)
+    for (uint32_t l : kLaneCandidates) P->cands.push_back(Shape{s.grid, l, s.chunk});
+  else if (stage == 2)
+    for (uint32_t k : {std::max(1u, P->cfg.chunk_sweeps / 4), std::max(1u, P->cfg.chunk_sweeps / 2), P->cfg.chunk_sweeps})
+      if (P->cands.empty() || P->cands.back().chunk != k) P->cands.push_back(Shape{s.grid, s.lane, k});
+  if (P->cands.empty()) {
+    P->tuned = true;
+    return;
+  }
+  P->cand_best.assign(P->cands.size(), 0.0);
+  P->cand_samples.assign(P->cands.size(), 0);
+  P->next_candidate = 0;
+}
+
 int collect(scamper_pool *P, int p, uint32_t *done) {
   InFlight &F = P->inflight[p];
   SC_HIP(hipEventSynchronize(P->ev_end[p]));
@@ -268,7 +302,7 @@ int collect(scamper_pool *P, int p, uint32_t *done) {
       P->cand_samples[F.candidate] += 1;
     }
   }
-  const uint32_t K = P->cfg.chunk_sweeps;
+  const uint32_t K = F.steps;
   for (size_t r = 0; r < F.rows.size(); ++r) {
     const RowInfo &ri = F.rows[r];
     Job &J = *P->jobs.at(ri.id);
@@ -289,16 +323,16 @@ int collect(scamper_pool *P, int p, uint32_t *done) {
   }
   F.valid = false;
   F.rows.clear();
-  // settle the autotuner once every candidate has been timed
-  if (!P->tuned && P->cfg.grid == 0) {
+  // close the tuning stage once every candidate has been timed
+  if (!P->tuned && !P->cands.empty()) {
     bool all = true;
-    for (int c = 0; c < kNumCandidates; ++c) all = all && P->cand_samples[c] >= kTuneRounds;
+    for (size_t c = 0; c < P->cands.size(); ++c) all = all && P->cand_samples[c] >= kTuneRounds;
     if (all) {
-      int best = 0;
-      for (int c = 1; c < kNumCandidates; ++c)
+      size_t best = 0;
+      for (size_t c = 1; c < P->cands.size(); ++c)
         if (P->cand_best[c] > P->cand_best[best]) best = c;
-      P->grid = kGridCandidates[best];
-      P->tuned = true;
+      P->shape = P->cands[best];
+      start_stage(P, P->stage + 1);
     }
   }
   return SCAMPER_OK;
@@ -345,19 +379,21 @@ int admit_queued(scamper_pool *P) {
   return SCAMPER_OK;
 }
 
-uint32_t choose_grid(scamper_pool *P, int *candidate) {
+// The shape of the next chunk: the settled one, or a candidate of the
+// current tuning stage that still needs timing (*candidate = its index).
+Shape choose_shape(scamper_pool *P, int *candidate) {
   *candidate = -1;
-  if (P->cfg.grid) return P->cfg.grid;
-  if (P->tuned) return P->grid;
-  for (int tries = 0; tries < kNumCandidates; ++tries) {
+  if (P->tuned || P->cands.empty()) return P->shape;
+  const int n = (int)P->cands.size();
+  for (int tries = 0; tries < n; ++tries) {
     const int c = P->next_candidate;
-    P->next_candidate = (P->next_candidate + 1) % kNumCandidates;
+    P->next_candidate = (P->next_candidate + 1) % n;
     if (P->cand_samples[c] < kTuneRounds) {
       *candidate = c;
-      return kGridCandidates[c];
+      return P->cands[c];
     }
   }
-  return P->grid ? P->grid : 1024;
+  return P->shape;
 }
 
 int step(scamper_pool *P, uint32_t *n_completed) {
@@ -380,8 +416,10 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   }
   if (int rc = admit_queued(P)) return rc;
 
-  // the job table of this chunk
-  const uint32_t K = P->cfg.chunk_sweeps;
+  // the job table of this chunk, in the chunk's launch shape
+  int candidate;
+  const Shape shape = choose_shape(P, &candidate);
+  const uint32_t K = shape.chunk;
   TileHeader *hdr = reinterpret_cast<TileHeader *>(P->h_table[p]);
   TileJob *rows = reinterpret_cast<TileJob *>(hdr + 1);
   InFlight &F = P->inflight[p];
@@ -390,7 +428,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   uint32_t tiles = 0;
   uint64_t total_sites = 0;
   for (uint64_t id : P->running) total_sites += P->jobs.at(id)->geom.nh;
-  const uint32_t lane = tile_lane_sites(total_sites);
+  const uint32_t lane = shape.lane ? shape.lane : tile_lane_sites(total_sites);
   for (uint64_t id : P->running) {
     Job &J = *P->jobs.at(id);
     const uint32_t steps = (uint32_t)std::min<uint64_t>(K, J.total - J.enqueued);
@@ -428,8 +466,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   hdr->n_jobs = (uint32_t)F.rows.size();
   hdr->n_tiles = tiles;
   hdr->pad[0] = hdr->pad[1] = 0;
-  int candidate;
-  uint32_t grid = choose_grid(P, &candidate);
+  uint32_t grid = shape.grid;
   // no more blocks than tiles (rounded up to a power of two, so that few
   // launch shapes and captured graphs arise as the table shrinks and grows)
   uint32_t fit = 8;
@@ -437,13 +474,13 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   grid = std::min(grid, fit);
   hipGraphExec_t exec = nullptr;
   if (P->cfg.use_graphs)
-    if (int rc = chunk_graph(P, p, grid, lane, &exec)) return rc;
+    if (int rc = chunk_graph(P, p, grid, lane, K, &exec)) return rc;
   SC_HIP(hipMemcpyAsync(P->d_table[p], P->h_table[p], sizeof(TileHeader) + F.rows.size() * sizeof(TileJob),
                         hipMemcpyHostToDevice, P->stream));
   SC_HIP(hipEventRecord(P->ev_start[p], P->stream));
   if (P->cfg.use_graphs) {
     SC_HIP(hipGraphLaunch(exec, P->stream));
-  } else if (int rc = enqueue_chunk_body(P, p, grid, lane, P->stream)) {
+  } else if (int rc = enqueue_chunk_body(P, p, grid, lane, K, P->stream)) {
     return rc;
   }
   SC_HIP(hipEventRecord(P->ev_graph[p], P->stream));
@@ -461,6 +498,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   }
   SC_HIP(hipEventRecord(P->ev_end[p], P->stream));
   F.valid = true;
+  F.steps = K;
   F.candidate = candidate;
   P->updates += F.updates;
   ++P->chunk;
@@ -488,7 +526,13 @@ int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out) {
   if (!P) return fail(SCAMPER_ENOMEM, "pool handle");
   c.spin_budget_bytes = (c.spin_budget_bytes + kAlign - 1) / kAlign * kAlign;
   P->cfg = c;
-  P->grid = c.grid;
+  // a configured grid fixes the shape (lanes by size, the configured
+  // chunk); otherwise the autotuner starts from 1024 blocks
+  P->shape = Shape{c.grid ? c.grid : 1024u, 0u, c.chunk_sweeps};
+  if (c.grid)
+    P->tuned = true;
+  else
+    start_stage(P.get(), 0);
   SC_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
   SC_HIP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
   if (hipMalloc(&P->spins_alloc, c.spin_budget_bytes + 2 * kSpinSlack) != hipSuccess)
@@ -624,8 +668,10 @@ int scamper_pool_stats_get(const scamper_pool *P, scamper_pool_stats *out) {
   out->spin_bytes_in_use = P->in_use;
   out->site_updates = P->updates;
   out->chunks = P->chunk;
-  out->grid = P->cfg.grid ? P->cfg.grid : (P->tuned ? P->grid : 0);
+  out->grid = P->tuned ? P->shape.grid : 0;
   out->tuned = P->tuned ? 1u : 0u;
+  out->lane_sites = P->tuned ? P->shape.lane : 0;
+  out->chunk_sweeps = P->tuned ? P->shape.chunk : 0;
   return SCAMPER_OK;
 }
 

@@ -96,7 +96,7 @@ class ServeResult:
     updates: int
     latencies: np.ndarray
     estimates: dict = field(default_factory=dict)
-    grid: int = 0
+    shape: dict = field(default_factory=dict)  # launch shape the pool's autotuner settled on
 
     @property
     def flips_per_ns(self) -> float:
@@ -142,7 +142,10 @@ def serve(requests: list[Request], spin_budget_bytes: int = 8 << 30, max_jobs: i
             time.sleep(max(0.0, min(requests[i].arrival - now, 0.002)))
     wall = time.perf_counter() - t0
     lat = done_at - np.array([r.arrival for r in requests])
-    return ServeResult(n, wall, sum(r.updates for r in requests), lat, estimates, pool.stats().grid)
+    st = pool.stats()
+    shape = {"grid": st.grid, "lane_sites": st.lane_sites or "by size", "chunk_sweeps": st.chunk_sweeps} \
+        if st.tuned else {"tuned": False}
+    return ServeResult(n, wall, sum(r.updates for r in requests), lat, estimates, shape)
 
 
 def plain(requests: list[Request], keep: int = 0) -> ServeResult:
