@@ -94,6 +94,7 @@ def _declare(L: C.CDLL) -> None:
         "orc_binned_estimate": (Estimate, [vp, u64, u64]),
         "orc_run_one_ising": (u64, [vp, dbl, dbl, i32, u64, u64, u64, u64, vp, vp, vp]),
         "orc_plain_sweeps": (u64, [vp, dbl, dbl, u64, u64]),
+        "orc_plain_chains": (u64, [vp, dbl, dbl, u64, u64, C.c_int, C.c_int]),
         "orc_xoshiro_seq": (None, [u64, u64, vp]),
         "orc_cluster_word": (u32, [u64, u64, u64, u32]),
         "orc_cluster_threshold": (u64, [i32, dbl, dbl, dbl]),
@@ -326,6 +327,10 @@ def run_one_ising(lat: Lattice, j, beta, initial_state, base_seed, therm, meas, 
 
 def plain_sweeps(lat: Lattice, j, beta, base_seed, n_sweeps) -> int:
     return lib().orc_plain_sweeps(lat.ptr, j, beta, base_seed, n_sweeps)
+
+
+def plain_chains(lat: Lattice, j, beta, base_seed, n_sweeps, n_chains, threads) -> int:
+    return lib().orc_plain_chains(lat.ptr, j, beta, base_seed, n_sweeps, n_chains, threads)
 
 
 def xoshiro_seq(seed: int, n: int) -> np.ndarray:

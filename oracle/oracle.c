@@ -1346,6 +1346,22 @@ uint64_t orc_plain_sweeps(const orc_lattice *lat, double j, double beta, uint64_
   return n * n_sweeps;
 }
 
+/* Scheduler::run_parallel (Carlo.rs/src/scheduler.rs:345-405): independent
+ * plain chains, one per task on its own thread, each as orc_plain_sweeps
+ * with base seed base_seed + chain, over one shared read-only lattice.
+ * Returns the attempted updates of all chains. */
+uint64_t orc_plain_chains(const orc_lattice *lat, double j, double beta, uint64_t base_seed, uint64_t n_sweeps,
+                          int n_chains, int threads) {
+  uint64_t total = 0;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+#pragma omp parallel for schedule(static, 1) reduction(+ : total)
+  for (int c = 0; c < n_chains; ++c) total += orc_plain_sweeps(lat, j, beta, base_seed + (uint64_t)c, n_sweeps);
+  return total;
+}
 
 /* ----------------------------------------------------- cluster updates */
 

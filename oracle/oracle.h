@@ -202,6 +202,11 @@ uint64_t orc_run_one_ising(const orc_lattice *lat, double j, double beta, int in
 /* Time-only variant for the CPU baseline: n sweeps of the same kernel. */
 uint64_t orc_plain_sweeps(const orc_lattice *lat, double j, double beta, uint64_t base_seed,
                           uint64_t n_sweeps);
+/* Independent plain chains on `threads` threads (Scheduler::run_parallel,
+ * Carlo.rs/src/scheduler.rs:345-405): chain c as orc_plain_sweeps with base
+ * seed base_seed + c, over one shared lattice; returns all attempted updates. */
+uint64_t orc_plain_chains(const orc_lattice *lat, double j, double beta, uint64_t base_seed, uint64_t n_sweeps,
+                          int n_chains, int threads);
 
 /* xoshiro256++ seeded like rand_xoshiro's seed_from_u64 (for tests). */
 void orc_xoshiro_seq(uint64_t seed, uint64_t n, uint64_t *out);
