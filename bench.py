@@ -43,43 +43,68 @@ METRIC = "spin-flips/ns (Ising 2D L=16384); batched-queue sweeps/s + p99 latency
 HBM_PEAK_GBS = 8000.0
 
 
-def traffic_from_profile(path: str):
-    """HBM bytes per half-sweep launch from a committed rocprofv3 --pmc summary
-    (line 'traffic_bytes_per_launch: <int>'), or None."""
+# wave64 VALU instructions issued per second by the whole chip: 256 CUs x 4
+# SIMDs at 2.4 GHz, one wave64 instruction per SIMD every 4 cycles
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def figure_from_profile(path: str, key: str):
+    """An integer figure from a committed rocprofv3 summary (a line
+    '<key>: <int>'), or None."""
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return int(line.split(":", 1)[1].strip())
     except OSError:
         pass
     return None
 
 
-def cpu_baseline(seconds_hint: float = 15.0):
+def traffic_from_profile(path: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary (line
+    'traffic_bytes_per_launch: <int>'), or None."""
+    return figure_from_profile(path, "traffic_bytes_per_launch")
+
+
+def host_threads() -> int:
+    """Host cores this process may use: OMP_NUM_THREADS where it is set (the
+    CPU share given to the job), else the affinity mask."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return n if n > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(single_sweeps: int = 2):
     """The oracle's restatement of the reference's plain sweep (f64 spins,
-    usize CSR, 32-byte bonds, Fisher-Yates order, xoshiro256++), one thread,
-    on a 4096 x 4096 periodic lattice (same per-site work and the same
-    out-of-cache access pattern as 16384^2, whose reference layout needs
-    ~38 GB)."""
+    usize CSR, 32-byte bonds, Fisher-Yates order, xoshiro256++) on a 4096 x
+    4096 periodic lattice (same per-site work and the same out-of-cache
+    access pattern as 16384^2, whose reference layout needs ~38 GB).
+
+    The reference's only parallelism is independent chains, one per worker
+    thread (Scheduler::run_parallel, Carlo.rs/src/scheduler.rs:345-405), so
+    the value is that: one chain per host core, one sweep each, all at
+    once, over one shared lattice.  The single-thread rate of the same sweep
+    is reported beside it."""
     import oracle
 
     lat = oracle.Lattice.square(4096, 4096, True)
     beta = 1.0 / 2.269185
-    oracle.plain_sweeps(lat, 1.0, beta, 42, 1)  # touch the pages
     t0 = time.perf_counter()
-    sweeps = 0
-    flips = 0
-    while True:
-        flips += oracle.plain_sweeps(lat, 1.0, beta, 42 + sweeps, 1)
-        sweeps += 1
-        if time.perf_counter() - t0 > seconds_hint or sweeps >= 8:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": flips / (dt * 1e9), "unit": "spin-flips/ns", "cores": 1, "kind": "port",
-            "sample": f"{sweeps} sweeps of the faithful-layout plain Metropolis path on a "
-                      f"4096^2 periodic lattice (oracle.c orc_plain_sweeps), 1 thread, "
-                      f"{dt:.1f} s; host nproc={os.cpu_count()}"}
+    flips = sum(oracle.plain_sweeps(lat, 1.0, beta, 42 + k, 1) for k in range(single_sweeps))
+    dt1 = time.perf_counter() - t0
+    threads = host_threads()
+    t0 = time.perf_counter()
+    flips_all = oracle.plain_chains(lat, 1.0, beta, 1000, 1, threads, threads)
+    dta = time.perf_counter() - t0
+    return {"value": flips_all / (dta * 1e9), "unit": "spin-flips/ns", "cores": threads, "kind": "port",
+            "sample": f"{threads} independent chains on {threads} threads, one sweep each of the faithful-layout "
+                      f"plain Metropolis path on a 4096^2 periodic lattice (oracle.c orc_plain_chains), "
+                      f"{dta:.1f} s; host nproc={os.cpu_count()}",
+            "single_thread": {"value": flips / (dt1 * 1e9), "cores": 1,
+                              "sample": f"{single_sweeps} sweeps of the same path on one thread, {dt1:.1f} s"}}
 
 
 def _cpu_timed(step, units_per_call: float, unit: str, what: str, seconds: float = 6.0, max_calls: int = 400) -> dict:
@@ -128,6 +153,42 @@ def cpu_on3() -> dict:
         ch.over_relaxation()
     return _cpu_timed(step, 2 * lat.n_sites, "site-updates/ns",
                       "heat-bath + over-relaxation sweeps of a 24^3 periodic lattice (f64, CSR)")
+
+
+def on3_oracle_error(sc, st, L: int, betas, seed: int, replicas=(0, 15, 31)) -> dict:
+    """Error of the FP32 device path against the FP64 oracle at the O(3)
+    workload's own shape: after the timed sweeps, replicas first, middle and
+    last are read back, one more hybrid sweep (heat bath, models.rs:617-662,
+    then over-relaxation, microcanonical.rs:42-80) runs on the device with
+    its fused FP64 measurement, and the oracle replays that sweep from the
+    same spins with the same draws.  Per replica: the largest and the mean
+    per-component spin error, and the errors of E (relative) and |M|
+    (absolute, |M| per site)."""
+    import numpy as np
+
+    import oracle
+
+    olat = oracle.Lattice.hypercubic([L, L, L], True)
+    m = oracle.model(oracle.ON, 1.0, 2, 3)
+    starts = {r: st.download_f64(r) for r in replicas}
+    sweep = st.sweeps
+    st.sweep(sc.HYBRID_HB_OR, 1, measure=True)
+    rows = []
+    for r in replicas:
+        ch = oracle.Chain(olat, m, starts[r].copy())
+        ch.on3_heat_bath(betas[r], oracle.replica_key(seed, r), sweep)
+        ch.over_relaxation()
+        err = np.abs(st.download_f64(r) - ch.spins)
+        e, mag = st.series(r)
+        e_ref, m_ref = ch.total_energy(), ch.magnetization()
+        rows.append({"replica": r, "T": round(1.0 / betas[r], 4), "spin_max_abs": float(err.max()),
+                     "spin_mean_abs": float(err.mean()), "energy_rel": float(abs(e[-1] - e_ref) / abs(e_ref)),
+                     "abs_m_abs": float(abs(mag[-1] - m_ref))})
+    return {"oracle": "orc_sweep_on3_heat_bath + orc_sweep_over_relaxation, f64, same draws and start",
+            "sweep": sweep, "replicas": rows,
+            "spin_max_abs": max(x["spin_max_abs"] for x in rows),
+            "energy_rel_max": max(x["energy_rel"] for x in rows),
+            "abs_m_abs_max": max(x["abs_m_abs"] for x in rows)}
 
 
 def cpu_xy() -> dict:
@@ -301,6 +362,12 @@ def pt3d(sc, args) -> dict:
     flips = 64 * n * args.pt_steps
     acc, ties, ex = ms_.stats()
     bytes_per_step = 3 * 8 * n  # u64 words: read own + read other + write own colour, both colours
+    # the state (16 MiB) stays in the caches; the bound is VALU issue, with
+    # the instructions per step (both colours and the exchange) counted by
+    # rocprofv3 on this workload
+    valu_per_step = figure_from_profile(os.path.join(ROOT, "profiles", "pt3d_L128_multispin_kernel_trace.txt"),
+                                        "valu_instructions_per_step")
+    valu_rate = valu_per_step and valu_per_step * args.pt_steps / (ms * 1e-3)
     return {"metric": "spin-flips/ns (3-D Ising L=128, 64-replica multi-spin PT)",
             "value": round(flips / (ms * 1e6), 3), "unit": "spin-flips/ns", "n_gpus": 1,
             "steps": args.pt_steps, "warmup": args.warmup, "ms_per_step": ms / args.pt_steps,
@@ -308,7 +375,10 @@ def pt3d(sc, args) -> dict:
             "config": {"workload": "pt3d_multispin", "L": L, "replicas": 64, "T": [4.30, 4.72],
                        "ladder": "geometric", "seed": 42, "exchanges_accepted": acc,
                        "near_tie_flags": ties, "exchange_steps": ex},
-            "roofline": {"bound": "valu (cache-resident state)",
+            "roofline": {"bound": "valu", "achieved": valu_rate and round(valu_rate / 1e12, 4),
+                         "peak": round(VALU_PEAK_WAVE_INSTS / 1e12, 4), "unit": "T wave64 VALU instructions/s",
+                         "frac": valu_rate and round(valu_rate / VALU_PEAK_WAVE_INSTS, 4),
+                         "valu_instructions_per_step": valu_per_step,
                          "hbm_equivalent_GBps": round(bytes_per_step * args.pt_steps / (ms * 1e-3) / 1e9, 1)},
             **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_pt3d()})}
 
@@ -319,9 +389,10 @@ def on3(sc, args) -> dict:
     L, R = 96, 32
     lat = sc.Lattice.hypercubic([L, L, L], True)
     st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 3), R, 42)
+    betas = [1.0 / (1.40 + 0.09 * r / 31) for r in range(R)]
     for r in range(R):
         st.init(True, r)
-        st.set_beta(1.0 / (1.40 + 0.09 * r / 31), r)
+        st.set_beta(betas[r], r)
     st.sweep(sc.HYBRID_HB_OR, args.warmup)
     st.synchronize()
     ms = st.sweep_timed(sc.HYBRID_HB_OR, args.on3_steps)
@@ -338,7 +409,8 @@ def on3(sc, args) -> dict:
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4),
                          "traffic": traffic_from_profile(os.path.join(ROOT, "profiles", "on3_L96_pmc.txt"))},
-            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_on3()})}
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_on3(),
+                                                "fp64_oracle_error": on3_oracle_error(sc, st, L, betas, 42)})}
 
 
 def xy(sc, args) -> dict:
@@ -544,8 +616,9 @@ def main() -> int:
                     default="ising2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[10.0, 1000.0])
+    ap.add_argument("--no-sublines", action="store_true", help="leave the pt3d and on3 lines out of the default line")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[10.0, 30.0, 100.0, 300.0, 1000.0])
     ap.add_argument("--plain-requests", type=int, default=48)
     ap.add_argument("--pt-steps", type=int, default=10_000)
     ap.add_argument("--on3-steps", type=int, default=200)
@@ -581,6 +654,9 @@ def main() -> int:
                 "serving": serving(sc, args)}
     else:
         line = ising2d(sc, args)
+        if not args.no_sublines:
+            line["pt3d"] = pt3d(sc, args)
+            line["on3"] = on3(sc, args)
         if not args.no_serving:
             line["serving"] = serving(sc, args)
         if not args.no_cpu_baseline:

@@ -14,7 +14,9 @@
 //   1 + cos =  log1p(u * expm1(2 kappa)) / kappa,
 // algebraically the reference's 1 + ln(u + (1 - u) e^{-2 kappa}) / kappa but
 // without its FP32 cancellation at small kappa and in the tails, and
-// sin(theta) = sqrt((1 - cos)(1 + cos)) without that of 1 - cos^2.
+// sin(theta) = sqrt((1 - cos)(1 + cos)) without that of 1 - cos^2.  Each
+// draw evaluates one logarithm, whichever form it takes, so a wave does not
+// run both forms when its lanes differ.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -113,23 +115,34 @@ __device__ __forceinline__ V3 heat_bath_site(V3 h, const ReplicaParams &P, uint6
     return random_unit(philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
                                      (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32)));
   const float u = fmaxf((float)w0 * 2.3283064365386963e-10f, 1.17549435e-38f);
-  // a = 1 - cos(theta) and b = 1 + cos(theta), each formed where it is the
-  // small one: a = -log1p((1 - u) expm1(-2 kappa)) / kappa near the field,
-  // b = log1p(u expm1(2 kappa)) / kappa opposite it (the same inverse CDF;
-  // the first form's argument approaches -1 there and cancels in FP32), so
-  // cos(theta) and sin(theta) = sqrt(a b) keep their relative precision at
-  // both ends
-  float a = fminf(fmaxf(-log1pf((1.0f - u) * expm1f(-2.0f * kappa)) / kappa, 0.0f), 2.0f), b;
-  if (a > 1.0f) {
-    b = kappa < 40.0f ? log1pf(u * expm1f(2.0f * kappa)) / kappa
-                      : 2.0f + logf(u + (1.0f - u) * expf(-2.0f * kappa)) / kappa;
-    b = fminf(fmaxf(b, 0.0f), 2.0f);
-    a = 2.0f - b;
-  } else {
-    b = 2.0f - a;
+  // With E = e^{2 kappa} - 1 and p = u + (1 - u) e^{-2 kappa} = (1 + u E) / (1 + E),
+  // cos(theta) = 1 + ln(p) / kappa.  The small one of a = 1 - cos and
+  // b = 1 + cos is formed, so cos(theta) and sin(theta) = sqrt(a b) keep
+  // their relative precision at both ends:
+  //   cos <  0:             b = log1p(u E) / kappa;
+  //   cos >= 0, p >= 1/2:   a = -log1p((1 - u) expm1(-2 kappa)) / kappa;
+  //   cos >= 0, p <  1/2:   a = 2 - log1p(u E) / kappa (the argument above
+  //                         would be near -1, where it cancels; a > ln 2 / kappa
+  //                         here, so the difference does not).
+  // cos < 0 exactly when u < 1 / (1 + e^kappa); the choice needs no
+  // precision, both sides being accurate near cos = 0.  Every lane takes one
+  // log1p.
+  const float E = expm1f(2.0f * kappa);  // inf from kappa ~ 44.4
+  const bool tail = u * (1.0f + __expf(kappa)) < 1.0f;
+  const float em = kappa < 40.0f ? -E * __builtin_amdgcn_rcpf(1.0f + E) : -1.0f;  // expm1(-2 kappa)
+  const float y = (1.0f - u) * em;
+  const bool via_e = tail || y < -0.5f;
+  const float l = log1pf(via_e ? u * E : y) / kappa;
+  float x = tail ? l : (via_e ? 2.0f - l : -l);  // the small one of a and b
+  if (via_e && !(E <= 3.4028235e38f)) {
+    // e^{2 kappa} beyond FP32 range (kappa > 44.3): ln p formed directly;
+    // u dominates p here unless u is zero's stand-in
+    const float lp = logf(u + (1.0f - u) * expf(-2.0f * kappa)) / kappa;
+    x = tail ? 2.0f + lp : -lp;
   }
-  const float c = a <= 1.0f ? 1.0f - a : b - 1.0f;
-  const float sn = sqrtf(a * b);
+  x = fminf(fmaxf(x, 0.0f), 2.0f);
+  const float c = tail ? x - 1.0f : 1.0f - x;
+  const float sn = sqrtf(x * (2.0f - x));
   float sphi, cphi;
   sincos_turns((float)w1 * 2.3283064365386963e-10f, &sphi, &cphi);
   const float lx = sn * cphi, ly = sn * sphi;
