@@ -61,9 +61,10 @@ __device__ __forceinline__ uint32_t propose2(uint32_t top, uint32_t mid, uint32_
   return (uint32_t)(((b2 | site_low(key, j, sweep, colour, kStreamPropose)) * m) >> 32);
 }
 
+// Per-wave partial sums of a flush, double-buffered so that consecutive
+// flushes need one barrier each.
 struct TileLds {
-  Threshold thr[kStencilTable];
-  unsigned long long red[2 + kMaxQ];
+  int part[2][kThreads / 64][2 + kMaxQ];
 };
 
 __device__ __forceinline__ uint32_t byte4(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
@@ -153,14 +154,14 @@ __device__ __forceinline__ void fold_nibbles(Meas &M, const Nibbles &B) {
 // One site: new spin from its own spin, its z neighbours and its draws.
 template <int ND, int KIND>
 __device__ __forceinline__ int8_t update_site(int8_t s, const int8_t (&n)[2 * ND], uint32_t top, uint32_t mid,
-                                              uint32_t ptop, uint32_t pmid, const TileLds &L, uint32_t q,
+                                              uint32_t ptop, uint32_t pmid, const Threshold *thr, uint32_t q,
                                               uint64_t key, uint32_t j, uint64_t sweep, uint32_t colour) {
   constexpr int z = 2 * ND;
   if (KIND == kIsing) {
     int anti = 0;
 #pragma unroll
     for (int c = 0; c < z; ++c) anti += (n[c] != s);
-    return decide2(top, mid, L.thr[anti], key, j, sweep, colour) ? (int8_t)-s : s;
+    return decide2(top, mid, thr[anti], key, j, sweep, colour) ? (int8_t)-s : s;
   }
   uint32_t r;
   if (q == 2) {
@@ -172,7 +173,7 @@ __device__ __forceinline__ int8_t update_site(int8_t s, const int8_t (&n)[2 * ND
   int d = 0;
 #pragma unroll
   for (int c = 0; c < z; ++c) d += ((uint32_t)n[c] == r) - (n[c] == s);
-  return decide2(top, mid, L.thr[d + z], key, j, sweep, colour) ? (int8_t)r : s;
+  return decide2(top, mid, thr[d + z], key, j, sweep, colour) ? (int8_t)r : s;
 }
 
 template <int ND, int KIND>
@@ -260,7 +261,7 @@ __device__ __forceinline__ uint32_t lut16(const uint32_t (&t)[4], uint32_t idx) 
 // every draw with top bytes 0xffff where K = 2^32) are settled exactly.
 template <bool SMALL_INDEX>
 __device__ __forceinline__ uint32_t accept_bytes(uint32_t idx, const LaneDraws &D, const TileJob &J,
-                                                 const TileLds &L, uint64_t key, uint32_t j0, uint64_t sweep,
+                                                 uint64_t key, uint32_t j0, uint64_t sweep,
                                                  uint32_t colour) {
   uint32_t k8, m8;
   if (SMALL_INDEX) {  // every index < 8
@@ -276,7 +277,7 @@ __device__ __forceinline__ uint32_t accept_bytes(uint32_t idx, const LaneDraws &
   while (tie) {
     const uint32_t k = (uint32_t)__builtin_ctz(tie) >> 3;
     tie &= tie - 1;
-    const Threshold &t = L.thr[byte4(idx, (int)k)];
+    const Threshold &t = J.thr[byte4(idx, (int)k)];
     if (t.k8 > 0xFFu || site_low(key, j0 + k, sweep, colour, kStreamAccept) < t.k16) acc |= 0x80u << (8u * k);
   }
   return acc;
@@ -289,14 +290,14 @@ __device__ __forceinline__ uint32_t ne16(uint32_t a, uint32_t b) { return ((a ^ 
 
 template <int ND>
 __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
-                                               const LaneDraws &D, const TileJob &J, const TileLds &L, uint64_t key,
+                                               const LaneDraws &D, const TileJob &J, uint64_t key,
                                                uint32_t j0, uint64_t sweep, uint32_t colour, bool meas, Meas &M) {
   constexpr int z = 2 * ND;
   uint32_t a2 = 0;  // spins are 0x01 / 0xff: bit 1 of s ^ n is set exactly when they differ
 #pragma unroll
   for (int c = 0; c < z; ++c) a2 += (S ^ N[c]) & 0x02020202u;
   const uint32_t anti = a2 >> 1;
-  const uint32_t flip = spread_msb(accept_bytes<true>(anti, D, J, L, key, j0, sweep, colour));
+  const uint32_t flip = spread_msb(accept_bytes<true>(anti, D, J, key, j0, sweep, colour));
   const uint32_t NS = S ^ (flip & 0xFEFEFEFEu);
   if (meas) {
     // a flipped site has z - anti anti-aligned neighbours
@@ -309,7 +310,7 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2
 
 template <int ND>
 __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
-                                               const LaneDraws &D, const TileJob &J, const TileLds &L, uint32_t q,
+                                               const LaneDraws &D, const TileJob &J, uint32_t q,
                                                uint64_t key, uint32_t j0, uint64_t sweep, uint32_t colour, bool meas,
                                                Meas &M, Nibbles &B) {
   constexpr int z = 2 * ND;
@@ -355,7 +356,7 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
     // 16 z + ds - dr lies in [0, 32 z] per byte, a multiple of 16
     idx = ((uint32_t)(16 * z) * kOnes + ds - dr) >> 4;
   }
-  const uint32_t am = spread_msb(accept_bytes<ND == 1>(idx, D, J, L, key, j0, sweep, colour));  // 0xff where accepted
+  const uint32_t am = spread_msb(accept_bytes<ND == 1>(idx, D, J, key, j0, sweep, colour));  // 0xff where accepted
   const uint32_t NS = pick(R, S, am);
   if (meas) {
     // bonds to unlike neighbours after the update: those of r where accepted
@@ -443,7 +444,7 @@ struct LaneSpan {
 // One tile of 1024 sites, 4 per lane (quad-shared Philox group); measured
 // counters are added to the lane's M.
 template <int ND, int KIND>
-__device__ __forceinline__ void run_tile4(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
+__device__ __forceinline__ void run_tile4(const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
                                          bool meas, Meas &M) {
   constexpr int z = 2 * ND;
   const uint32_t nh = J.nh;
@@ -453,9 +454,6 @@ __device__ __forceinline__ void run_tile4(TileLds &L, const TileJob &J, uint32_t
   const uint32_t m = q - 1;
   const bool need_prop = KIND == kPotts && q > 2;
   const bool need_pmid = need_prop && (m & (m - 1)) != 0;
-
-  if (threadIdx.x < kStencilTable) L.thr[threadIdx.x] = J.thr[threadIdx.x];
-  __syncthreads();
 
   // the quad's group must be drawn by all four lanes even where some of
   // them are past the end of the colour class
@@ -497,8 +495,8 @@ __device__ __forceinline__ void run_tile4(TileLds &L, const TileJob &J, uint32_t
 #pragma unroll
       for (int c = 0; c < z - 2; ++c) N[2 + c] = pick(Na[c], Nb[c], amask);
       Nibbles B;
-      const uint32_t NS = KIND == kIsing ? ising_word<ND>(O, N, C, D, J, L, key, j0, sweep, colour, meas, M)
-                                         : potts_word<ND>(O, N, C, D, J, L, q, key, j0, sweep, colour, meas, M, B);
+      const uint32_t NS = KIND == kIsing ? ising_word<ND>(O, N, C, D, J, key, j0, sweep, colour, meas, M)
+                                         : potts_word<ND>(O, N, C, D, J, q, key, j0, sweep, colour, meas, M, B);
       if (KIND == kPotts && meas && q <= 4u) fold_nibbles(M, B);
       if (NS != O) store4(gown + j0, NS);
     } else {
@@ -525,20 +523,19 @@ __device__ __forceinline__ void run_tile4(TileLds &L, const TileJob &J, uint32_t
         }
         const int8_t s = own[j];
         const int8_t ns = update_site<ND, KIND>(s, n, byte4(D.top, k), byte4(D.mid, k), byte4(D.ptop, k),
-                                                byte4(D.pmid, k), L, q, key, j, sweep, colour);
+                                                byte4(D.pmid, k), J.thr, q, key, j, sweep, colour);
         if (ns != s) own[j] = ns;
         if (meas) measure_site<ND, KIND>(M, ns, oth[j], n);
       }
     }
   }
 
-  __syncthreads();  // the block's next tile rewrites L.thr
 }
 
 // One tile of 4096 sites, 16 per lane (one Philox group each); measured
 // counters are added to the lane's M.
 template <int ND, int KIND>
-__device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
+__device__ __forceinline__ void run_tile16(const TileJob &J, uint32_t tile, uint64_t sweep, uint32_t colour,
                                          bool meas, Meas &M) {
   constexpr int z = 2 * ND;
   const uint32_t nh = J.nh;
@@ -548,9 +545,6 @@ __device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_
   const uint32_t m = q - 1;
   const bool need_prop = KIND == kPotts && q > 2;
   const bool need_pmid = need_prop && (m & (m - 1)) != 0;
-
-  if (threadIdx.x < kStencilTable) L.thr[threadIdx.x] = J.thr[threadIdx.x];
-  __syncthreads();
 
   if (j0 < nh) {
     int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
@@ -613,8 +607,8 @@ __device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_
         const LaneDraws D{word_of(T, (uint32_t)w), word_of(Mb, (uint32_t)w), word_of(PT, (uint32_t)w),
                           word_of(PM, (uint32_t)w)};
         const uint32_t S = O.d[w], jw = j0 + 4u * (uint32_t)w;
-        NS.d[w] = KIND == kIsing ? ising_word<ND>(S, N, C.d[w], D, J, L, key, jw, sweep, colour, meas, M)
-                                 : potts_word<ND>(S, N, C.d[w], D, J, L, q, key, jw, sweep, colour, meas, M, B);
+        NS.d[w] = KIND == kIsing ? ising_word<ND>(S, N, C.d[w], D, J, key, jw, sweep, colour, meas, M)
+                                 : potts_word<ND>(S, N, C.d[w], D, J, q, key, jw, sweep, colour, meas, M, B);
         changed |= NS.d[w] != S;
       }
       if (KIND == kPotts && meas && q <= 4u) fold_nibbles(M, B);
@@ -645,34 +639,53 @@ __device__ __forceinline__ void run_tile16(TileLds &L, const TileJob &J, uint32_
         const uint32_t wk = k >> 2;
         const int8_t ns = update_site<ND, KIND>(s, n, byte4(word_of(T, wk), (int)(k & 3)), byte4(word_of(Mb, wk), (int)(k & 3)),
                                                 byte4(word_of(PT, wk), (int)(k & 3)), byte4(word_of(PM, wk), (int)(k & 3)),
-                                                L, q, key, j, sweep, colour);
+                                                J.thr, q, key, j, sweep, colour);
         if (ns != s) own[j] = ns;
         if (meas) measure_site<ND, KIND>(M, ns, oth[j], n);
       }
     }
   }
 
-  __syncthreads();  // the block's next tile rewrites L.thr
+}
+
+// Sum of v over the 64 lanes of a wave (every lane active): quad, half-row
+// and row sums by DPP, then rows 0 + 1 and 2 + 3 by row broadcasts, whose
+// total lands in lane 63.
+__device__ __forceinline__ int wave_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1, 0, 3, 2]
+  v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2, 3, 0, 1]
+  v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row_mirror
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1 and 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2 and 3
+  return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Block sum of the lanes' counters, added to the record at `rec` (nc values).
-__device__ __forceinline__ void flush(TileLds &L, Meas &M, int nc, bool ising, long long *rec) {
-  const uint32_t lane = threadIdx.x & 63;
-  if (threadIdx.x < 2 + kMaxQ) L.red[threadIdx.x] = 0;
-  __syncthreads();
+// Every lane's counters stay far below 2^31 between flushes (at most 2 LANE
+// sites and 2 LANE z bonds per tile, 255 / (2 LANE) tiles), and so do a
+// wave's and the block's.  Flush k writes partial buffer k & 1: a wave can
+// rewrite a buffer only after passing the next flush's barrier, which every
+// wave reaches after reading this one.
+__device__ __forceinline__ void flush(TileLds &L, Meas &M, int nc, bool ising, long long *rec, uint32_t &flushes) {
+  const uint32_t wave = threadIdx.x >> 6, buf = flushes & 1u;
   for (int c = 0; c < nc; ++c) {
-    long long v;
+    int v;
     if (c == 0) v = M.e;
     else if (c == 1) v = ising ? M.m : 0;
-    else v = (long long)(((c - 2 < 8 ? M.cnt_lo : M.cnt_hi) >> (8 * ((c - 2) & 7))) & 0xFFu);
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (lane == 0 && v != 0) atomicAdd(&L.red[c], (unsigned long long)v);
+    else v = (int)(((c - 2 < 8 ? M.cnt_lo : M.cnt_hi) >> (8 * ((c - 2) & 7))) & 0xFFu);
+    const int sum = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) L.part[buf][wave][c] = sum;
   }
   __syncthreads();
-  if ((int)threadIdx.x < nc && L.red[threadIdx.x] != 0)
-    atomicAdd(reinterpret_cast<unsigned long long *>(rec) + threadIdx.x, L.red[threadIdx.x]);
+  if ((int)threadIdx.x < nc) {
+    long long t = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) t += L.part[buf][w][threadIdx.x];
+    if (t != 0) atomicAdd(reinterpret_cast<unsigned long long *>(rec) + threadIdx.x, (unsigned long long)t);
+  }
   M = Meas{};
-  __syncthreads();
+  ++flushes;
 }
 
 template <int LANE>
@@ -694,7 +707,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
   long long *rec = nullptr;
   int nc = 0;
   bool ising = false;
-  uint32_t cur = ~0u, pending = 0;
+  uint32_t cur = ~0u, pending = 0, flushes = 0;
   for (uint32_t t = lo + local; t < hi; t += per) {
     uint32_t a = 0, b = n_jobs;  // last job with tile_start <= t
     while (b - a > 1) {
@@ -706,7 +719,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
     if (sweep >= J.sweep_end) continue;
     const bool meas = measure && colour == 1 && sweep >= J.therm;
     if (pending && (a != cur || pending == kFlushTiles)) {
-      flush(L, M, nc, ising, rec);
+      flush(L, M, nc, ising, rec, flushes);
       pending = 0;
     }
     if (meas) {
@@ -721,25 +734,25 @@ __global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__res
     const uint32_t sel = J.nd * 2 + (J.kind == kPotts ? 1 : 0);
     if (LANE == 16) {
       switch (sel) {
-        case 2: run_tile16<1, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-        case 3: run_tile16<1, kPotts>(L, J, tile, sweep, colour, meas, M); break;
-        case 4: run_tile16<2, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-        case 5: run_tile16<2, kPotts>(L, J, tile, sweep, colour, meas, M); break;
-        case 6: run_tile16<3, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-        default: run_tile16<3, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+        case 2: run_tile16<1, kIsing>(J, tile, sweep, colour, meas, M); break;
+        case 3: run_tile16<1, kPotts>(J, tile, sweep, colour, meas, M); break;
+        case 4: run_tile16<2, kIsing>(J, tile, sweep, colour, meas, M); break;
+        case 5: run_tile16<2, kPotts>(J, tile, sweep, colour, meas, M); break;
+        case 6: run_tile16<3, kIsing>(J, tile, sweep, colour, meas, M); break;
+        default: run_tile16<3, kPotts>(J, tile, sweep, colour, meas, M); break;
       }
     } else {
       switch (sel) {
-        case 2: run_tile4<1, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-        case 3: run_tile4<1, kPotts>(L, J, tile, sweep, colour, meas, M); break;
-        case 4: run_tile4<2, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-        case 5: run_tile4<2, kPotts>(L, J, tile, sweep, colour, meas, M); break;
-        case 6: run_tile4<3, kIsing>(L, J, tile, sweep, colour, meas, M); break;
-        default: run_tile4<3, kPotts>(L, J, tile, sweep, colour, meas, M); break;
+        case 2: run_tile4<1, kIsing>(J, tile, sweep, colour, meas, M); break;
+        case 3: run_tile4<1, kPotts>(J, tile, sweep, colour, meas, M); break;
+        case 4: run_tile4<2, kIsing>(J, tile, sweep, colour, meas, M); break;
+        case 5: run_tile4<2, kPotts>(J, tile, sweep, colour, meas, M); break;
+        case 6: run_tile4<3, kIsing>(J, tile, sweep, colour, meas, M); break;
+        default: run_tile4<3, kPotts>(J, tile, sweep, colour, meas, M); break;
       }
     }
   }
-  if (pending) flush(L, M, nc, ising, rec);
+  if (pending) flush(L, M, nc, ising, rec, flushes);
 }
 
 __global__ void k_tile_zero(const TileHeader *__restrict__ hdr, long long *__restrict__ p, uint64_t per_row) {
