@@ -310,9 +310,12 @@ typedef struct {
   uint64_t spin_budget_bytes; /* device bytes for spins (one byte per site) */
   uint32_t max_jobs;          /* running jobs at once (1..4096) */
   uint32_t chunk_sweeps;      /* steps per graph launch (1..1024); 0 = 32 */
-  uint32_t grid;              /* blocks per launch, multiple of 8; 0 = autotune the grid, then
-                                 the sites per lane (4, 16 or by size), then the chunk length
-                                 (chunk_sweeps / 4, / 2 or chunk_sweeps), each on real chunks */
+  uint32_t grid;              /* blocks per launch, multiple of 8; 0 = autotune: the lane-width
+                                 crossover (1, 2 or 4 Mi resident colour-class sites for 16
+                                 sites per lane, else 4), then the grid, then the chunk length
+                                 (chunk_sweeps / 4, / 2 or chunk_sweeps), each by the median
+                                 rate of three real chunks of one set of resident jobs, once
+                                 the set holds 1 Mi colour-class sites (8 Mi for the grid) */
   uint32_t use_graphs;        /* 1 = hipGraph per chunk, 0 = plain launches */
 } scamper_pool_config;
 
@@ -341,7 +344,8 @@ typedef struct {
   uint64_t chunks;
   uint32_t grid;          /* blocks per launch in use (after autotuning) */
   uint32_t tuned;         /* 1 once the autotuner has settled */
-  uint32_t lane_sites;    /* sites per lane it settled on (0: by each launch's size) */
+  uint32_t lane16_from_mi; /* lane-width crossover it settled on: launches over at least this
+                              many Mi colour-class sites run 16 sites per lane, smaller ones 4 */
   uint32_t chunk_sweeps;  /* steps per chunk it settled on */
 } scamper_pool_stats;
 

@@ -62,7 +62,7 @@ class PoolStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("queued", C.c_uint64), ("running", C.c_uint64),
                 ("done", C.c_uint64), ("spin_bytes_in_use", C.c_uint64), ("site_updates", C.c_uint64),
                 ("chunks", C.c_uint64), ("grid", C.c_uint32), ("tuned", C.c_uint32),
-                ("lane_sites", C.c_uint32), ("chunk_sweeps", C.c_uint32)]
+                ("lane16_from_mi", C.c_uint32), ("chunk_sweeps", C.c_uint32)]
 
 
 _lib = None

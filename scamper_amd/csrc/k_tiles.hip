@@ -688,8 +688,11 @@ __device__ __forceinline__ void flush(TileLds &L, Meas &M, int nc, bool ising, l
   ++flushes;
 }
 
+// At most 96 VGPRs, so five waves share a SIMD: the 3-D Potts variant
+// alone would take 113 (four waves); the compiler spills one value, saved
+// at the start of the kernel and restored at its end.
 template <int LANE>
-__global__ __launch_bounds__(kThreads) void k_tile_sweep(const TileHeader *__restrict__ hdr,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_tile_sweep(const TileHeader *__restrict__ hdr,
                                                          const TileJob *__restrict__ jobs, uint64_t step,
                                                          uint32_t colour, int measure, EmLayout em) {
   __shared__ TileLds L;

@@ -17,6 +17,7 @@
 // is enqueued after that, so it runs after every launch that touched the
 // region.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -36,9 +37,25 @@ namespace {
 constexpr uint32_t kRec = 8;   // counters per (job, step) record: em[0], em[1], counts of q <= 6 states
 constexpr uint32_t kSubs = 8;  // per-XCD partial records folded after the chunk
 constexpr uint32_t kMaxPoolQ = kRec - 2;
-constexpr uint32_t kGridCandidates[] = {256, 512, 1024, 2048, 4096};
-constexpr uint32_t kLaneCandidates[] = {0, 4, 16};  // 0: by the launch's size (tile_lane_sites)
-constexpr int kTuneRounds = 2;  // timed chunks per candidate
+constexpr uint32_t kGridCandidates[] = {1024, 2048, 4096};
+// Sites per lane follow the launch's resident colour-class sites: 16 from a
+// crossover on, 4 below it.  The crossover (in Mi sites) is what the tuner
+// chooses, so the lane width still follows the work as the set changes.
+constexpr uint32_t kLaneCrossovers[] = {2, 1, 4};  // 2 Mi: the measured one (tile_lane_sites)
+constexpr int kTuneRounds = 3;  // timed chunks per candidate; the median rate counts
+// Tuning waits for a resident set of at least this many colour-class sites:
+// on less work the shape hardly matters and its timings would not carry over
+// to larger sets.  Crossovers are compared only where they give the set
+// different lane widths, and grids only up to the set's tile count (larger
+// ones are capped to it).
+constexpr uint64_t kTuneMinSites = 1ull << 20;
+// The grid is timed only on sets of at least this many sites: a grid that
+// wins on a light set loses on the heavy ones that hold most of the work.
+constexpr uint64_t kTuneGridSites = 8ull << 20;
+// A stage's candidates are compared only on chunks of one set of resident
+// jobs: a change of the set restarts the stage's timing, and after this many
+// restarts the shape in use is kept.
+constexpr int kMaxTuneRestarts = 16;
 constexpr uint64_t kAlign = 256;
 constexpr uint32_t kMaxBypass = 64;  // later jobs admitted past a waiting head
 
@@ -72,14 +89,18 @@ struct InFlight {
   std::vector<RowInfo> rows;
   uint64_t updates = 0;
   uint32_t steps = 0;  // chunk length of this chunk
-  int candidate = -1;  // autotuner candidate timed by this chunk
+  int candidate = -1;  // autotuner candidate timed by this chunk, an index into that stage's list
+  int stage = 0;       // the tuning stage it was chosen in
+  uint64_t set = 0;    // signature of the resident jobs it ran
 };
 
-// Launch shape of a chunk: blocks per launch, sites per lane (0: by the
-// launch's size) and steps per chunk.
+// Launch shape of a chunk: blocks per launch, the lane-width crossover (Mi
+// colour-class sites from which lanes hold 16 sites) and steps per chunk.
 struct Shape {
-  uint32_t grid, lane, chunk;
+  uint32_t grid, cross, chunk;
 };
+
+uint32_t lane_for(const Shape &s, uint64_t sites) { return sites >= ((uint64_t)s.cross << 20) ? 16u : 4u; }
 
 // First-fit allocator over [0, size).
 class Arena {
@@ -147,15 +168,18 @@ struct scamper_pool {
   std::vector<uint64_t> completed;  // finished, not yet reported
   uint64_t n_submitted = 0, n_done = 0, updates = 0;
   // autotuner: launch shapes timed on real chunks, one coordinate at a time
-  // (stage 0 the grid, 1 the lane width at the best grid, 2 the chunk length
-  // at the best grid and lane width); results never depend on the shape
+  // (stage 0 the lane width, 1 the grid at that lane width, 2 the chunk
+  // length at both); results never depend on the shape
   Shape shape{0, 0, 0};  // in use (settled, or the fallback while tuning)
   bool tuned = false;
-  int stage = 0;
+  int stage = -1;  // -1 until a resident set large enough to tune on
   std::vector<Shape> cands;
-  std::vector<double> cand_best;
+  std::vector<std::array<double, kTuneRounds>> cand_rates;
   std::vector<int> cand_samples;
   int next_candidate = 0;
+  uint64_t cand_set = 0;  // signature of the resident jobs the stage's samples were timed on
+  int tune_restarts = 0;
+  uint64_t cur_sites = 0;  // resident colour-class sites of the latest chunk
 
   ~scamper_pool() {
     if (stream) hipStreamSynchronize(stream);
@@ -274,18 +298,31 @@ void start_stage(scamper_pool *P, int stage) {
   P->stage = stage;
   P->cands.clear();
   const Shape s = P->shape;
-  if (stage == 0)
-    for (uint32_t g : kGridCandidates) P->cands.push_back(Shape{g, s.lane, s.chunk});
-  else if (stage == 1)
-    for (uint32_t l : kLaneCandidates) P->cands.push_back(Shape{s.grid, l, s.chunk});
-  else if (stage == 2)
+  if (stage == 0) {
+    for (uint32_t c : kLaneCrossovers) {  // one candidate per distinct lane width on this set
+      const Shape t{s.grid, c, s.chunk};
+      bool seen = false;
+      for (const Shape &o : P->cands) seen = seen || lane_for(o, P->cur_sites) == lane_for(t, P->cur_sites);
+      if (!seen) P->cands.push_back(t);
+    }
+  } else if (stage == 1) {
+    const uint32_t lane = lane_for(s, P->cur_sites);
+    const uint64_t tiles = (P->cur_sites + tile_sites(lane) - 1) / tile_sites(lane);
+    for (uint32_t g : kGridCandidates)
+      if (g <= tiles) P->cands.push_back(Shape{g, s.cross, s.chunk});
+  } else if (stage == 2)
     for (uint32_t k : {std::max(1u, P->cfg.chunk_sweeps / 4), std::max(1u, P->cfg.chunk_sweeps / 2), P->cfg.chunk_sweeps})
-      if (P->cands.empty() || P->cands.back().chunk != k) P->cands.push_back(Shape{s.grid, s.lane, k});
+      if (P->cands.empty() || P->cands.back().chunk != k) P->cands.push_back(Shape{s.grid, s.cross, k});
   if (P->cands.empty()) {
     P->tuned = true;
     return;
   }
-  P->cand_best.assign(P->cands.size(), 0.0);
+  if (P->cands.size() == 1) {  // nothing to compare
+    P->shape = P->cands[0];
+    start_stage(P, stage + 1);
+    return;
+  }
+  P->cand_rates.assign(P->cands.size(), {});
   P->cand_samples.assign(P->cands.size(), 0);
   P->next_candidate = 0;
 }
@@ -293,13 +330,16 @@ void start_stage(scamper_pool *P, int stage) {
 int collect(scamper_pool *P, int p, uint32_t *done) {
   InFlight &F = P->inflight[p];
   SC_HIP(hipEventSynchronize(P->ev_end[p]));
-  if (F.candidate >= 0) {
+  // a sample counts only for the stage and the job set it was chosen for:
+  // the other in-flight chunk may have closed that stage meanwhile, and the
+  // next stage's list is shorter
+  if (F.candidate >= 0 && !P->tuned && F.stage == P->stage && F.set == P->cand_set &&
+      (size_t)F.candidate < P->cands.size()) {
     float ms = 0.f;
     SC_HIP(hipEventElapsedTime(&ms, P->ev_start[p], P->ev_graph[p]));
     if (ms > 0.f) {
       const double rate = (double)F.updates / (double)ms;
-      P->cand_best[F.candidate] = std::max(P->cand_best[F.candidate], rate);
-      P->cand_samples[F.candidate] += 1;
+      if (P->cand_samples[F.candidate] < kTuneRounds) P->cand_rates[F.candidate][P->cand_samples[F.candidate]++] = rate;
     }
   }
   const uint32_t K = F.steps;
@@ -328,9 +368,13 @@ int collect(scamper_pool *P, int p, uint32_t *done) {
     bool all = true;
     for (size_t c = 0; c < P->cands.size(); ++c) all = all && P->cand_samples[c] >= kTuneRounds;
     if (all) {
+      auto median = [](std::array<double, kTuneRounds> r) {
+        std::sort(r.begin(), r.end());
+        return r[kTuneRounds / 2];
+      };
       size_t best = 0;
       for (size_t c = 1; c < P->cands.size(); ++c)
-        if (P->cand_best[c] > P->cand_best[best]) best = c;
+        if (median(P->cand_rates[c]) > median(P->cand_rates[best])) best = c;
       P->shape = P->cands[best];
       start_stage(P, P->stage + 1);
     }
@@ -381,9 +425,30 @@ int admit_queued(scamper_pool *P) {
 
 // The shape of the next chunk: the settled one, or a candidate of the
 // current tuning stage that still needs timing (*candidate = its index).
-Shape choose_shape(scamper_pool *P, int *candidate) {
+// `set` identifies the resident jobs: when it differs from the set the
+// stage's samples were timed on, their timing starts again (rates of
+// different job sets do not compare), up to kMaxTuneRestarts times.
+Shape choose_shape(scamper_pool *P, uint64_t set, int *candidate) {
   *candidate = -1;
-  if (P->tuned || P->cands.empty()) return P->shape;
+  if (P->tuned || P->cur_sites < (P->stage == 1 ? kTuneGridSites : kTuneMinSites)) return P->shape;
+  if (P->stage < 0) {
+    start_stage(P, 0);
+    P->cand_set = set;
+    if (P->tuned) return P->shape;
+  }
+  if (set != P->cand_set) {
+    const bool had_samples = std::any_of(P->cand_samples.begin(), P->cand_samples.end(), [](int k) { return k > 0; });
+    P->cand_set = set;
+    if (had_samples) {
+      if (++P->tune_restarts > kMaxTuneRestarts) {
+        P->tuned = true;
+        return P->shape;
+      }
+      std::fill(P->cand_rates.begin(), P->cand_rates.end(), std::array<double, kTuneRounds>{});
+      std::fill(P->cand_samples.begin(), P->cand_samples.end(), 0);
+      P->next_candidate = 0;
+    }
+  }
   const int n = (int)P->cands.size();
   for (int tries = 0; tries < n; ++tries) {
     const int c = P->next_candidate;
@@ -417,8 +482,15 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   if (int rc = admit_queued(P)) return rc;
 
   // the job table of this chunk, in the chunk's launch shape
+  uint64_t set = 1469598103934665603ull;  // FNV-1a over the resident job ids, in order
+  uint64_t total_sites = 0;
+  for (uint64_t id : P->running) {
+    set = (set ^ id) * 1099511628211ull;
+    total_sites += P->jobs.at(id)->geom.nh;
+  }
+  P->cur_sites = total_sites;
   int candidate;
-  const Shape shape = choose_shape(P, &candidate);
+  const Shape shape = choose_shape(P, set, &candidate);
   const uint32_t K = shape.chunk;
   TileHeader *hdr = reinterpret_cast<TileHeader *>(P->h_table[p]);
   TileJob *rows = reinterpret_cast<TileJob *>(hdr + 1);
@@ -426,9 +498,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   F.rows.clear();
   F.updates = 0;
   uint32_t tiles = 0;
-  uint64_t total_sites = 0;
-  for (uint64_t id : P->running) total_sites += P->jobs.at(id)->geom.nh;
-  const uint32_t lane = shape.lane ? shape.lane : tile_lane_sites(total_sites);
+  const uint32_t lane = lane_for(shape, total_sites);
   for (uint64_t id : P->running) {
     Job &J = *P->jobs.at(id);
     const uint32_t steps = (uint32_t)std::min<uint64_t>(K, J.total - J.enqueued);
@@ -500,6 +570,8 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   F.valid = true;
   F.steps = K;
   F.candidate = candidate;
+  F.stage = P->stage;
+  F.set = set;
   P->updates += F.updates;
   ++P->chunk;
   if (n_completed) *n_completed = (uint32_t)(P->n_done - done_before);
@@ -528,11 +600,8 @@ int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out) {
   P->cfg = c;
   // a configured grid fixes the shape (lanes by size, the configured
   // chunk); otherwise the autotuner starts from 1024 blocks
-  P->shape = Shape{c.grid ? c.grid : 1024u, 0u, c.chunk_sweeps};
-  if (c.grid)
-    P->tuned = true;
-  else
-    start_stage(P.get(), 0);
+  P->shape = Shape{c.grid ? c.grid : 1024u, kLaneCrossovers[0], c.chunk_sweeps};
+  if (c.grid) P->tuned = true;
   SC_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
   SC_HIP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
   if (hipMalloc(&P->spins_alloc, c.spin_budget_bytes + 2 * kSpinSlack) != hipSuccess)
@@ -670,7 +739,7 @@ int scamper_pool_stats_get(const scamper_pool *P, scamper_pool_stats *out) {
   out->chunks = P->chunk;
   out->grid = P->tuned ? P->shape.grid : 0;
   out->tuned = P->tuned ? 1u : 0u;
-  out->lane_sites = P->tuned ? P->shape.lane : 0;
+  out->lane16_from_mi = P->tuned ? P->shape.cross : 0;
   out->chunk_sweeps = P->tuned ? P->shape.chunk : 0;
   return SCAMPER_OK;
 }

@@ -143,7 +143,7 @@ def serve(requests: list[Request], spin_budget_bytes: int = 8 << 30, max_jobs: i
     wall = time.perf_counter() - t0
     lat = done_at - np.array([r.arrival for r in requests])
     st = pool.stats()
-    shape = {"grid": st.grid, "lane_sites": st.lane_sites or "by size", "chunk_sweeps": st.chunk_sweeps} \
+    shape = {"grid": st.grid, "lane16_from_mi_sites": st.lane16_from_mi, "chunk_sweeps": st.chunk_sweeps} \
         if st.tuned else {"tuned": False}
     return ServeResult(n, wall, sum(r.updates for r in requests), lat, estimates, shape)
 

@@ -146,19 +146,33 @@ def test_launch_shape_does_not_change_results(sc, cfg):
 
 
 def test_autotuner_settles_grid_lane_and_chunk(sc):
-    """With grid = 0 the pool times candidates on real chunks, one coordinate
-    at a time: the grid, then the sites per lane (4, 16 or by size), then the
-    chunk length (chunk_sweeps / 4, / 2, chunk_sweeps).  It settles and
-    records its choice, and every shape it tried gives the results of a
+    """With grid = 0 the pool times candidates on real chunks of one resident
+    set, one coordinate at a time: the lane-width crossover (1, 2 or 4 Mi
+    colour-class sites for 16 sites per lane), then the grid, then the chunk
+    length (chunk_sweeps / 4, / 2, chunk_sweeps).  It starts once the
+    resident set holds 1 Mi colour-class sites (8 Mi for the grid), settles
+    and records its choice, and every shape it tried gives the results of a
     fixed shape."""
-    specs = [_spec(sc, (64, 64), 1, 3, 1.0, 10, 390 + 7 * i, 700 + i) for i in range(3)] + \
-            [_spec(sc, (16, 16, 16), 0, 2, 0.22, 10, 400, 710)]
-    ref, _ = _run_pool(sc, specs, chunk_sweeps=16, max_jobs=16, spin_budget_bytes=1 << 24, grid=1024)
+    specs = [_spec(sc, (2048, 2048), 1, 3, 1.0, 10, 290 + 7 * i, 700 + i) for i in range(3)] + \
+            [_spec(sc, (200, 200, 200), 0, 2, 0.22, 10, 300, 710)]
+    ref, _ = _run_pool(sc, specs, chunk_sweeps=16, max_jobs=16, spin_budget_bytes=1 << 25, grid=1024)
+    got, pool = _run_pool(sc, specs, chunk_sweeps=8, max_jobs=16, spin_budget_bytes=1 << 25, grid=0)
+    st = pool.stats()
+    print(f"settled: grid {st.grid} lane16 from {st.lane16_from_mi} Mi chunk {st.chunk_sweeps}", flush=True)
+    assert st.tuned == 1 and st.grid in (1024, 2048, 4096)
+    assert st.lane16_from_mi in (1, 2, 4) and st.chunk_sweeps in (2, 4, 8)
+    for i in ref:
+        _same(ref[i], got[i])
+
+
+def test_autotuner_waits_for_enough_work(sc):
+    """A resident set below 1 Mi colour-class sites runs the starting shape
+    (1024 blocks, 16 sites per lane from 2 Mi sites, chunk_sweeps) untuned."""
+    specs = [_spec(sc, (64, 64), 1, 3, 1.0, 10, 90 + 7 * i, 720 + i) for i in range(3)]
     got, pool = _run_pool(sc, specs, chunk_sweeps=8, max_jobs=16, spin_budget_bytes=1 << 24, grid=0)
     st = pool.stats()
-    print(f"settled: grid {st.grid} lane {st.lane_sites} chunk {st.chunk_sweeps}", flush=True)
-    assert st.tuned == 1 and st.grid in (256, 512, 1024, 2048, 4096)
-    assert st.lane_sites in (0, 4, 16) and st.chunk_sweeps in (2, 4, 8)
+    assert st.tuned == 0
+    ref, _ = _run_pool(sc, specs, chunk_sweeps=8, max_jobs=16, spin_budget_bytes=1 << 24, grid=1024)
     for i in ref:
         _same(ref[i], got[i])
 
