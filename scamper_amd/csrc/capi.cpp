@@ -391,7 +391,7 @@ int upload_tables(scamper_state *st, int alg) {
       TileJob &t = rows[r];
       memset(&t, 0, sizeof t);
       t.key = st->keys[r];
-      t.spins = st->d_spins + 2ull * r * st->geom.nh;
+      t.spins = st->d_spins + 2ull * r * st->geom.cs;
       t.sweep_base = 0;
       t.sweep_end = ~0ull;
       t.therm = 0;
@@ -402,6 +402,7 @@ int upload_tables(scamper_state *st, int alg) {
       t.ly = st->geom.ly;
       t.lz = st->geom.lz;
       t.nh = (uint32_t)st->geom.nh;
+      t.cs = (uint32_t)st->geom.cs;
       t.kind = (uint32_t)st->model.kind;
       t.q = (uint32_t)st->model.q;
       t.tile_start = r * per;
@@ -488,7 +489,9 @@ int upload_i8_impl(scamper_state *st, uint32_t replica, const int8_t *spins) {
     const uint64_t i = split_index(st->geom, s, &c);
     split[c * st->geom.nh + i] = spins[s];
   }
-  SC_HIP(hipMemcpy(st->d_spins + 2ull * replica * st->geom.nh, split.data(), n, hipMemcpyHostToDevice));
+  int8_t *base = st->d_spins + 2ull * replica * st->geom.cs;
+  SC_HIP(hipMemcpy(base, split.data(), st->geom.nh, hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(base + st->geom.cs, split.data() + st->geom.nh, st->geom.nh, hipMemcpyHostToDevice));
   return SCAMPER_OK;
 }
 
@@ -500,7 +503,9 @@ int download_i8_impl(scamper_state *st, uint32_t replica, int8_t *spins) {
     return SCAMPER_OK;
   }
   std::vector<int8_t> split(n);
-  SC_HIP(hipMemcpy(split.data(), st->d_spins + 2ull * replica * st->geom.nh, n, hipMemcpyDeviceToHost));
+  const int8_t *base = st->d_spins + 2ull * replica * st->geom.cs;
+  SC_HIP(hipMemcpy(split.data(), base, st->geom.nh, hipMemcpyDeviceToHost));
+  SC_HIP(hipMemcpy(split.data() + st->geom.nh, base + st->geom.cs, st->geom.nh, hipMemcpyDeviceToHost));
   for (uint64_t s = 0; s < n; ++s) {
     uint32_t c;
     const uint64_t i = split_index(st->geom, s, &c);
@@ -697,6 +702,7 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
     st->geom.ly = (uint32_t)l.dims[1];
     st->geom.lz = (uint32_t)l.dims[2];
     st->geom.nh = l.n_sites / 2;
+    st->geom.cs = aligned_colour(st->geom.nh);
     set_divisors(st->geom);
     st->z = 2 * l.nd;
   } else if (model->kind == kON) {
@@ -802,7 +808,8 @@ int scamper_state_create(const scamper_lattice *lh, const scamper_model *model, 
     const size_t bytes = (size_t)n_replicas * l.n_sites * model->spin_dim * sizeof(float);
     if (hipMalloc(&st->d_vec, bytes) != hipSuccess) return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
   } else {
-    const size_t bytes = (size_t)n_replicas * l.n_sites;
+    // graphs: site-major replicas; stencils: two colour arrays of cs bytes each
+    const size_t bytes = (size_t)n_replicas * (st->csr ? l.n_sites : 2 * st->geom.cs);
     if (hipMalloc(&st->d_spins_alloc, bytes + 2 * kSpinSlack) != hipSuccess)
       return fail(SCAMPER_ENOMEM, "spins: %zu bytes", bytes);
     st->d_spins = st->d_spins_alloc + kSpinSlack;
@@ -1236,7 +1243,7 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
           e = launch_cluster_transform(nullptr, n, spins, st->d_parent, st->model.kind, st->model.q, wolff,
                                        st->keys[r], step, st->d_cscratch, st->stream);
       } else {
-        int8_t *spins = st->d_spins + 2ull * r * st->geom.nh;
+        int8_t *spins = st->d_spins + 2ull * r * st->geom.cs;
         e = launch_cluster_bonds_stencil(st->geom, spins, st->d_parent, st->keys[r], step,
                                          thr[(size_t)r * kClusterWeights + 1], st->stream);
         if (e == hipSuccess)

@@ -38,8 +38,11 @@ struct alignas(16) ReplicaParams {
 };
 
 // Regular periodic hypercubic geometry with even sides, stored colour-split:
-// colour c of replica r occupies spins[(2r + c) * nh ...], rows of lx2 bytes,
-// row index = y + ly * z, element i of a row is site x = 2i + ((c+y+z)&1).
+// colour c of replica r occupies spins[(2r + c) * cs ...] (int8 spins; cs is
+// nh rounded up to kColourAlign bytes, so that every colour array starts
+// aligned), rows of lx2 bytes, row index = y + ly * z, element i of a row is
+// site x = 2i + ((c+y+z)&1).  Continuous spins keep their own plane layout
+// over nh.
 struct StencilGeom {
   int nd;        // 1, 2 or 3
   uint32_t lx2;  // Lx / 2
@@ -47,7 +50,15 @@ struct StencilGeom {
   uint32_t lz;
   uint64_t nh;   // sites per colour = lx2 * ly * lz
   uint64_t div_lx2 = 0, div_ly = 0;  // division magics (set_divisors)
+  uint64_t cs = 0;                   // int8 spins: bytes from a colour array to the next (>= nh)
 };
+
+// Colour arrays of int8 spins start at multiples of kColourAlign bytes, so
+// that 16-byte windows at multiples of 16 ranks are aligned, and each is
+// followed by a gap of at least 16 bytes, which the tiled kernel's last
+// window of a colour class reads and writes (bytes of no site).
+constexpr uint64_t kColourAlign = 256;
+inline uint64_t aligned_colour(uint64_t nh) { return (nh + 16 + kColourAlign - 1) / kColourAlign * kColourAlign; }
 
 // ceil(2^64 / d) for d >= 2, 0 for d = 1: floor(n / d) = mulhi64(n, magic)
 // for every 32-bit n (the error term n * (magic * d - 2^64) / 2^64 stays
@@ -203,7 +214,7 @@ constexpr uint32_t kSpinSlack = 64;
 
 struct alignas(16) TileJob {
   uint64_t key;          // Philox key of the job's replica 0
-  int8_t *spins;         // colour 0 at spins, colour 1 at spins + nh
+  int8_t *spins;         // colour 0 at spins, colour 1 at spins + cs
   uint64_t sweep_base;   // sweep number of step 0
   uint64_t sweep_end;    // steps whose sweep is >= sweep_end do nothing
   uint64_t therm;        // sweeps >= therm are measured (when asked)
@@ -211,7 +222,7 @@ struct alignas(16) TileJob {
   uint64_t div_ly;       // ceil(2^64 / ly), 0 when ly == 1
   uint32_t nd, lx2, ly, lz;
   uint32_t nh, kind, q, tile_start;
-  uint32_t n_tiles, em_row, pad0, pad1;
+  uint32_t n_tiles, em_row, cs, pad1;  // cs: bytes from colour 0 to colour 1
   Threshold thr[kStencilTable];  // by anti-aligned count (Ising) or d + z (Potts)
   // byte look-up tables over the same index (v_perm operands, 4 entries per
   // word): top and second byte of K, both 0xff where K = 2^32 (a draw whose

@@ -80,18 +80,18 @@ __device__ __forceinline__ StencilSite stencil_site(uint64_t s, uint32_t lx, uin
   const uint64_t r = s / lx;
   return StencilSite{(uint32_t)(s - r * lx), (uint32_t)(r % ly), (uint32_t)(r / ly)};
 }
-__device__ __forceinline__ uint64_t stencil_byte(uint64_t s, const StencilSite &c, uint64_t nh) {
-  return (uint64_t)((c.x + c.y + c.z) & 1u) * nh + (s >> 1);
+__device__ __forceinline__ uint64_t stencil_byte(uint64_t s, const StencilSite &c, uint64_t cs) {
+  return (uint64_t)((c.x + c.y + c.z) & 1u) * cs + (s >> 1);
 }
 
 // Edges of site s in build_hypercubic order: e = nd s + a, to +1 along axis a.
 template <int ND>
 __global__ void k_bonds_stencil(const int8_t *__restrict__ spins, uint32_t *parent, uint32_t lx, uint32_t ly,
-                                uint32_t lz, uint64_t nh, uint64_t key, uint64_t step, uint64_t thr) {
+                                uint32_t lz, uint64_t nh, uint64_t cs, uint64_t key, uint64_t step, uint64_t thr) {
   const uint64_t n = 2 * nh;
   for (uint64_t s = thread_id(); s < n; s += grid_threads()) {
     const StencilSite c = stencil_site(s, lx, ly);
-    const int8_t v = spins[stencil_byte(s, c, nh)];
+    const int8_t v = spins[stencil_byte(s, c, cs)];
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
       StencilSite t = c;
@@ -106,7 +106,7 @@ __global__ void k_bonds_stencil(const int8_t *__restrict__ spins, uint32_t *pare
         t.z = c.z + 1 == lz ? 0u : c.z + 1;
         ts = s + ((uint64_t)t.z - c.z) * (uint64_t)lx * ly;
       }
-      if (spins[stencil_byte(ts, t, nh)] != v) continue;
+      if (spins[stencil_byte(ts, t, cs)] != v) continue;
       if ((uint64_t)cluster_word(key, (uint64_t)ND * s + a, step, kStreamClusterBond) < thr)
         unite(parent, (uint32_t)s, (uint32_t)ts);
     }
@@ -182,7 +182,7 @@ __device__ void unite_lds(uint32_t *lp, uint32_t a, uint32_t b) {
 // unite runs in LDS.
 template <int ND>
 __global__ __launch_bounds__(kThreads) void k_bonds_tile(const int8_t *__restrict__ spins, uint32_t *__restrict__ parent,
-                                                         uint32_t lx, uint32_t ly, uint64_t nh, uint64_t key,
+                                                         uint32_t lx, uint32_t ly, uint64_t cs, uint64_t key,
                                                          uint64_t step, uint64_t thr) {
   using T = Tile<ND>;
   static_assert(T::N == 4 * kThreads, "one quad per thread");
@@ -198,8 +198,8 @@ __global__ __launch_bounds__(kThreads) void k_bonds_tile(const int8_t *__restric
   const uint32_t x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
   const uint32_t s0 = ox + x0 + lx * (oy + y + ly * (oz + z));
   const uint32_t c = (ox + x0 + oy + y + oz + z) & 1u;
-  const uint32_t own = (uint16_t)*reinterpret_cast<const int16_t *>(spins + (uint64_t)c * nh + s0 / 2);
-  const uint32_t oth = (uint16_t)*reinterpret_cast<const int16_t *>(spins + (uint64_t)(c ^ 1u) * nh + s0 / 2);
+  const uint32_t own = (uint16_t)*reinterpret_cast<const int16_t *>(spins + (uint64_t)c * cs + s0 / 2);
+  const uint32_t oth = (uint16_t)*reinterpret_cast<const int16_t *>(spins + (uint64_t)(c ^ 1u) * cs + s0 / 2);
   uint32_t w[4 * ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) {
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kThreads) void k_bonds_tile(const int8_t *__restric
 // (periodic): X-face, Y-face (and Z-face) per tile, one thread each.
 template <int ND>
 __global__ void k_bonds_faces(const int8_t *__restrict__ spins, uint32_t *parent, uint32_t lx, uint32_t ly, uint32_t lz,
-                              uint64_t nh, uint64_t n_tiles, uint64_t key, uint64_t step, uint64_t thr) {
+                              uint64_t cs, uint64_t n_tiles, uint64_t key, uint64_t step, uint64_t thr) {
   using T = Tile<ND>;
   constexpr uint32_t fx = T::Y * T::Z, fy = T::X * T::Z, fz = ND == 3 ? T::X * T::Y : 0u;
   constexpr uint32_t per_tile = fx + fy + fz;
@@ -275,7 +275,7 @@ __global__ void k_bonds_faces(const int8_t *__restrict__ spins, uint32_t *parent
     const uint32_t z2 = a == 2 ? (z + 1 == lz ? 0u : z + 1) : z;
     const uint64_t s = (uint64_t)x + (uint64_t)lx * ((uint64_t)y + (uint64_t)ly * z);
     const uint64_t t = (uint64_t)x2 + (uint64_t)lx * ((uint64_t)y2 + (uint64_t)ly * z2);
-    if (spins[stencil_byte(s, StencilSite{x, y, z}, nh)] != spins[stencil_byte(t, StencilSite{x2, y2, z2}, nh)]) continue;
+    if (spins[stencil_byte(s, StencilSite{x, y, z}, cs)] != spins[stencil_byte(t, StencilSite{x2, y2, z2}, cs)]) continue;
     // start from the tile-local roots, so that the entry of a site that is
     // not a local root keeps pointing at its own tile's local root
     if ((uint64_t)cluster_word(key, (uint64_t)ND * s + a, step, kStreamClusterBond) < thr)
@@ -291,11 +291,11 @@ __device__ __forceinline__ int8_t sw_target(uint32_t root, int kind, int q, uint
 
 // The Wolff seed's root and target, from the spins before the update.
 __global__ void k_wolff_seed(const int8_t *__restrict__ spins, uint32_t *parent, int stencil, uint32_t lx,
-                             uint32_t ly, uint64_t nh, uint64_t n_sites, int kind, int q, uint64_t key,
+                             uint32_t ly, uint64_t cs, uint64_t n_sites, int kind, int q, uint64_t key,
                              uint64_t step, uint32_t *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint64_t seed = ((uint64_t)cluster_word(key, 0, step, kStreamClusterSeed) * n_sites) >> 32;
-  const int8_t s0 = stencil ? spins[stencil_byte(seed, stencil_site(seed, lx, ly), nh)] : spins[seed];
+  const int8_t s0 = stencil ? spins[stencil_byte(seed, stencil_site(seed, lx, ly), cs)] : spins[seed];
   int8_t target;
   if (kind == kIsing) {
     target = (int8_t)-s0;
@@ -309,7 +309,7 @@ __global__ void k_wolff_seed(const int8_t *__restrict__ spins, uint32_t *parent,
 }
 
 __global__ void k_transform(int8_t *__restrict__ spins, uint32_t *parent, int stencil, uint32_t lx, uint32_t ly,
-                            uint64_t nh, uint64_t n_sites, int kind, int q, int wolff, uint64_t key, uint64_t step,
+                            uint64_t cs, uint64_t n_sites, int kind, int q, int wolff, uint64_t key, uint64_t step,
                             const uint32_t *__restrict__ seed) {
   const uint32_t seed_root = wolff ? seed[0] : 0u;
   const int8_t wolff_target = wolff ? (int8_t)(uint8_t)seed[1] : (int8_t)0;
@@ -322,7 +322,7 @@ __global__ void k_transform(int8_t *__restrict__ spins, uint32_t *parent, int st
     } else {
       v = sw_target(root, kind, q, key, step);
     }
-    const uint64_t at = stencil ? stencil_byte(s, stencil_site(s, lx, ly), nh) : s;
+    const uint64_t at = stencil ? stencil_byte(s, stencil_site(s, lx, ly), cs) : s;
     spins[at] = v;
   }
 }
@@ -332,7 +332,7 @@ __global__ void k_transform(int8_t *__restrict__ spins, uint32_t *parent, int st
 // s0 + 1 and s0 + 3 at the same bytes of the other colour, so each colour
 // takes one 16-bit store.
 __global__ void k_transform_stencil(int8_t *__restrict__ spins, uint32_t *parent, uint32_t lx, uint32_t ly,
-                                    uint64_t nh, uint32_t n_sites, int kind, int q, int wolff, uint64_t key,
+                                    uint64_t cs, uint32_t n_sites, int kind, int q, int wolff, uint64_t key,
                                     uint64_t step, const uint32_t *__restrict__ seed) {
   const uint32_t seed_root = wolff ? seed[0] : 0u;
   const int8_t wolff_target = wolff ? (int8_t)(uint8_t)seed[1] : (int8_t)0;
@@ -340,8 +340,8 @@ __global__ void k_transform_stencil(int8_t *__restrict__ spins, uint32_t *parent
     const uint32_t s0 = (uint32_t)g * 4, r = s0 / lx, c = (r % ly + r / ly) & 1u;
     const uint4 p4 = *reinterpret_cast<const uint4 *>(parent + s0);
     const uint32_t first[4] = {p4.x, p4.y, p4.z, p4.w};
-    int16_t *at_c = reinterpret_cast<int16_t *>(spins + (uint64_t)c * nh + s0 / 2);
-    int16_t *at_o = reinterpret_cast<int16_t *>(spins + (uint64_t)(c ^ 1u) * nh + s0 / 2);
+    int16_t *at_c = reinterpret_cast<int16_t *>(spins + (uint64_t)c * cs + s0 / 2);
+    int16_t *at_o = reinterpret_cast<int16_t *>(spins + (uint64_t)(c ^ 1u) * cs + s0 / 2);
     uint32_t v[4];
     bool any = !wolff;
 #pragma unroll
@@ -375,7 +375,7 @@ __global__ void k_transform_stencil(int8_t *__restrict__ spins, uint32_t *parent
 // tile's root walks on its own.
 template <int ND>
 __global__ __launch_bounds__(kThreads) void k_transform_tile(int8_t *__restrict__ spins, uint32_t *parent,
-                                                             uint32_t lx, uint32_t ly, uint64_t nh, uint64_t div_lx2,
+                                                             uint32_t lx, uint32_t ly, uint64_t cs, uint64_t div_lx2,
                                                              uint64_t div_ly, int kind, int q, int wolff, uint64_t key,
                                                              uint64_t step, const uint32_t *__restrict__ seed) {
   using T = Tile<ND>;
@@ -431,8 +431,8 @@ __global__ __launch_bounds__(kThreads) void k_transform_tile(int8_t *__restrict_
   // sites s0, s0 + 2 of colour c at bytes s0/2, s0/2 + 1; s0 + 1, s0 + 3 of the other
   const uint32_t x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
   const uint32_t c = (ox + x0 + oy + y + oz + z) & 1u;
-  int16_t *at_c = reinterpret_cast<int16_t *>(spins + (uint64_t)c * nh + s0 / 2);
-  int16_t *at_o = reinterpret_cast<int16_t *>(spins + (uint64_t)(c ^ 1u) * nh + s0 / 2);
+  int16_t *at_c = reinterpret_cast<int16_t *>(spins + (uint64_t)c * cs + s0 / 2);
+  int16_t *at_o = reinterpret_cast<int16_t *>(spins + (uint64_t)(c ^ 1u) * cs + s0 / 2);
   if (wolff) {
     const uint32_t oc = (uint16_t)*at_c, oo = (uint16_t)*at_o;
     v[0] = v[0] == kKeep ? oc & 0xFFu : v[0];
@@ -672,35 +672,36 @@ bool tiled(const StencilGeom &g) {
 
 hipError_t launch_cluster_bonds_stencil(const StencilGeom &g, const int8_t *spins, uint32_t *parent, uint64_t key,
                                         uint64_t step, uint64_t thr, hipStream_t stream) {
+  if (g.cs < g.nh) return hipErrorInvalidValue;  // the colour stride is unset
   const uint64_t n = 2 * g.nh;
   const uint32_t lxs = 2 * g.lx2;
   if (g.nd == 2 && tiled(g)) {
     const uint64_t tiles = n / Tile<2>::N;
     hipLaunchKernelGGL(k_bonds_tile<2>, dim3((uint32_t)tiles), dim3(kThreads), 0, stream, spins, parent, lxs, g.ly,
-                       g.nh, key, step, thr);
+                       g.cs, key, step, thr);
     hipLaunchKernelGGL(k_bonds_faces<2>, dim3(blocks_for(tiles * 64)), dim3(kThreads), 0, stream, spins, parent, lxs,
-                       g.ly, g.lz, g.nh, tiles, key, step, thr);
+                       g.ly, g.lz, g.cs, tiles, key, step, thr);
     return hipGetLastError();
   }
   if (g.nd == 3 && tiled(g)) {
     const uint64_t tiles = n / Tile<3>::N;
     hipLaunchKernelGGL(k_bonds_tile<3>, dim3((uint32_t)tiles), dim3(kThreads), 0, stream, spins, parent, lxs, g.ly,
-                       g.nh, key, step, thr);
+                       g.cs, key, step, thr);
     hipLaunchKernelGGL(k_bonds_faces<3>, dim3(blocks_for(tiles * 320)), dim3(kThreads), 0, stream, spins, parent, lxs,
-                       g.ly, g.lz, g.nh, tiles, key, step, thr);
+                       g.ly, g.lz, g.cs, tiles, key, step, thr);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n)), dim3(kThreads), 0, stream, parent, n);
   const uint32_t lx = 2 * g.lx2;
   if (g.nd == 1)
     hipLaunchKernelGGL(k_bonds_stencil<1>, dim3(blocks_for(n)), dim3(kThreads), 0, stream, spins, parent, lx,
-                       g.ly, g.lz, g.nh, key, step, thr);
+                       g.ly, g.lz, g.nh, g.cs, key, step, thr);
   else if (g.nd == 2)
     hipLaunchKernelGGL(k_bonds_stencil<2>, dim3(blocks_for(n)), dim3(kThreads), 0, stream, spins, parent, lx,
-                       g.ly, g.lz, g.nh, key, step, thr);
+                       g.ly, g.lz, g.nh, g.cs, key, step, thr);
   else
     hipLaunchKernelGGL(k_bonds_stencil<3>, dim3(blocks_for(n)), dim3(kThreads), 0, stream, spins, parent, lx,
-                       g.ly, g.lz, g.nh, key, step, thr);
+                       g.ly, g.lz, g.nh, g.cs, key, step, thr);
   return hipGetLastError();
 }
 
@@ -756,28 +757,29 @@ hipError_t launch_cluster_transform(const StencilGeom *g, uint64_t n_sites, int8
                                     int kind, int q, int wolff, uint64_t key, uint64_t step, uint32_t *scratch,
                                     hipStream_t stream) {
   const int stencil = g != nullptr;
+  if (stencil && g->cs < g->nh) return hipErrorInvalidValue;
   const uint32_t lx = stencil ? 2 * g->lx2 : 0u, ly = stencil ? g->ly : 0u;
-  const uint64_t nh = stencil ? g->nh : 0u;
+  const uint64_t cs = stencil ? g->cs : 0u;  // bytes from colour 0 to colour 1
   if (wolff)
-    hipLaunchKernelGGL(k_wolff_seed, dim3(1), dim3(64), 0, stream, spins, parent, stencil, lx, ly, nh, n_sites, kind,
+    hipLaunchKernelGGL(k_wolff_seed, dim3(1), dim3(64), 0, stream, spins, parent, stencil, lx, ly, cs, n_sites, kind,
                        q, key, step, scratch);
   if (stencil && tiled(*g)) {
     const uint32_t tiles = (uint32_t)(n_sites / Tile<2>::N);
     if (g->nd == 2)
-      hipLaunchKernelGGL(k_transform_tile<2>, dim3(tiles), dim3(kThreads), 0, stream, spins, parent, lx, ly, nh,
+      hipLaunchKernelGGL(k_transform_tile<2>, dim3(tiles), dim3(kThreads), 0, stream, spins, parent, lx, ly, cs,
                          g->div_lx2, g->div_ly, kind, q, wolff, key, step, scratch);
     else
-      hipLaunchKernelGGL(k_transform_tile<3>, dim3(tiles), dim3(kThreads), 0, stream, spins, parent, lx, ly, nh,
+      hipLaunchKernelGGL(k_transform_tile<3>, dim3(tiles), dim3(kThreads), 0, stream, spins, parent, lx, ly, cs,
                          g->div_lx2, g->div_ly, kind, q, wolff, key, step, scratch);
     return hipGetLastError();
   }
   if (stencil && g->nd >= 2 && lx % 4 == 0) {
     hipLaunchKernelGGL(k_transform_stencil, dim3(blocks_for(n_sites / 4)), dim3(kThreads), 0, stream, spins, parent,
-                       lx, ly, nh, (uint32_t)n_sites, kind, q, wolff, key, step, scratch);
+                       lx, ly, cs, (uint32_t)n_sites, kind, q, wolff, key, step, scratch);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_transform, dim3(blocks_for(n_sites)), dim3(kThreads), 0, stream, spins, parent, stencil, lx, ly,
-                     nh, n_sites, kind, q, wolff, key, step, scratch);
+                     cs, n_sites, kind, q, wolff, key, step, scratch);
   return hipGetLastError();
 }
 

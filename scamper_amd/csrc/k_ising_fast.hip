@@ -167,8 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_ising_fast(int8_t *__restrict__ spin
     const uint32_t zz = (uint32_t)(rg / groups_per_plane);
     const uint32_t y0 = (uint32_t)(rg % groups_per_plane) * R;
     const uint32_t i0 = chunk * 16;
-    int8_t *__restrict__ own = spins + (2ull * rep + colour) * g.nh;
-    const int8_t *__restrict__ oth = spins + (2ull * rep + 1 - colour) * g.nh;
+    int8_t *__restrict__ own = spins + (2ull * rep + colour) * g.cs;
+    const int8_t *__restrict__ oth = spins + (2ull * rep + 1 - colour) * g.cs;
     // offsets inside one colour array: 32-bit when it has < 2^32 sites, so
     // every load takes one offset register beside the array's scalar base
     const Off lx2 = g.lx2;
@@ -353,6 +353,7 @@ bool ising_fast_path_ok(const StencilGeom &g) {
 hipError_t launch_ising_fast(const StencilGeom &g, int8_t *spins, const ReplicaParams *reps,
                              uint32_t n_rep, uint32_t colour, uint64_t sweep, long long *em,
                              hipStream_t st, const StepClock *clk) {
+  if (g.cs < g.nh) return hipErrorInvalidValue;  // the colour stride is unset
   const uint32_t R = fast_rows(g);
   const uint64_t threads = (uint64_t)(g.lx2 / 16) * (g.ly / R) * g.lz;
   dim3 grid((unsigned)((threads + kBlock - 1) / kBlock), n_rep);

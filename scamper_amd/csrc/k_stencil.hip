@@ -114,8 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_site(int8_t *__restrict__ spins,
   // grid-stride over sites: the grid is capped so that a measured launch
   // issues a bounded number of same-address atomics
   for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < g.nh; j += (uint64_t)gridDim.x * kBlock) {
-    int8_t *__restrict__ own = spins + (2ull * rep + colour) * g.nh;
-    const int8_t *__restrict__ oth = spins + (2ull * rep + 1 - colour) * g.nh;
+    int8_t *__restrict__ own = spins + (2ull * rep + colour) * g.cs;
+    const int8_t *__restrict__ oth = spins + (2ull * rep + 1 - colour) * g.cs;
     const SiteCoords sc_ = site_coords(j, g);
     const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     const uint32_t p = (colour + y + zz) & 1u;
@@ -221,8 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_measure(const int8_t *__restrict__ s
 #pragma unroll
   for (int k = 0; k < 2 + kMaxQ; ++k) acc[k] = 0;
   for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < g.nh; j += (uint64_t)gridDim.x * kBlock) {
-    const int8_t *own = spins + (2ull * rep + 1) * g.nh;
-    const int8_t *oth = spins + (2ull * rep) * g.nh;
+    const int8_t *own = spins + (2ull * rep + 1) * g.cs;
+    const int8_t *oth = spins + (2ull * rep) * g.cs;
     const SiteCoords sc_ = site_coords(j, g);
     const uint32_t row = sc_.row, i = sc_.i, y = sc_.y, zz = sc_.z;
     const uint32_t p = (1 + y + zz) & 1u;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_init(int8_t *__restrict__ spins, Ste
     v = kind == kIsing ? ((site_byte(key, site, 0, 0, kStreamInit, 0) & 1u) ? 1 : -1)
                        : (int8_t)(((uint64_t)site_word(key, site, 0, 0, kStreamInit) * (uint64_t)q) >> 32);
   }
-  spins[(2ull * rep + colour) * g.nh + j] = v;
+  spins[(2ull * rep + colour) * g.cs + j] = v;
 }
 
 template <int ND>
@@ -295,6 +295,7 @@ hipError_t launch_stencil_discrete(const StencilGeom &g, int kind, int q, int al
                                    const ReplicaParams *reps, uint32_t n_rep, uint32_t colour,
                                    uint64_t sweep, const double *hb_exp, int hb_min, long long *em,
                                    hipStream_t st) {
+  if (g.cs < g.nh) return hipErrorInvalidValue;  // the colour stride is unset
   if (kind == kIsing && alg == kMetropolis && ising_fast_path_ok(g))
     return launch_ising_fast(g, spins, reps, n_rep, colour, sweep, em, st);
   switch (g.nd) {
@@ -307,6 +308,7 @@ hipError_t launch_stencil_discrete(const StencilGeom &g, int kind, int q, int al
 
 hipError_t launch_stencil_measure(const StencilGeom &g, int kind, int q, const int8_t *spins,
                                   uint32_t n_rep, long long *em, hipStream_t st) {
+  if (g.cs < g.nh) return hipErrorInvalidValue;
   dim3 grid(capped_blocks(g.nh), n_rep);
 #define SC_M(ND)                                                                                   \
   if (kind == kIsing)                                                                              \
@@ -325,6 +327,7 @@ hipError_t launch_stencil_measure(const StencilGeom &g, int kind, int q, const i
 
 hipError_t launch_stencil_init(const StencilGeom &g, int kind, int q, int8_t *spins, uint64_t key,
                                uint32_t rep, int random, hipStream_t st) {
+  if (g.cs < g.nh) return hipErrorInvalidValue;
   dim3 grid((unsigned)((g.nh + kBlock - 1) / kBlock), 2);
   hipLaunchKernelGGL(k_init, grid, dim3(kBlock), 0, st, spins, g, kind, q, key, rep, random);
   return hipGetLastError();

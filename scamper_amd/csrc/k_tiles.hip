@@ -291,7 +291,8 @@ __device__ __forceinline__ uint32_t ne16(uint32_t a, uint32_t b) { return ((a ^ 
 template <int ND>
 __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
                                                const LaneDraws &D, const TileJob &J, uint64_t key,
-                                               uint32_t j0, uint64_t sweep, uint32_t colour, bool meas, Meas &M) {
+                                               uint32_t j0, uint64_t sweep, uint32_t colour, bool meas, Meas &M,
+                                               uint32_t vm) {
   constexpr int z = 2 * ND;
   uint32_t a2 = 0;  // spins are 0x01 / 0xff: bit 1 of s ^ n is set exactly when they differ
 #pragma unroll
@@ -300,10 +301,12 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t S, const uint32_t (&N)[2
   const uint32_t flip = spread_msb(accept_bytes<true>(anti, D, J, key, j0, sweep, colour));
   const uint32_t NS = S ^ (flip & 0xFEFEFEFEu);
   if (meas) {
-    // a flipped site has z - anti anti-aligned neighbours
-    const uint32_t b2 = pick((uint32_t)(2 * z) * kOnes - a2, a2, flip);
+    // a flipped site has z - anti anti-aligned neighbours; a missing site
+    // counts z / 2 of them, which adds z - 2 (z / 2) = 0
+    const uint32_t b2 = pick(pick((uint32_t)(2 * z) * kOnes - a2, a2, flip), (uint32_t)z * kOnes, vm);
     M.e += 4 * z - 2 * (int)__builtin_amdgcn_sad_u8(b2 >> 1, 0u, 0u);
-    M.m += 8 - 2 * (__builtin_popcount(NS & kMsb) + __builtin_popcount(C & kMsb));
+    M.m += 2 * __builtin_popcount(vm & kMsb) -
+           2 * (__builtin_popcount(NS & vm & kMsb) + __builtin_popcount(C & vm & kMsb));
   }
   return NS;
 }
@@ -312,7 +315,7 @@ template <int ND>
 __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2 * ND], uint32_t C,
                                                const LaneDraws &D, const TileJob &J, uint32_t q,
                                                uint64_t key, uint32_t j0, uint64_t sweep, uint32_t colour, bool meas,
-                                               Meas &M, Nibbles &B) {
+                                               Meas &M, Nibbles &B, uint32_t vm) {
   constexpr int z = 2 * ND;
   uint32_t R, idx;  // proposed states; table index d + z = z + #(n != s) - #(n != r) per byte, 0 .. 2z
   uint32_t ds = 0, dr;  // 16 #(n != s), 16 #(n != r) per byte
@@ -338,8 +341,8 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
       const uint32_t pe = (D.ptop & 0x00FF00FFu) * m8, po = ((D.ptop >> 8) & 0x00FF00FFu) * m8;
       a = ((pe >> 8) & 0x000F000Fu) | (po & 0x0F000F00u);
       const uint32_t bias = (m + 255u) * 0x00010001u;
-      uint32_t strad = ((((pe & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) |
-                       (((((po & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) << 8);
+      uint32_t strad = (((((pe & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) |
+                        (((((po & 0x00FF00FFu) + bias) >> 9) & 0x00010001u) << 8));
       while (strad) {
         const uint32_t k = (uint32_t)__builtin_ctz(strad) >> 3;
         strad &= strad - 1;
@@ -360,14 +363,18 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
   const uint32_t NS = pick(R, S, am);
   if (meas) {
     // bonds to unlike neighbours after the update: those of r where accepted
-    M.e += 4 * z - (int)(__builtin_amdgcn_sad_u8(pick(dr, ds, am), 0u, 0u) >> 4);
+    // (a missing site: z unlike bonds, so it adds nothing)
+    M.e += 4 * z - (int)(__builtin_amdgcn_sad_u8(pick(pick(dr, ds, am), (uint32_t)(16 * z) * kOnes, vm), 0u, 0u) >> 4);
     // sites in each state among the four updated and the four other-colour
     // sites: v_perm look-ups give the nibble counts of states 0 .. 3 (q <= 4)
-    // or 1 in every byte in state v (q <= 8)
+    // or 1 in every byte in state v (q <= 8); a missing site's byte becomes
+    // a selector of zero (v_perm) or a value of no state (byte compares)
+    const uint32_t none = q <= 8u ? 0x0C0C0C0Cu : 0xFFFFFFFFu;
+    const uint32_t NSm = pick(NS, none, vm), Cm = pick(C, none, vm);
     if (q <= 4u) {
       constexpr uint32_t k01 = 0x00001001u, k23 = 0x10010000u;  // state -> nibble
-      B.n01 += __builtin_amdgcn_perm(0u, k01, NS) + __builtin_amdgcn_perm(0u, k01, C);
-      if (q > 2u) B.n23 += __builtin_amdgcn_perm(0u, k23, NS) + __builtin_amdgcn_perm(0u, k23, C);
+      B.n01 += __builtin_amdgcn_perm(0u, k01, NSm) + __builtin_amdgcn_perm(0u, k01, Cm);
+      if (q > 2u) B.n23 += __builtin_amdgcn_perm(0u, k23, NSm) + __builtin_amdgcn_perm(0u, k23, Cm);
       return NS;
     }
 #pragma unroll
@@ -376,11 +383,11 @@ __device__ __forceinline__ uint32_t potts_word(uint32_t S, const uint32_t (&N)[2
       uint32_t n;
       if (q <= 8u) {
         const uint32_t lo = v < 4u ? 1u << (8u * v) : 0u, hi = v >= 4u ? 1u << (8u * (v - 4u)) : 0u;
-        n = __builtin_amdgcn_sad_u8(__builtin_amdgcn_perm(hi, lo, NS), 0u,
-                                    __builtin_amdgcn_sad_u8(__builtin_amdgcn_perm(hi, lo, C), 0u, 0u));
+        n = __builtin_amdgcn_sad_u8(__builtin_amdgcn_perm(hi, lo, NSm), 0u,
+                                    __builtin_amdgcn_sad_u8(__builtin_amdgcn_perm(hi, lo, Cm), 0u, 0u));
       } else {
         const uint32_t vv = v * kOnes;
-        n = __builtin_amdgcn_sad_u8(eq_bytes(NS, vv) >> 7, 0u, __builtin_amdgcn_sad_u8(eq_bytes(C, vv) >> 7, 0u, 0u));
+        n = __builtin_amdgcn_sad_u8(eq_bytes(NSm, vv) >> 7, 0u, __builtin_amdgcn_sad_u8(eq_bytes(Cm, vv) >> 7, 0u, 0u));
       }
       if (v < 8u) M.cnt_lo += (uint64_t)n << (8u * v); else M.cnt_hi += (uint64_t)n << (8u * (v - 8u));
     }
@@ -410,8 +417,10 @@ struct LaneSpan {
     const uint32_t zz = ND == 3 ? div_by(row, J.div_ly) : 0u;
     const uint32_t y = ND == 1 ? 0u : row - zz * ly;
     left = lx2 - i0;
-    two_rows = left < W;
-    ks = two_rows ? left : W;
+    // a lane past the end of the colour class has its sites in the last
+    // row: row b would lie outside the lattice
+    two_rows = left < W && j0 + W <= J.nh;
+    ks = left < W ? left : W;
     pa = (colour + y + zz) & 1u;
     const uint32_t yb = ND == 1 ? 0u : (y + 1u == ly ? 0u : y + 1u);
     const uint32_t zb = ND == 3 ? (y + 1u == ly ? zz + 1u : zz) : 0u;
@@ -460,8 +469,8 @@ __device__ __forceinline__ void run_tile4(const TileJob &J, uint32_t tile, uint6
   const uint32_t g = j0 >> 4;
   const bool group_active = (g << 4) < nh;
   if (group_active) {
-    int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
-    const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * nh;
+    int8_t *__restrict__ own = J.spins + (uint64_t)colour * J.cs;
+    const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * J.cs;
     const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
     const bool vec = lx2 >= 4u && j0 + 4u <= nh;
     g8 *const gown = (g8 *)own;
@@ -495,8 +504,8 @@ __device__ __forceinline__ void run_tile4(const TileJob &J, uint32_t tile, uint6
 #pragma unroll
       for (int c = 0; c < z - 2; ++c) N[2 + c] = pick(Na[c], Nb[c], amask);
       Nibbles B;
-      const uint32_t NS = KIND == kIsing ? ising_word<ND>(O, N, C, D, J, key, j0, sweep, colour, meas, M)
-                                         : potts_word<ND>(O, N, C, D, J, q, key, j0, sweep, colour, meas, M, B);
+      const uint32_t NS = KIND == kIsing ? ising_word<ND>(O, N, C, D, J, key, j0, sweep, colour, meas, M, ~0u)
+                                         : potts_word<ND>(O, N, C, D, J, q, key, j0, sweep, colour, meas, M, B, ~0u);
       if (KIND == kPotts && meas && q <= 4u) fold_nibbles(M, B);
       if (NS != O) store4(gown + j0, NS);
     } else {
@@ -547,10 +556,16 @@ __device__ __forceinline__ void run_tile16(const TileJob &J, uint32_t tile, uint
   const bool need_pmid = need_prop && (m & (m - 1)) != 0;
 
   if (j0 < nh) {
-    int8_t *__restrict__ own = J.spins + (uint64_t)colour * nh;
-    const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * nh;
+    int8_t *__restrict__ own = J.spins + (uint64_t)colour * J.cs;
+    const int8_t *__restrict__ oth = J.spins + (uint64_t)(1 - colour) * J.cs;
     const uint32_t lx2 = J.lx2, ly = J.ly, lz = J.lz;
-    const bool vec = lx2 >= 16u && j0 + 16u <= nh;
+    // Rows of >= 16 sites: 16-byte windows.  A lane with fewer than 16 ranks
+    // left lies inside the class's last row; its windows reach into the gap
+    // of >= 16 bytes after the colour array, and its missing sites are
+    // masked out (so no lane falls back to the site-by-site path there).
+    // The rank test is always true (j0 < nh, a multiple of 16), written so
+    // that the compiler keeps the register allocation of the full lanes.
+    const bool vec = lx2 >= 16u && j0 + 16u <= ((nh + 15u) & ~15u);
     g8 *const gown = (g8 *)own;
     const g8 *const obase = (const g8 *)oth - kSpinSlack;
     const uint32_t js = j0 + kSpinSlack;
@@ -597,6 +612,12 @@ __device__ __forceinline__ void run_tile16(const TileJob &J, uint32_t tile, uint
       W16 NS;
       Nibbles B;
       bool changed = false;
+      // Where the colour class ends inside the lane, the bytes past its end
+      // hold no site: they are updated like sites (every byte-wise carry
+      // and borrow runs towards higher bytes, away from the real sites
+      // below them), written back into the gap, and left out of the
+      // measured counters (vm: 0xff in the bytes of real sites).
+      const uint32_t nv = min(nh - j0, 16u);
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
         uint32_t N[z];
@@ -607,8 +628,10 @@ __device__ __forceinline__ void run_tile16(const TileJob &J, uint32_t tile, uint
         const LaneDraws D{word_of(T, (uint32_t)w), word_of(Mb, (uint32_t)w), word_of(PT, (uint32_t)w),
                           word_of(PM, (uint32_t)w)};
         const uint32_t S = O.d[w], jw = j0 + 4u * (uint32_t)w;
-        NS.d[w] = KIND == kIsing ? ising_word<ND>(S, N, C.d[w], D, J, key, jw, sweep, colour, meas, M)
-                                 : potts_word<ND>(S, N, C.d[w], D, J, q, key, jw, sweep, colour, meas, M, B);
+        const uint32_t have = nv - min(nv, 4u * (uint32_t)w);  // real sites from byte 0 of this word on
+        const uint32_t vm = have >= 4u ? ~0u : (1u << (8u * have)) - 1u;
+        NS.d[w] = KIND == kIsing ? ising_word<ND>(S, N, C.d[w], D, J, key, jw, sweep, colour, meas, M, vm)
+                                 : potts_word<ND>(S, N, C.d[w], D, J, q, key, jw, sweep, colour, meas, M, B, vm);
         changed |= NS.d[w] != S;
       }
       if (KIND == kPotts && meas && q <= 4u) fold_nibbles(M, B);

@@ -76,6 +76,7 @@ extern "C" int scamper_ising_sweep_noalloc(int8_t *d_spins, const uint64_t *dims
   g.ly = (uint32_t)dims[1];
   g.lz = n_dims == 3 ? (uint32_t)dims[2] : 1u;
   g.nh = (uint64_t)g.lx2 * g.ly * g.lz;
+  g.cs = g.nh;  // the caller's colour arrays are contiguous
   set_divisors(g);
   if (!ising_fast_path_ok(g))
     return fail(SCAMPER_EUNSUPPORTED, "the no-allocation sweep needs Lx a multiple of 32 (got %llu)",

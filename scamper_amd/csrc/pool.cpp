@@ -70,8 +70,8 @@ struct Job {
   Threshold thr[kStencilTable]{};
   std::unique_ptr<DefaultObservables> obs;
   std::vector<double> e_series, m_series;
-  std::vector<int8_t> split_spins;  // colour-split copy of the final spins
-  int8_t *h_spins = nullptr;        // pinned staging of the final spins' copy
+  std::vector<int8_t> split_spins;  // copy of the final spins' two colour arrays (2 cs bytes)
+  int8_t *h_spins = nullptr;        // pinned staging of that copy
   scamper_estimate est[5]{};
   ~Job() {
     if (h_spins) hipHostFree(h_spins);
@@ -276,13 +276,13 @@ void unsplit(const Job &J, const int8_t *split, int8_t *out) {
     const uint64_t x = s % lx, row = s / lx;
     const uint64_t y = row % g.ly, zz = row / g.ly;
     const uint32_t c = (uint32_t)((x + y + zz) & 1);
-    out[s] = split[c * g.nh + row * g.lx2 + (x >> 1)];
+    out[s] = split[c * g.cs + row * g.lx2 + (x >> 1)];
   }
 }
 
 void finish(scamper_pool *P, Job &J) {
   if (J.h_spins) {  // its copy completed before the chunk's end event
-    J.split_spins.assign(J.h_spins, J.h_spins + J.n_sites);
+    J.split_spins.assign(J.h_spins, J.h_spins + 2 * J.geom.cs);
     hipHostFree(J.h_spins);
     J.h_spins = nullptr;
   }
@@ -390,9 +390,9 @@ int admit(scamper_pool *P, Job &J) {
   P->in_use += J.bytes;
   if (J.total == 0) {  // nothing to run: done at admission
     if (J.spec.keep_spins) {
-      J.split_spins.resize(J.n_sites);
+      J.split_spins.resize(2 * J.geom.cs);
       SC_HIP(hipStreamSynchronize(P->stream));
-      SC_HIP(hipMemcpy(J.split_spins.data(), P->spins + J.off, J.n_sites, hipMemcpyDeviceToHost));
+      SC_HIP(hipMemcpy(J.split_spins.data(), P->spins + J.off, 2 * J.geom.cs, hipMemcpyDeviceToHost));
     }
     P->arena.release(J.off, J.bytes);
     P->in_use -= J.bytes;
@@ -516,6 +516,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
     t.ly = J.geom.ly;
     t.lz = J.geom.lz;
     t.nh = (uint32_t)J.geom.nh;
+    t.cs = (uint32_t)J.geom.cs;
     t.kind = (uint32_t)J.spec.model.kind;
     t.q = (uint32_t)J.spec.model.q;
     t.tile_start = tiles;
@@ -561,9 +562,9 @@ int step(scamper_pool *P, uint32_t *n_completed) {
     J.enqueued += ri.steps;
     if (J.enqueued == J.total && J.spec.keep_spins) {
       // asynchronous copies go to pinned memory only; finish() moves it
-      if (hipHostMalloc(&J.h_spins, J.n_sites, hipHostMallocDefault) != hipSuccess)
+      if (hipHostMalloc(&J.h_spins, 2 * J.geom.cs, hipHostMallocDefault) != hipSuccess)
         return fail(SCAMPER_ENOMEM, "pinned staging of %llu final spins", (unsigned long long)J.n_sites);
-      SC_HIP(hipMemcpyAsync(J.h_spins, P->spins + J.off, J.n_sites, hipMemcpyDeviceToHost, P->stream));
+      SC_HIP(hipMemcpyAsync(J.h_spins, P->spins + J.off, 2 * J.geom.cs, hipMemcpyDeviceToHost, P->stream));
     }
   }
   SC_HIP(hipEventRecord(P->ev_end[p], P->stream));
@@ -656,7 +657,8 @@ int scamper_pool_submit(scamper_pool *P, const scamper_job_spec *spec, uint64_t 
   J->geom.nh = n / 2;
   set_divisors(J->geom);
   J->n_sites = n;
-  J->bytes = (n + kAlign - 1) / kAlign * kAlign;
+  J->geom.cs = aligned_colour(J->geom.nh);  // colour arrays at aligned offsets
+  J->bytes = (2 * J->geom.cs + kAlign - 1) / kAlign * kAlign;
   if (J->bytes > P->cfg.spin_budget_bytes)
     return fail(SCAMPER_EBUDGET, "job needs %llu spin bytes, pool budget is %llu", (unsigned long long)J->bytes,
                 (unsigned long long)P->cfg.spin_budget_bytes);

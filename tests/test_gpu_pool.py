@@ -178,14 +178,17 @@ def test_autotuner_waits_for_enough_work(sc):
 
 
 def test_admission_by_budget(sc):
-    # room for two 64x64 jobs (4096 bytes each) at a time; never over budget
+    # room for two 64x64 jobs at a time; never over budget.  A job holds its
+    # two colour arrays of 2048 sites, each padded to a multiple of 256 bytes
+    # with a gap of at least 16: 2 * 2304 bytes.
+    job_bytes = 2 * 2304
     specs = [_spec(sc, (64, 64), 1, 3, 1.0, 5, 40 + 3 * i, 300 + i) for i in range(6)]
-    pool = sc.Pool(spin_budget_bytes=2 * 4096, max_jobs=16, chunk_sweeps=8)
+    pool = sc.Pool(spin_budget_bytes=2 * job_bytes, max_jobs=16, chunk_sweeps=8)
     ids = [pool.submit(s) for s in specs]
     peak_running = 0
     while True:
         st = pool.stats()
-        assert st.spin_bytes_in_use <= 2 * 4096
+        assert st.spin_bytes_in_use <= 2 * job_bytes
         peak_running = max(peak_running, st.running)
         if st.done == len(specs):
             break

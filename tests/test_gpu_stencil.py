@@ -175,3 +175,4 @@ def test_cold_start_observables(sc):
         st = sc.State(lat, sc.model(0), 1, 0)
         e, m = st.observables()
         assert e == -float(lat.n_edges) and m == 1.0
+
