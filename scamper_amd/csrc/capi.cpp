@@ -1268,6 +1268,12 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
 
 }  // namespace
 
+// Bytes of O(N) stencil spins swept as one group of replicas (see
+// scamper_sweep), inside the 256 MB Infinity Cache: C4 (340 MB) runs as two
+// groups of 16 replicas (123 -> 147 site-updates/ns); a state that already
+// fits (XY at 96^3 x 32, 226 MB) stays one group, splitting it cost 2%.
+constexpr uint64_t kReplicaBlockBytes = 240ull << 20;
+
 int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measure) {
   if (!st) return fail(SCAMPER_EINVAL, "null state");
   if (algorithm == kSwendsenWang || algorithm == kWolff) return cluster_sweeps(st, algorithm, n_sweeps, measure);
@@ -1286,37 +1292,58 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
     SC_HIP(hipMemsetAsync(st->d_series + st->series_len * slot, 0, n_sweeps * slot * sizeof(long long), st->stream));
     if (int rc = ensure_ring(st, algorithm)) return rc;
   }
+  if (st->continuous && !st->csr) {
+    // O(N) on a stencil: every sweep of a group of replicas before the next
+    // group, each group small enough to stay in the 256 MB Infinity Cache
+    // across its half-sweeps (C4's 32 replicas at 96^3 are 340 MB, which
+    // would stream from HBM at every launch).  Draws are keyed by (replica,
+    // sweep, colour, rank), so the order of groups changes no result.
+    const int d = st->model.spin_dim;
+    const uint64_t per_rep = 2ull * d * st->geom.nh * sizeof(float);
+    const uint64_t groups = std::max<uint64_t>(1, (per_rep * st->n_rep + kReplicaBlockBytes - 1) / kReplicaBlockBytes);
+    const uint32_t g_size = (uint32_t)((st->n_rep + groups - 1) / groups);
+    // HybridCore: heat bath then over-relaxation each sweep (hybrid.rs:46-59)
+    const int parts[2] = {algorithm == kOverRelaxation ? kOverRelaxation : kContinuousHeatBath,
+                          algorithm == kHybridHeatBathOverRelaxation ? kOverRelaxation : -1};
+    for (uint32_t r0 = 0; r0 < st->n_rep; r0 += g_size) {
+      const uint32_t nr = std::min(g_size, st->n_rep - r0);
+      float *spins = st->d_vec + (uint64_t)r0 * 2 * d * st->geom.nh;
+      for (uint64_t t = 0; t < n_sweeps; ++t) {
+        for (int pi = 0; pi < 2; ++pi) {
+          if (parts[pi] < 0) continue;
+          const bool last = pi == 1 || parts[1] < 0;
+          for (uint32_t c = 0; c < 2; ++c) {
+            double *em = (measure && last && c == 1)
+                             ? reinterpret_cast<double *>(st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride) +
+                                   (size_t)r0 * kEmStride
+                             : nullptr;
+            hipError_t e = launch_on3_sweep(st->geom, d, parts[pi], spins, st->d_params + r0, nr, c, st->sweeps + t,
+                                            (float)st->model.j, em, st->stream);
+            if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+          }
+        }
+      }
+    }
+    st->sweeps += n_sweeps;
+    if (measure) st->series_len += n_sweeps;
+    return SCAMPER_OK;
+  }
   for (uint64_t t = 0; t < n_sweeps; ++t) {
     const uint64_t sweep = st->sweeps + t;
-    if (st->continuous) {
+    if (st->continuous) {  // a graph lattice (stencils took the grouped loop above)
       // HybridCore: heat bath then over-relaxation each sweep (hybrid.rs:46-59)
       const int parts[2] = {algorithm == kOverRelaxation ? kOverRelaxation : kContinuousHeatBath,
                             algorithm == kHybridHeatBathOverRelaxation ? kOverRelaxation : -1};
-      if (st->csr) {
-        // graph lattice: colour classes in order, then the measurement
-        hipError_t e = hipSuccess;
-        for (int pi = 0; pi < 2 && e == hipSuccess; ++pi)
-          if (parts[pi] >= 0)
-            e = launch_on_csr_sweep(st->ogeom, st->model.spin_dim, parts[pi], st->d_vec, st->d_params, st->n_rep,
-                                    st->class_start.data(), (uint32_t)(st->class_start.size() - 1), sweep,
-                                    st->model.j, st->stream);
-        if (e == hipSuccess && measure)
-          e = measure_on(st, reinterpret_cast<double *>(st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride));
-        if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
-        continue;
-      }
-      for (int pi = 0; pi < 2; ++pi) {
-        if (parts[pi] < 0) continue;
-        const bool last = pi == 1 || parts[1] < 0;
-        for (uint32_t c = 0; c < 2; ++c) {
-          double *em = (measure && last && c == 1)
-                           ? reinterpret_cast<double *>(st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride)
-                           : nullptr;
-          hipError_t e = launch_on3_sweep(st->geom, st->model.spin_dim, parts[pi], st->d_vec, st->d_params, st->n_rep, c, sweep,
-                                          (float)st->model.j, em, st->stream);
-          if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
-        }
-      }
+      // colour classes in order, then the measurement
+      hipError_t e = hipSuccess;
+      for (int pi = 0; pi < 2 && e == hipSuccess; ++pi)
+        if (parts[pi] >= 0)
+          e = launch_on_csr_sweep(st->ogeom, st->model.spin_dim, parts[pi], st->d_vec, st->d_params, st->n_rep,
+                                  st->class_start.data(), (uint32_t)(st->class_start.size() - 1), sweep,
+                                  st->model.j, st->stream);
+      if (e == hipSuccess && measure)
+        e = measure_on(st, reinterpret_cast<double *>(st->d_series + (st->series_len + t) * (size_t)st->n_rep * kEmStride));
+      if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
       continue;
     }
     if (st->csr) {
