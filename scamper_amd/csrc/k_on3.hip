@@ -106,10 +106,41 @@ __device__ __forceinline__ V3 local_field(const float *__restrict__ oth, const S
 // about the local field h from the word w0, the azimuth from w1, rotated
 // into h's frame.  A vanishing field draws a uniform direction from a block
 // of its own (stream kStreamContFallback, by rank).
+//
+// The arithmetic uses the hardware's exp2, log2, reciprocal and square-root
+// instructions (about 1 ulp each): log1p and expm1 are formed from log and
+// exp with the correction of Goldberg and Kahan (f(w) z / (w - 1), which keeps
+// the relative accuracy of f at small arguments), divisions are products with
+// a reciprocal, and square roots of sums of squares come from one reciprocal
+// square root.  FP32 spins are renormalised after every update, and the
+// error against the FP64 oracle stays at the level of the FP32 arithmetic.
+__device__ __forceinline__ float fast_log1p(float z) {  // z > -1
+  const float w = 1.0f + z;
+  if (w == 1.0f) return z;
+  const float l = __builtin_amdgcn_logf(w) * 0.69314718f;  // ln w
+  return w == __builtin_huge_valf() ? w : l * (z * __builtin_amdgcn_rcpf(w - 1.0f));
+}
+
+__device__ __forceinline__ float fast_expm1(float t) {  // t >= 0
+  const float e = __builtin_amdgcn_exp2f(t * 1.44269504f);  // e^t
+  if (e == 1.0f) return t;
+  if (e == __builtin_huge_valf()) return e;
+  return (e - 1.0f) * (t * __builtin_amdgcn_rcpf(__builtin_amdgcn_logf(e) * 0.69314718f));
+}
+
+__device__ __forceinline__ V3 unit_fast(V3 v) {
+  const float n2 = v.x * v.x + v.y * v.y + v.z * v.z;
+  if (!(n2 > 1e-30f)) return v;
+  const float inv = __builtin_amdgcn_rsqf(n2);
+  return V3{v.x * inv, v.y * inv, v.z * inv};
+}
+
 __device__ __forceinline__ V3 heat_bath_site(V3 h, const ReplicaParams &P, uint64_t j, uint64_t sweep,
                                              uint32_t colour, uint32_t w0, uint32_t w1) {
   const float beta = (float)P.beta;
-  const float norm = sqrtf(h.x * h.x + h.y * h.y + h.z * h.z);
+  const float n2 = h.x * h.x + h.y * h.y + h.z * h.z;
+  const float inv_n = n2 > 0.0f ? __builtin_amdgcn_rsqf(n2) : 0.0f;
+  const float norm = n2 * inv_n;
   const float kappa = beta * norm;
   if (kappa < 1e-12f)
     return random_unit(philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
@@ -127,32 +158,32 @@ __device__ __forceinline__ V3 heat_bath_site(V3 h, const ReplicaParams &P, uint6
   // cos < 0 exactly when u < 1 / (1 + e^kappa); the choice needs no
   // precision, both sides being accurate near cos = 0.  Every lane takes one
   // log1p.
-  const float E = expm1f(2.0f * kappa);  // inf from kappa ~ 44.4
+  const float E = fast_expm1(2.0f * kappa);  // inf from kappa ~ 44.4
+  const float ik = __builtin_amdgcn_rcpf(kappa);
   const bool tail = u * (1.0f + __expf(kappa)) < 1.0f;
   const float em = kappa < 40.0f ? -E * __builtin_amdgcn_rcpf(1.0f + E) : -1.0f;  // expm1(-2 kappa)
   const float y = (1.0f - u) * em;
   const bool via_e = tail || y < -0.5f;
-  const float l = log1pf(via_e ? u * E : y) / kappa;
+  const float l = fast_log1p(via_e ? u * E : y) * ik;
   float x = tail ? l : (via_e ? 2.0f - l : -l);  // the small one of a and b
   if (via_e && !(E <= 3.4028235e38f)) {
     // e^{2 kappa} beyond FP32 range (kappa > 44.3): ln p formed directly;
     // u dominates p here unless u is zero's stand-in
-    const float lp = logf(u + (1.0f - u) * expf(-2.0f * kappa)) / kappa;
+    const float lp = logf(u + (1.0f - u) * expf(-2.0f * kappa)) * ik;
     x = tail ? 2.0f + lp : -lp;
   }
   x = fminf(fmaxf(x, 0.0f), 2.0f);
   const float c = tail ? x - 1.0f : 1.0f - x;
-  const float sn = sqrtf(x * (2.0f - x));
+  const float sn = __builtin_amdgcn_sqrtf(x * (2.0f - x));
   float sphi, cphi;
   sincos_turns((float)w1 * 2.3283064365386963e-10f, &sphi, &cphi);
   const float lx = sn * cphi, ly = sn * sphi;
-  const float inv_n = 1.0f / norm;
   const float ux = h.x * inv_n, uy = h.y * inv_n, uz = h.z * inv_n;
-  const float tr = sqrtf(ux * ux + uy * uy);
-  if (tr < 1e-6f) return normalised(V3{lx, ly, uz >= 0.0f ? c : -c});
-  const float it = 1.0f / tr;
-  return normalised(V3{-uy * it * lx - ux * uz * it * ly + ux * c, ux * it * lx - uy * uz * it * ly + uy * c,
-                       tr * ly + uz * c});
+  const float t2 = ux * ux + uy * uy;
+  if (t2 < 1e-12f) return unit_fast(V3{lx, ly, uz >= 0.0f ? c : -c});
+  const float it = __builtin_amdgcn_rsqf(t2), tr = t2 * it;
+  return unit_fast(V3{-uy * it * lx - ux * uz * it * ly + ux * c, ux * it * lx - uy * uz * it * ly + uy * c,
+                      tr * ly + uz * c});
 }
 
 // Over-relaxation (MicrocanonicalCore, microcanonical.rs:54-75): s reflected
