@@ -11,7 +11,10 @@ def _bits(words, r):
     return (((words >> np.uint64(r)) & np.uint64(1)).astype(np.int8) * 2 - 1)
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 8), (32, 32), (16, 6, 4)])
+# (32, 32), (32, 6, 4) and (64,): rows of a multiple of 16 sites per colour,
+# where a group of 16 ranks lies in one row; the others cross rows, and
+# (48, 6) ends its colour class inside a group
+@pytest.mark.parametrize("dims", [(8, 8, 8), (32, 32), (16, 6, 4), (32, 6, 4), (64,), (48, 6)])
 def test_bits_equal_single_replica_runs(sc, dims):
     seed, n = 5, 4
     lat = sc.Lattice.hypercubic(list(dims))
@@ -51,10 +54,11 @@ def test_two_bundles_match_oracle(sc, orc):
         assert np.array_equal(got[b], w)
 
 
-def test_exchange_trajectory_matches_oracle(sc, orc):
+@pytest.mark.parametrize("dims", [(4, 4, 4), (32, 4, 4), (64,)])
+def test_exchange_trajectory_matches_oracle(sc, orc, dims):
     """Sweeps with per-label betas, then exchange rounds; the label
     permutation, energies and magnetisations follow the oracle exactly."""
-    dims, seed, steps = (4, 4, 4), 21, 40
+    seed, steps = 21, 40
     lat = sc.Lattice.hypercubic(list(dims))
     olat = orc.Lattice.hypercubic(list(dims))
     ladder = 4.30 * (4.72 / 4.30) ** (np.arange(64) / 63.0)
