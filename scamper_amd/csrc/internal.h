@@ -189,11 +189,13 @@ hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const
                            const StencilGeom &g, uint32_t n_bundles, uint32_t colour, uint64_t sweep,
                            unsigned long long *counts, hipStream_t stream, const StepClock *clk = nullptr);
 // With a clock, the round is clk->exchange and, when measure_to_clock is
-// set, the energies and |m| go to the clock's series slot ([slot][2][n]).
+// set, the energies and |m| go to the clock's series slot ([slot][2][n]);
+// with tick set the kernel then advances the clock as launch_clock_tick
+// would (sweep + 1, exchange + do_exchange, slot + measure_to_clock).
 hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const double *beta_by_label, uint32_t n,
                               double n_bonds, double n_sites, double j, uint64_t ex_key, uint64_t step, int do_exchange,
                               double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t stream,
-                              const StepClock *clk = nullptr, int measure_to_clock = 0);
+                              StepClock *clk = nullptr, int measure_to_clock = 0, int tick = 0);
 
 // ------------------------------------------------------------ tiled jobs
 // Checkerboard Metropolis of Ising and Potts spins for any number of

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
   __shared__ Threshold sthr[64 * (z + 1)];
   __shared__ uint32_t sk8[64][2];  // per bit: top byte of K by anti-aligned count (0xff where K = 2^32), v_perm table
   __shared__ uint64_t skey[64];
-  __shared__ unsigned long long sred[2 * 64];
+  __shared__ uint32_t sred[64];  // measured sweeps, per bit: anti-aligned bonds | +1 spins << 16 (block sums < 2^16)
   __shared__ uint32_t sq[kWaves][64 * 16];  // per wave: queued ties (lane | bit r << 6 | tied sites << 10)
   __shared__ uint32_t sfix[2][4][kBlock];   // per thread: settled accepts, byte layout
   for (uint32_t e = threadIdx.x; e < 64 * (z + 1); e += kBlock) {
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
     sk8[r][1] = lut[1];
     skey[r] = keys[bundle * 64 + r];
   }
-  if (MEAS && threadIdx.x < 128) sred[threadIdx.x] = 0;
+  if (MEAS && threadIdx.x < 64) sred[threadIdx.x] = 0;
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -368,22 +368,20 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
         up[4] ^= cc;
       }
       // per bit of the quarter: over the four sites of every word, then over
-      // the lanes of the wave with the same quarter, then over the block
+      // the 16 lanes of the row with the same quarter (two DPP rotates), then
+      // over the rows and waves of the block (LDS atomics); a lane's counts
+      // are below 2^8, so both halves of the packed word stay exact
       for (uint32_t i = 0; i < 8; ++i) {
         const uint32_t m = 0x01010101u << i, r = 8u * h + i;
-        unsigned long long anti = 0, plus = 0;
+        uint32_t anti = __popc(v[5] & m), plus = __popc(up[4] & m);
 #pragma unroll
-        for (int b = 0; b < 6; ++b) anti += (unsigned long long)__popc(v[b] & m) << b;
+        for (int b = 4; b >= 0; --b) anti = 2u * anti + __popc(v[b] & m);
 #pragma unroll
-        for (int b = 0; b < 5; ++b) plus += (unsigned long long)__popc(up[b] & m) << b;
-        for (int off = 4; off < 64; off <<= 1) {
-          anti += __shfl_xor(anti, off);
-          plus += __shfl_xor(plus, off);
-        }
-        if (lane < 4) {
-          atomicAdd(&sred[16 * lane + r], anti);
-          atomicAdd(&sred[64 + 16 * lane + r], plus);
-        }
+        for (int b = 3; b >= 0; --b) plus = 2u * plus + __popc(up[b] & m);
+        uint32_t x = anti | (plus << 16);
+        x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false);  // row_ror:8
+        x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x124, 0xF, 0xF, false);  // row_ror:4
+        if ((lane & 12u) == 0) atomicAdd(&sred[16 * (lane & 3u) + r], x);
       }
     }
   }
@@ -418,10 +416,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
         ok = site_low(key, j, sweep, colour, kStreamAccept) < t.k16;
       if (ok) {
         atomicOr(&sfix[r >> 3][k >> 2][wave * 64 + ol], 1u << (8 * (k & 3) + (r & 7)));
-        if (MEAS) {  // the flip turns a anti-aligned bonds into z - a, and the spin's sign
-          atomicAdd(&sred[obit], (unsigned long long)(long long)(z - 2 * (int)a));
-          atomicAdd(&sred[64 + obit], spin ? ~0ull : 1ull);
-        }
+        if (MEAS)  // the flip turns a anti-aligned bonds into z - a, and the spin's sign (mod 2^32)
+          atomicAdd(&sred[obit], (uint32_t)(z - 2 * (int)a) + (spin ? 0xFFFF0000u : 0x00010000u));
       }
     }
   }
@@ -443,8 +439,11 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
   }
   if (MEAS) {
     __syncthreads();
-    if (threadIdx.x < 128 && sred[threadIdx.x] != 0)
-      atomicAdd(&counts[(uint64_t)bundle * 128 + threadIdx.x], sred[threadIdx.x]);
+    if (threadIdx.x < 64) {
+      const uint32_t x = sred[threadIdx.x];
+      if (x & 0xFFFFu) atomicAdd(&counts[(uint64_t)bundle * 128 + threadIdx.x], (unsigned long long)(x & 0xFFFFu));
+      if (x >> 16) atomicAdd(&counts[(uint64_t)bundle * 128 + 64 + threadIdx.x], (unsigned long long)(x >> 16));
+    }
   }
 }
 
@@ -479,7 +478,7 @@ __global__ __launch_bounds__(1024) void k_ms_exchange(int32_t *__restrict__ labe
                                                       uint64_t step, int do_exchange, double *__restrict__ energy_out,
                                                       double *__restrict__ mag_out,
                                                       unsigned long long *__restrict__ stats,
-                                                      const StepClock *__restrict__ clk, int measure_to_clock) {
+                                                      StepClock *__restrict__ clk, int measure_to_clock, int tick) {
   if (clk) {
     step = clk->exchange;
     if (measure_to_clock) {
@@ -501,6 +500,13 @@ __global__ __launch_bounds__(1024) void k_ms_exchange(int32_t *__restrict__ labe
     cfg_of[labels[t]] = (int32_t)t;
   }
   __syncthreads();
+  // every thread has read the clock: the step's tick (the next kernel on
+  // the stream sees it)
+  if (tick && t == 0) {
+    clk->sweep += 1;
+    clk->exchange += do_exchange ? 1u : 0u;
+    clk->slot += measure_to_clock ? 1u : 0u;
+  }
   if (!do_exchange || n < 2) return;
   const uint32_t offset = (uint32_t)((step > 0 ? step - 1 : 0) & 1u);
   const uint32_t lower = offset + 2 * t;
@@ -576,9 +582,10 @@ hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const
 hipError_t launch_ms_exchange(int32_t *labels, unsigned long long *counts, const double *beta_by_label, uint32_t n,
                               double n_bonds, double n_sites, double j, uint64_t ex_key, uint64_t step, int do_exchange,
                               double *energy_out, double *mag_out, unsigned long long *stats, hipStream_t st,
-                              const StepClock *clk, int measure_to_clock) {
+                              StepClock *clk, int measure_to_clock, int tick) {
+  if (tick && !clk) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_ms_exchange, dim3(1), dim3(1024), 0, st, labels, counts, beta_by_label, n, n_bonds, n_sites, j,
-                     ex_key, step, do_exchange, energy_out, mag_out, stats, clk, measure_to_clock);
+                     ex_key, step, do_exchange, energy_out, mag_out, stats, clk, measure_to_clock, tick);
   return hipGetLastError();
 }
 

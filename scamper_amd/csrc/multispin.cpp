@@ -205,8 +205,8 @@ constexpr uint64_t kGraphSteps = 16;  // steps per captured chunk
 
 // One step on `st`, its sweep number, exchange round and series slot taken
 // from the device clock: the colour-0 and colour-1 sweeps, then the
-// exchange kernel (which also turns the counters into energies and records
-// them when measuring), then the clock's tick.
+// exchange kernel (which also turns the counters into energies, records
+// them when measuring, and advances the clock), or the clock's tick alone.
 int enqueue_ms_step(scamper_multispin *ms, int exchange, int measure, hipStream_t st) {
   const bool record = measure || exchange;
   const double n_bonds = (double)ms->z * (double)ms->n_sites / 2.0;
@@ -219,10 +219,11 @@ int enqueue_ms_step(scamper_multispin *ms, int exchange, int measure, hipStream_
   if (record) {
     hipError_t e = launch_ms_exchange(ms->d_labels, ms->d_counts, ms->d_beta, ms->n, n_bonds, (double)ms->n_sites, ms->j,
                                       ex_key, 0, exchange != 0, nullptr, nullptr, ms->d_stats, st, ms->d_clock,
-                                      measure != 0);
+                                      measure != 0, 1);
     if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "multispin exchange: %s", hipGetErrorString(e));
+    return SCAMPER_OK;
   }
-  SC_HIP(launch_clock_tick(ms->d_clock, exchange ? 1 : 0, measure ? 1 : 0, st));
+  SC_HIP(launch_clock_tick(ms->d_clock, 0, 0, st));
   return SCAMPER_OK;
 }
 
