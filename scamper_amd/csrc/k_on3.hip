@@ -477,6 +477,7 @@ template <int ND, int ALG, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_on2(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
                                                 StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
                                                 float jc, double *__restrict__ em, const StepClock *__restrict__ clk) {
+  static_assert(ALG == kOverRelaxation, "the O(2) heat bath is k_on2_hb");
   if (clk) sweep = clk->sweep;
   uint32_t rep, bx;
   xcd_item(nbx, &rep, &bx);
@@ -511,33 +512,14 @@ __global__ __launch_bounds__(kBlock) void k_on2(float *__restrict__ spins, const
     }
     const V2 s = load2(own, g.nh, j);
     V2 ns = s;
-    if (ALG == kContinuousHeatBath) {
-      const ReplicaParams &P = reps[rep];
-      const double norm = sqrt(hx * hx + hy * hy);
-      double out[2];
-      if (norm < 1e-15) {
-        const U4 fb = philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
-                                    (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32));
-        random_unit2(fb.x, fb.y, out);
-      } else {
-        // theta = a + atan2(hy, hx) as a rotation of the field's direction by a
-        double ca, sa;
-        von_mises_cs(P, j, sweep, colour, P.beta * norm, &ca, &sa);
-        const double ux = hx / norm, uy = hy / norm;
-        out[0] = ca * ux - sa * uy;
-        out[1] = sa * ux + ca * uy;
-      }
-      ns = V2{(float)out[0], (float)out[1]};
-    } else {
-      // over-relaxation (microcanonical.rs:54-75): s reflected about h
-      const float fx = (float)hx, fy = (float)hy;
-      const float nsq = fx * fx + fy * fy;
-      if (nsq >= 1e-28f) {
-        const float proj = (s.x * fx + s.y * fy) / nsq;
-        float rx = 2.0f * proj * fx - s.x, ry = 2.0f * proj * fy - s.y;
-        const float inv = rsqrtf(rx * rx + ry * ry);
-        ns = V2{rx * inv, ry * inv};
-      }
+    // over-relaxation (microcanonical.rs:54-75): s reflected about h
+    const float fx = (float)hx, fy = (float)hy;
+    const float nsq = fx * fx + fy * fy;
+    if (nsq >= 1e-28f) {
+      const float proj = (s.x * fx + s.y * fy) / nsq;
+      float rx = 2.0f * proj * fx - s.x, ry = 2.0f * proj * fy - s.y;
+      const float inv = rsqrtf(rx * rx + ry * ry);
+      ns = V2{rx * inv, ry * inv};
     }
     store2(own, g.nh, j, ns);
     if (MEAS) {
@@ -555,6 +537,163 @@ __global__ __launch_bounds__(kBlock) void k_on2(float *__restrict__ spins, const
       double v = acc[k];
       for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
       if ((threadIdx.x & 63) == 0) atomicAdd(&red[k], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) atomicAdd(&em[(uint64_t)rep * kEmStride + threadIdx.x], red[threadIdx.x]);
+  }
+}
+
+// The O(2) heat bath with four sites per lane.  A wave of the one-site
+// kernel runs the Best-Fisher loop until its slowest lane accepts.  Here a
+// lane owns four sites (ranks s * kBlock + lane of the block's 4 * kBlock,
+// s = 0 .. 3): it forms their fields and sampler constants first, then
+// takes one try per iteration and moves to its next pending site as soon
+// as one accepts, writing that site's new spin there, so a wave runs about
+// the sum of its lanes' tries over four sites instead of four maxima.
+// Every site's arithmetic (its field, the sampler constants, try k's draws
+// and tests, the rotation) is the one-site kernel's, in the same order: the
+// spins are the same bit for bit.
+constexpr int kHbSites = 4;
+
+// Local field of rank j of the colour `colour` in f64 (models.rs:579-595),
+// and the other colour's spin at rank j (for M).
+template <int ND>
+__device__ __forceinline__ void on2_field(const float *__restrict__ oth, const StencilGeom &g, uint32_t colour,
+                                          uint64_t j, float jc, double *hx, double *hy, V2 *centre) {
+  const SiteCoords c = site_coords(j, g);
+  const uint32_t p = (colour + c.y + c.z) & 1u;
+  const uint64_t rb = (uint64_t)c.row * g.lx2;
+  uint64_t nb[2 * ND];
+  nb[0] = rb + (p ? c.i : wrap_dec(c.i, g.lx2));
+  nb[1] = rb + (p ? wrap_inc(c.i, g.lx2) : c.i);
+  if (ND >= 2) {
+    nb[2] = ((uint64_t)(wrap_dec(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+    nb[3] = ((uint64_t)(wrap_inc(c.y, g.ly) + g.ly * c.z)) * g.lx2 + c.i;
+  }
+  if (ND >= 3) {
+    nb[4] = ((uint64_t)(c.y + g.ly * wrap_dec(c.z, g.lz))) * g.lx2 + c.i;
+    nb[5] = ((uint64_t)(c.y + g.ly * wrap_inc(c.z, g.lz))) * g.lx2 + c.i;
+  }
+  double x = 0.0, y = 0.0;
+  V2 ce{0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 2 * ND; ++k) {
+    const V2 v = load2(oth, g.nh, nb[k]);
+    x += (double)jc * (double)v.x;
+    y += (double)jc * (double)v.y;
+    if (nb[k] == j) ce = v;
+  }
+  *hx = x;
+  *hy = y;
+  *centre = ce;
+}
+
+template <int ND, bool MEAS>
+__global__ __launch_bounds__(kBlock) void k_on2_hb(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
+                                                   StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
+                                                   float jc, double *__restrict__ em, const StepClock *__restrict__ clk) {
+  if (clk) sweep = clk->sweep;
+  uint32_t rep, bx;
+  xcd_item(nbx, &rep, &bx);
+  // per site of the lane: kappa and r of the sampler, and the field's unit
+  // direction (ux, uy)
+  __shared__ double skr[kHbSites][2][kBlock];
+  __shared__ double sdir[kHbSites][2][kBlock];
+  const ReplicaParams P = reps[rep];
+  float *__restrict__ own = spins + (2ull * rep + colour) * 2ull * g.nh;
+  const float *__restrict__ oth = spins + (2ull * rep + 1 - colour) * 2ull * g.nh;
+  const uint64_t jb = (uint64_t)bx * (kBlock * kHbSites) + threadIdx.x;
+
+  uint32_t pending = 0;  // bit s: site s still draws
+#pragma unroll
+  for (int s = 0; s < kHbSites; ++s) {
+    const uint64_t j = jb + (uint64_t)s * kBlock;
+    if (j >= g.nh) continue;
+    double hx, hy;
+    V2 ce;
+    on2_field<ND>(oth, g, colour, j, jc, &hx, &hy, &ce);
+    const double norm = sqrt(hx * hx + hy * hy);
+    if (norm < 1e-15) {
+      const U4 fb = philox4x32_10((uint32_t)j, (uint32_t)sweep, ctr2(sweep, colour, kStreamContFallback),
+                                  (uint32_t)(j >> 32), (uint32_t)P.key, (uint32_t)(P.key >> 32));
+      double out[2];
+      random_unit2(fb.x, fb.y, out);
+      store2(own, g.nh, j, V2{(float)out[0], (float)out[1]});
+      continue;
+    }
+    const double kappa = P.beta * norm;
+    const double ux = hx / norm, uy = hy / norm;
+    if (kappa < 1e-8) {
+      double ca, sa;
+      sincos((double)vm_block(P, j, sweep, colour, 0).x * kTwoM32 * kTau, &sa, &ca);
+      store2(own, g.nh, j, V2{(float)(ca * ux - sa * uy), (float)(sa * ux + ca * uy)});
+      continue;
+    }
+    const double tau = 1.0 + sqrt(1.0 + 4.0 * kappa * kappa);
+    const double rho = (tau - sqrt(2.0 * tau)) / (2.0 * kappa);
+    skr[s][0][threadIdx.x] = kappa;
+    skr[s][1][threadIdx.x] = (1.0 + rho * rho) / (2.0 * rho);
+    sdir[s][0][threadIdx.x] = ux;
+    sdir[s][1][threadIdx.x] = uy;
+    pending |= 1u << s;
+  }
+
+  // Best-Fisher tries (sample_von_mises, models.rs:666-689): try k of rank j
+  // takes Philox block (j, k); the 64th try is kept whatever its test says
+  uint32_t s = pending ? (uint32_t)__builtin_ctz(pending) : (uint32_t)kHbSites, k = 0;
+  double kappa = s < kHbSites ? skr[s][0][threadIdx.x] : 0.0, r = s < kHbSites ? skr[s][1][threadIdx.x] : 0.0;
+  while (__any(s < kHbSites)) {
+    if (s < kHbSites) {
+      const uint64_t j = jb + (uint64_t)s * kBlock;
+      const U4 w = vm_block(P, j, sweep, colour, k);
+      const double zc = cos(kPi * ((double)w.x * kTwoM32));
+      const double value = (1.0 + r * zc) / (r + zc);
+      const double c = kappa * (r - value);
+      const double u = fmax((double)w.y * kTwoM32, 2.2250738585072014e-308);
+      if (c * (2.0 - c) > u || log(c / u) + 1.0 >= c || k == 63) {
+        // theta = a + atan2(hy, hx) as the field's direction rotated by a
+        const double ca = fmin(fmax(value, -1.0), 1.0);
+        const double sn = sqrt((1.0 - ca) * (1.0 + ca));
+        const double sa = w.z < 0x80000000u ? sn : -sn;
+        const double ux = sdir[s][0][threadIdx.x], uy = sdir[s][1][threadIdx.x];
+        store2(own, g.nh, j, V2{(float)(ca * ux - sa * uy), (float)(sa * ux + ca * uy)});
+        pending &= pending - 1;  // s is the lowest pending site
+        s = pending ? (uint32_t)__builtin_ctz(pending) : (uint32_t)kHbSites;
+        k = 0;
+        if (s < kHbSites) {
+          kappa = skr[s][0][threadIdx.x];
+          r = skr[s][1][threadIdx.x];
+        }
+      } else {
+        ++k;
+      }
+    }
+  }
+
+  if (MEAS) {
+    // the new spins (written by this lane) against their fields, and the
+    // other colour's spins at the same ranks
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < kHbSites; ++t) {
+      const uint64_t j = jb + (uint64_t)t * kBlock;
+      if (j >= g.nh) continue;
+      double hx, hy;
+      V2 ce;
+      on2_field<ND>(oth, g, colour, j, jc, &hx, &hy, &ce);
+      const V2 ns = load2(own, g.nh, j);
+      acc[0] += (double)ns.x * hx + (double)ns.y * hy;
+      acc[1] += (double)ns.x + (double)ce.x;
+      acc[2] += (double)ns.y + (double)ce.y;
+    }
+    __shared__ double red[4];
+    if (threadIdx.x < 4) red[threadIdx.x] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double v = acc[c];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if ((threadIdx.x & 63) == 0) atomicAdd(&red[c], v);
     }
     __syncthreads();
     if (threadIdx.x < 3) atomicAdd(&em[(uint64_t)rep * kEmStride + threadIdx.x], red[threadIdx.x]);
@@ -786,8 +925,18 @@ hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins
   else                                                                                                                \
     hipLaunchKernelGGL((k_on2<ND, A, false>), grid, dim3(kBlock), 0, st, spins, reps, g, nbx, colour, sweep, j, em,   \
                        clk);
-#define SC_ON2_ND(ND)                                                  \
-  if (alg == kContinuousHeatBath) { SC_ON2(ND, kContinuousHeatBath) } \
+    const uint32_t nbh = (uint32_t)((g.nh + kBlock * kHbSites - 1) / (kBlock * kHbSites));
+    const bool flat_hb = (uint64_t)nbh * n_rep * kBlock < (1ull << 32);
+    const dim3 grid_hb = flat_hb ? dim3(nbh * n_rep) : dim3(nbh, n_rep);
+#define SC_ON2_HB(ND)                                                                                               \
+  if (em)                                                                                                           \
+    hipLaunchKernelGGL((k_on2_hb<ND, true>), grid_hb, dim3(kBlock), 0, st, spins, reps, g, nbh, colour, sweep, j, em, \
+                       clk);                                                                                        \
+  else                                                                                                              \
+    hipLaunchKernelGGL((k_on2_hb<ND, false>), grid_hb, dim3(kBlock), 0, st, spins, reps, g, nbh, colour, sweep, j,   \
+                       em, clk);
+#define SC_ON2_ND(ND)                                \
+  if (alg == kContinuousHeatBath) { SC_ON2_HB(ND) } \
   else { SC_ON2(ND, kOverRelaxation) }
     switch (g.nd) {
       case 1: SC_ON2_ND(1) break;
@@ -796,6 +945,7 @@ hipError_t launch_on3_sweep(const StencilGeom &g, int dim, int alg, float *spins
       default: return hipErrorInvalidValue;
     }
 #undef SC_ON2_ND
+#undef SC_ON2_HB
 #undef SC_ON2
     return hipGetLastError();
   }
