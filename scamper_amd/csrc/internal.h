@@ -182,7 +182,12 @@ hipError_t launch_on3_measure(const StencilGeom &g, int dim, const float *spins,
 hipError_t launch_on3_init(const StencilGeom &g, int dim, float *spins, uint64_t key, uint32_t rep, int random,
                            hipStream_t stream);
 
-// Multi-spin coded Ising (u64 words, bit r = replica 64b + r).
+// Multi-spin coded Ising (u64 words, bit r = replica 64b + r).  A measured
+// sweep's counters are kMsCountSlots copies per bundle of 128 u64 (per bit:
+// anti-aligned bonds, then +1 spins); block b adds into copy b % kMsCountSlots,
+// which keeps the global atomics on one address few, and the exchange kernel
+// sums and clears the copies.
+constexpr uint32_t kMsCountSlots = 16;
 hipError_t launch_ms_init(uint64_t *words, const uint64_t *keys, const StencilGeom &g, uint32_t bundle, int random,
                           hipStream_t stream);
 hipError_t launch_ms_sweep(uint64_t *words, const Threshold *thr_by_label, const int32_t *labels, const uint64_t *keys,

@@ -441,8 +441,9 @@ __global__ __launch_bounds__(kBlock) void k_ms_sweep(uint64_t *__restrict__ word
     __syncthreads();
     if (threadIdx.x < 64) {
       const uint32_t x = sred[threadIdx.x];
-      if (x & 0xFFFFu) atomicAdd(&counts[(uint64_t)bundle * 128 + threadIdx.x], (unsigned long long)(x & 0xFFFFu));
-      if (x >> 16) atomicAdd(&counts[(uint64_t)bundle * 128 + 64 + threadIdx.x], (unsigned long long)(x >> 16));
+      unsigned long long *c = counts + ((uint64_t)bundle * kMsCountSlots + blockIdx.x % kMsCountSlots) * 128;
+      if (x & 0xFFFFu) atomicAdd(&c[threadIdx.x], (unsigned long long)(x & 0xFFFFu));
+      if (x >> 16) atomicAdd(&c[64 + threadIdx.x], (unsigned long long)(x >> 16));
     }
   }
 }
@@ -490,13 +491,17 @@ __global__ __launch_bounds__(1024) void k_ms_exchange(int32_t *__restrict__ labe
   __shared__ double energy[1024];
   const uint32_t t = threadIdx.x;
   if (t < n) {
-    const unsigned long long anti = counts[(uint64_t)(t / 64) * 128 + (t % 64)];
-    const unsigned long long plus = counts[(uint64_t)(t / 64) * 128 + 64 + (t % 64)];
+    unsigned long long anti = 0, plus = 0;
+    unsigned long long *c = counts + (uint64_t)(t / 64) * kMsCountSlots * 128 + (t % 64);
+    for (uint32_t s = 0; s < kMsCountSlots; ++s) {
+      anti += c[s * 128];
+      plus += c[s * 128 + 64];
+      c[s * 128] = 0;
+      c[s * 128 + 64] = 0;
+    }
     energy[t] = -j * (n_bonds - 2.0 * (double)anti);
     if (energy_out) energy_out[t] = energy[t];
     if (mag_out) mag_out[t] = fabs(2.0 * (double)plus - n_sites) / n_sites;
-    counts[(uint64_t)(t / 64) * 128 + (t % 64)] = 0;
-    counts[(uint64_t)(t / 64) * 128 + 64 + (t % 64)] = 0;
     cfg_of[labels[t]] = (int32_t)t;
   }
   __syncthreads();

@@ -130,7 +130,7 @@ int scamper_multispin_create(const scamper_lattice *lh, double j, uint32_t n_bun
   SC_HIP(hipMalloc(&ms->d_thr, (size_t)ms->n * (ms->z + 1) * sizeof(Threshold)));
   SC_HIP(hipMalloc(&ms->d_labels, ms->n * 4));
   SC_HIP(hipMalloc(&ms->d_beta, ms->n * 8));
-  SC_HIP(hipMalloc(&ms->d_counts, (size_t)n_bundles * 128 * 8));
+  SC_HIP(hipMalloc(&ms->d_counts, (size_t)n_bundles * kMsCountSlots * 128 * 8));
   SC_HIP(hipMalloc(&ms->d_stats, 4 * 8));
   std::vector<uint64_t> keys(ms->n);
   std::vector<int32_t> labels(ms->n);
@@ -140,7 +140,7 @@ int scamper_multispin_create(const scamper_lattice *lh, double j, uint32_t n_bun
   }
   SC_HIP(hipMemcpy(ms->d_keys, keys.data(), ms->n * 8, hipMemcpyHostToDevice));
   SC_HIP(hipMemcpy(ms->d_labels, labels.data(), ms->n * 4, hipMemcpyHostToDevice));
-  SC_HIP(hipMemset(ms->d_counts, 0, (size_t)n_bundles * 128 * 8));
+  SC_HIP(hipMemset(ms->d_counts, 0, (size_t)n_bundles * kMsCountSlots * 128 * 8));
   SC_HIP(hipMemset(ms->d_stats, 0, 4 * 8));
   for (uint32_t b = 0; b < n_bundles; ++b) {
     hipError_t e = launch_ms_init(ms->d_words, ms->d_keys, ms->geom, b, 0, ms->stream);
