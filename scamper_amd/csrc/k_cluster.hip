@@ -577,6 +577,228 @@ __global__ void k_transform_on(OnPlanes P, uint32_t *parent, uint32_t lx, uint32
   }
 }
 
+// ------------------------------------------------ O(N) clusters, tile by tile
+// Stencils whose sides are multiples of the tile: the embedding bonds inside
+// a tile come from the sites' projections on r kept in LDS, x bonds as runs
+// and y (and z) bonds through an LDS union-find, and every site's tile-local
+// root is written; k_bonds_on_faces unites the bonds across tile faces in
+// global memory; k_transform_on_tile resolves each local root once.  Draws,
+// products, probabilities and reflections are the global kernels', in the
+// same operation order, so the partition and the spins are theirs.
+
+// Bond activity (cluster_bond_probability, models.rs:537-558) for the
+// product (s_i.r)(s_j.r) of a bond's lower and upper site.
+__device__ __forceinline__ bool on_bond_active(double product, double beta, double coupling, uint32_t word) {
+  if (coupling == 0.0 || product <= 0.0) return false;
+  double p = 1.0 - exp(-2.0 * beta * coupling * product);
+  p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
+  return (double)word * 0x1p-32 < p;
+}
+
+// The quad of sites s0 .. s0 + 3 (4 | s0): s0 and s0 + 2 are ranks s0/2 and
+// s0/2 + 1 of colour c, s0 + 1 and s0 + 3 the same ranks of the other.
+template <int D>
+__device__ __forceinline__ void load_quad_on(const OnPlanes &P, uint64_t s0, uint32_t c, double (&v)[4][D]) {
+  const uint64_t j = s0 >> 1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const float2 a = *reinterpret_cast<const float2 *>(P.v + (D * (uint64_t)c + k) * P.nh + j);
+    const float2 b = *reinterpret_cast<const float2 *>(P.v + (D * (uint64_t)(c ^ 1u) + k) * P.nh + j);
+    v[0][k] = (double)a.x;
+    v[2][k] = (double)a.y;
+    v[1][k] = (double)b.x;
+    v[3][k] = (double)b.y;
+  }
+}
+
+template <int D, int ND>
+__global__ __launch_bounds__(kThreads) void k_bonds_on_tile(OnPlanes P, uint32_t *__restrict__ parent, uint32_t lx,
+                                                            uint32_t ly, uint64_t key, uint64_t step, double beta,
+                                                            double coupling, const double *__restrict__ r_) {
+  using T = Tile<ND>;
+  static_assert(T::N == 4 * kThreads, "one quad per thread");
+  constexpr uint32_t QR = T::X / 4;  // quads per tile row
+  __shared__ uint32_t lp[T::N];
+  __shared__ double proj[T::N];    // s.r per site
+  __shared__ uint8_t xb[kThreads];  // the quads' x-bond nibbles
+  double r[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = r_[c];
+  const uint32_t tiles_x = lx / T::X, tiles_y = ly / T::Y;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t ox = (tile % tiles_x) * T::X, oy = ((tile / tiles_x) % tiles_y) * T::Y,
+                 oz = (tile / (tiles_x * tiles_y)) * T::Z;
+  const uint32_t q = threadIdx.x, i0 = 4 * q;
+  const uint32_t x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
+  const uint32_t s0 = ox + x0 + lx * (oy + y + ly * (oz + z));
+  const uint32_t c = (ox + x0 + oy + y + oz + z) & 1u;
+  double v[4][D];
+  load_quad_on<D>(P, s0, c, v);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) proj[i0 + u] = dotn<D>(v[u], r);
+  uint32_t w[4 * ND];  // bond words of edges nd s + a for the quad's sites
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const U4 blk = cluster_block(key, (uint64_t)ND * s0 / 4 + k, step, kStreamClusterBond);
+    w[4 * k] = blk.x, w[4 * k + 1] = blk.y, w[4 * k + 2] = blk.z, w[4 * k + 3] = blk.w;
+  }
+  __syncthreads();
+  auto bond = [&](uint32_t u, int a, uint32_t i, uint32_t t) {
+    return on_bond_active(proj[i] * proj[t], beta, coupling, w[u * ND + a]);
+  };
+  uint32_t xm = 0;  // bit u: the bond from site u to u + 1 is active and inside the tile
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u)
+    if (x0 + u + 1 < T::X && bond(u, 0, i0 + u, i0 + u + 1)) xm |= 1u << u;
+  xb[q] = (uint8_t)xm;
+  __syncthreads();
+  const uint32_t r0 = q - q % QR;
+  uint32_t row = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < QR; ++k) row |= (uint32_t)xb[r0 + k] << (4 * k);
+  const uint32_t starts = ~(row << 1);  // bit x: no active bond from x - 1 to x
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t x = x0 + u;
+    const uint32_t first = 31u - (uint32_t)__builtin_clz(starts & (0xFFFFFFFFu >> (31u - x)));
+    lp[i0 + u] = i0 - x0 + first;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t i = i0 + u;
+    if (y + 1 < T::Y && bond(u, 1, i, i + T::X)) unite_lds(lp, i, i + T::X);
+    if (ND == 3 && z + 1 < T::Z && bond(u, ND - 1, i, i + T::X * T::Y)) unite_lds(lp, i, i + T::X * T::Y);
+  }
+  __syncthreads();
+  uint32_t rt[4];
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t l = find_lds(lp, i0 + u);
+    rt[u] = ox + l % T::X + lx * (oy + (l / T::X) % T::Y + ly * (oz + l / (T::X * T::Y)));
+  }
+  *reinterpret_cast<uint4 *>(parent + s0) = make_uint4(rt[0], rt[1], rt[2], rt[3]);
+}
+
+// The embedding bonds from each tile's last layer along every axis to the
+// next tile (periodic), one thread each.
+template <int D, int ND>
+__global__ void k_bonds_on_faces(OnPlanes P, uint32_t *parent, uint32_t lx, uint32_t ly, uint32_t lz, uint64_t n_tiles,
+                                 uint64_t key, uint64_t step, double beta, double coupling,
+                                 const double *__restrict__ r_) {
+  using T = Tile<ND>;
+  constexpr uint32_t fx = T::Y * T::Z, fy = T::X * T::Z, fz = ND == 3 ? T::X * T::Y : 0u;
+  constexpr uint32_t per_tile = fx + fy + fz;
+  double r[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = r_[c];
+  const uint32_t tiles_x = lx / T::X, tiles_y = ly / T::Y;
+  for (uint64_t k = thread_id(); k < n_tiles * per_tile; k += grid_threads()) {
+    const uint32_t tile = (uint32_t)(k / per_tile), f = (uint32_t)(k - (uint64_t)tile * per_tile);
+    const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
+    uint32_t x, y, z, a;
+    if (f < fx) {  // +x bond of the tile's last column
+      a = 0, x = tx * T::X + T::X - 1, y = ty * T::Y + f % T::Y, z = tz * T::Z + f / T::Y;
+    } else if (f < fx + fy) {  // +y bond of its last row
+      const uint32_t g = f - fx;
+      a = 1, x = tx * T::X + g % T::X, y = ty * T::Y + T::Y - 1, z = tz * T::Z + g / T::X;
+    } else {  // +z bond of its last plane
+      const uint32_t g = f - fx - fy;
+      a = 2, x = tx * T::X + g % T::X, y = ty * T::Y + g / T::X, z = tz * T::Z + T::Z - 1;
+    }
+    const uint32_t x2 = a == 0 ? (x + 1 == lx ? 0u : x + 1) : x;
+    const uint32_t y2 = a == 1 ? (y + 1 == ly ? 0u : y + 1) : y;
+    const uint32_t z2 = a == 2 ? (z + 1 == lz ? 0u : z + 1) : z;
+    const uint64_t s = (uint64_t)x + (uint64_t)lx * ((uint64_t)y + (uint64_t)ly * z);
+    const uint64_t t = (uint64_t)x2 + (uint64_t)lx * ((uint64_t)y2 + (uint64_t)ly * z2);
+    double vs[D], vt[D];
+    load_on<D>(P, s, StencilSite{x, y, z}, vs);
+    load_on<D>(P, t, StencilSite{x2, y2, z2}, vt);
+    const double product = dotn<D>(vs, r) * dotn<D>(vt, r);
+    // start from the tile-local roots, so that the entry of a site that is
+    // not a local root keeps pointing at its own tile's local root
+    if (on_bond_active(product, beta, coupling, cluster_word(key, (uint64_t)ND * s + a, step, kStreamClusterBond)))
+      unite(parent, load_parent(parent + s), load_parent(parent + t));
+  }
+}
+
+// Reflection of the flipped clusters (transform_cluster_spin,
+// models.rs:560-575), each tile-local root resolved once per tile.
+template <int D, int ND>
+__global__ __launch_bounds__(kThreads) void k_transform_on_tile(OnPlanes P, uint32_t *parent, uint32_t lx, uint32_t ly,
+                                                                uint64_t div_lx2, uint64_t div_ly, int wolff,
+                                                                uint64_t key, uint64_t step,
+                                                                const uint32_t *__restrict__ seed,
+                                                                const double *__restrict__ r_) {
+  using T = Tile<ND>;
+  __shared__ uint8_t res[T::N];
+  __shared__ uint8_t need[T::N];
+  const uint32_t seed_root = wolff ? seed[0] : 0u;
+  const uint32_t tiles_x = lx / T::X, tiles_y = ly / T::Y;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t ox = (tile % tiles_x) * T::X, oy = ((tile / tiles_x) % tiles_y) * T::Y,
+                 oz = (tile / (tiles_x * tiles_y)) * T::Z;
+  auto global_of = [&](uint32_t i) -> uint32_t {
+    const uint32_t x = i % T::X, y = (i / T::X) % T::Y, z = i / (T::X * T::Y);
+    return ox + x + lx * (oy + y + ly * (oz + z));
+  };
+  // tile-local index of global site g, or T::N when g lies in another tile
+  auto local_of = [&](uint32_t g) -> uint32_t {
+    const uint32_t row = magic_div(g >> 1, div_lx2), x = g - row * lx;
+    const uint32_t z = magic_div(row, div_ly), y = row - z * ly;
+    const uint32_t dx = x - ox, dy = y - oy, dz = z - oz;
+    return dx < T::X && dy < T::Y && dz < T::Z ? dx + T::X * (dy + T::Y * dz) : T::N;
+  };
+  // Swendsen-Wang reflects a cluster when its root's target word is below
+  // 2^31 (sw_cluster_auxiliary's coin), Wolff the seed's cluster
+  auto flips = [&](uint32_t root) -> uint32_t {
+    if (wolff) return root == seed_root ? 1u : 0u;
+    return cluster_word(key, root, step, kStreamClusterTarget) < 0x80000000u ? 1u : 0u;
+  };
+  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads) need[i] = 0;
+  __syncthreads();
+  const uint32_t i0 = 4 * threadIdx.x;
+  const uint32_t s0 = global_of(i0);
+  const uint4 p4 = *reinterpret_cast<const uint4 *>(parent + s0);
+  const uint32_t p[4] = {p4.x, p4.y, p4.z, p4.w};
+  uint32_t loc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    loc[k] = local_of(p[k]);
+    if (loc[k] < T::N) need[loc[k]] = 1;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < T::N; i += kThreads)
+    if (need[i]) res[i] = (uint8_t)flips(find_root(parent, global_of(i)));
+  __syncthreads();
+  uint32_t fm = 0;  // bit k: site s0 + k is reflected
+#pragma unroll
+  for (int k = 0; k < 4; ++k) fm |= (loc[k] < T::N ? (uint32_t)res[loc[k]] : flips(find_root(parent, p[k]))) << k;
+  if (!fm) return;
+  double r[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) r[c] = r_[c];
+  const uint32_t x0 = i0 % T::X, y = (i0 / T::X) % T::Y, z = i0 / (T::X * T::Y);
+  const uint32_t c = (ox + x0 + oy + y + oz + z) & 1u;
+  double v[4][D];
+  load_quad_on<D>(P, s0, c, v);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!((fm >> k) & 1u)) continue;
+    const double pr = dotn<D>(v[k], r);
+    double u[D];
+#pragma unroll
+    for (int m = 0; m < D; ++m) u[m] = v[k][m] - 2.0 * pr * r[m];
+    const double norm = sqrt(dotn<D>(u, u));
+    if (norm > 1e-15)
+#pragma unroll
+      for (int m = 0; m < D; ++m) u[m] /= norm;
+    const uint64_t col = (k & 1) ? (c ^ 1u) : c, j = (s0 >> 1) + (uint64_t)(k >> 1);
+#pragma unroll
+    for (int m = 0; m < D; ++m) P.v[(D * col + m) * P.nh + j] = (float)u[m];
+  }
+}
+
 uint32_t blocks_for(uint64_t n) {
   const uint64_t b = (n + kThreads - 1) / kThreads;
   return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
@@ -637,6 +859,15 @@ __global__ void k_transform_on_csr(float *__restrict__ v, uint64_t n, uint32_t *
   }
 }
 
+// Stencils labelled tile by tile: 2-D or 3-D with every side a multiple of
+// the tile's.
+bool tiled(const StencilGeom &g) {
+  const uint32_t lx = 2 * g.lx2;
+  if (g.nd == 2) return lx % Tile<2>::X == 0 && g.ly % Tile<2>::Y == 0;
+  if (g.nd == 3) return lx % Tile<3>::X == 0 && g.ly % Tile<3>::Y == 0 && g.lz % Tile<3>::Z == 0;
+  return false;
+}
+
 template <int D>
 void launch_on_update(const StencilGeom &g, float *planes, uint32_t *parent, uint64_t key, uint64_t step,
                       double beta, double j, int wolff, uint32_t *seed, double *r, hipStream_t stream) {
@@ -644,6 +875,24 @@ void launch_on_update(const StencilGeom &g, float *planes, uint32_t *parent, uin
   const uint32_t lx = 2 * g.lx2;
   const OnPlanes P{planes, g.nh};
   hipLaunchKernelGGL(k_on_direction<D>, dim3(1), dim3(64), 0, stream, key, step, r);
+  if ((g.nd == 2 || g.nd == 3) && tiled(g)) {
+#define SC_ON_TILED(ND)                                                                                              \
+  {                                                                                                                  \
+    using T = Tile<ND>;                                                                                              \
+    constexpr uint64_t per_tile = T::Y * T::Z + T::X * T::Z + (ND == 3 ? T::X * T::Y : 0);                           \
+    const uint64_t tiles = n / T::N;                                                                                 \
+    hipLaunchKernelGGL((k_bonds_on_tile<D, ND>), dim3((uint32_t)tiles), dim3(kThreads), 0, stream, P, parent, lx,    \
+                       g.ly, key, step, beta, j, r);                                                                 \
+    hipLaunchKernelGGL((k_bonds_on_faces<D, ND>), dim3(blocks_for(tiles * per_tile)), dim3(kThreads), 0, stream, P,  \
+                       parent, lx, g.ly, g.lz, tiles, key, step, beta, j, r);                                       \
+    if (wolff) hipLaunchKernelGGL(k_on_seed, dim3(1), dim3(64), 0, stream, parent, n, key, step, seed);             \
+    hipLaunchKernelGGL((k_transform_on_tile<D, ND>), dim3((uint32_t)tiles), dim3(kThreads), 0, stream, P, parent,   \
+                       lx, g.ly, g.div_lx2, g.div_ly, wolff, key, step, seed, r);                                   \
+  }
+    if (g.nd == 2) SC_ON_TILED(2) else SC_ON_TILED(3)
+#undef SC_ON_TILED
+    return;
+  }
   hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n)), dim3(kThreads), 0, stream, parent, n);
   if (g.nd == 1)
     hipLaunchKernelGGL((k_bonds_on<D, 1>), dim3(blocks_for(n)), dim3(kThreads), 0, stream, P, parent, lx, g.ly, g.lz,
@@ -659,14 +908,6 @@ void launch_on_update(const StencilGeom &g, float *planes, uint32_t *parent, uin
                      key, step, seed, r);
 }
 
-// Stencils labelled tile by tile: 2-D or 3-D with every side a multiple of
-// the tile's.
-bool tiled(const StencilGeom &g) {
-  const uint32_t lx = 2 * g.lx2;
-  if (g.nd == 2) return lx % Tile<2>::X == 0 && g.ly % Tile<2>::Y == 0;
-  if (g.nd == 3) return lx % Tile<3>::X == 0 && g.ly % Tile<3>::Y == 0 && g.lz % Tile<3>::Z == 0;
-  return false;
-}
 
 }  // namespace
 

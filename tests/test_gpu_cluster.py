@@ -140,7 +140,9 @@ def test_energy_matches_kaufman_at_criticality(sc, alg):
     assert all(abs(z) < 4.5 for z in zs), zs
 
 
-@pytest.mark.parametrize("dims", [(8, 6, 4), (16, 16), (14,)])
+# (64, 32) and (32, 16, 16) are labelled tile by tile (2-D tiles of 32 x 32,
+# 3-D of 16 x 8 x 8 sites), the others by the global union-find
+@pytest.mark.parametrize("dims", [(8, 6, 4), (16, 16), (14,), (64, 32), (32, 16, 16)])
 @pytest.mark.parametrize("wolff", [False, True])
 @pytest.mark.parametrize("n_comp", [3, 2])
 def test_on3_bit_exact_against_oracle(sc, orc, dims, wolff, n_comp):
