@@ -588,6 +588,22 @@ __device__ __forceinline__ void on2_field(const float *__restrict__ oth, const S
   *centre = ce;
 }
 
+// The sampler's second test, ln(c / u) + 1 >= c, decided on the hardware's
+// FP32 log2 where its result is certain and in f64 elsewhere.  With c and u
+// rounded to FP32 (both normal; relative error 2^-24 each) and v_log_f32
+// within 2 ulp of log2, ln c - ln u is within 2.4e-7 (|ln c| + |ln u|) +
+// 3e-7 of ln(c / u); a margin above 1e-6 (1 + |ln c| + |ln u|) gives the f64
+// answer, and below it the f64 path decides.
+__device__ __forceinline__ bool log_test(double c, double u) {
+  const float cf = (float)c, uf = (float)u;
+  if (cf >= 1.17549435e-38f && uf >= 1.17549435e-38f && cf <= 3.0e38f) {
+    const float lc = __builtin_amdgcn_logf(cf) * 0.69314718f, lu = __builtin_amdgcn_logf(uf) * 0.69314718f;
+    const double d = (double)lc - (double)lu + 1.0 - c;
+    if (fabs(d) > 1e-6 * (1.0 + (double)fabsf(lc) + (double)fabsf(lu))) return d > 0.0;
+  }
+  return log(c / u) + 1.0 >= c;
+}
+
 template <int ND, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_on2_hb(float *__restrict__ spins, const ReplicaParams *__restrict__ reps,
                                                    StencilGeom g, uint32_t nbx, uint32_t colour, uint64_t sweep,
@@ -650,7 +666,7 @@ __global__ __launch_bounds__(kBlock) void k_on2_hb(float *__restrict__ spins, co
       const double value = (1.0 + r * zc) / (r + zc);
       const double c = kappa * (r - value);
       const double u = fmax((double)w.y * kTwoM32, 2.2250738585072014e-308);
-      if (c * (2.0 - c) > u || log(c / u) + 1.0 >= c || k == 63) {
+      if (c * (2.0 - c) > u || log_test(c, u) || k == 63) {
         // theta = a + atan2(hy, hx) as the field's direction rotated by a
         const double ca = fmin(fmax(value, -1.0), 1.0);
         const double sn = sqrt((1.0 - ca) * (1.0 + ca));
