@@ -434,6 +434,22 @@ __device__ __forceinline__ U4 vm_block(const ReplicaParams &P, uint64_t j, uint6
                        (uint32_t)(j >> 32) | (k << 20), (uint32_t)P.key, (uint32_t)(P.key >> 32));
 }
 
+// The sampler's second test, ln(c / u) + 1 >= c, decided on the hardware's
+// FP32 log2 where its result is certain and in f64 elsewhere.  With c and u
+// rounded to FP32 (both normal; relative error 2^-24 each) and v_log_f32
+// within 2 ulp of log2, ln c - ln u is within 2.4e-7 (|ln c| + |ln u|) +
+// 3e-7 of ln(c / u); a margin above 1e-6 (1 + |ln c| + |ln u|) gives the f64
+// answer, and below it the f64 path decides.
+__device__ __forceinline__ bool log_test(double c, double u) {
+  const float cf = (float)c, uf = (float)u;
+  if (cf >= 1.17549435e-38f && uf >= 1.17549435e-38f && cf <= 3.0e38f) {
+    const float lc = __builtin_amdgcn_logf(cf) * 0.69314718f, lu = __builtin_amdgcn_logf(uf) * 0.69314718f;
+    const double d = (double)lc - (double)lu + 1.0 - c;
+    if (fabs(d) > 1e-6 * (1.0 + (double)fabsf(lc) + (double)fabsf(lu))) return d > 0.0;
+  }
+  return log(c / u) + 1.0 >= c;
+}
+
 // The von Mises draw as (cos a, sin a): the accepted try's value is cos a
 // itself, and sin a = +-sqrt((1 - v)(1 + v)) with the sign word's sign, so
 // the angle a = +-acos(v) is never formed; kappa < 1e-8 gives a = 2 pi u.
@@ -455,7 +471,7 @@ __device__ void von_mises_cs(const ReplicaParams &P, uint64_t j, uint64_t sweep,
     const double c = kappa * (r - value);
     const double u = fmax((double)w.y * kTwoM32, 2.2250738585072014e-308);
     sign = w.z;
-    if (c * (2.0 - c) > u || log(c / u) + 1.0 >= c) break;
+    if (c * (2.0 - c) > u || log_test(c, u)) break;
   }
   const double v = fmin(fmax(value, -1.0), 1.0);
   const double sn = sqrt((1.0 - v) * (1.0 + v));
@@ -586,22 +602,6 @@ __device__ __forceinline__ void on2_field(const float *__restrict__ oth, const S
   *hx = x;
   *hy = y;
   *centre = ce;
-}
-
-// The sampler's second test, ln(c / u) + 1 >= c, decided on the hardware's
-// FP32 log2 where its result is certain and in f64 elsewhere.  With c and u
-// rounded to FP32 (both normal; relative error 2^-24 each) and v_log_f32
-// within 2 ulp of log2, ln c - ln u is within 2.4e-7 (|ln c| + |ln u|) +
-// 3e-7 of ln(c / u); a margin above 1e-6 (1 + |ln c| + |ln u|) gives the f64
-// answer, and below it the f64 path decides.
-__device__ __forceinline__ bool log_test(double c, double u) {
-  const float cf = (float)c, uf = (float)u;
-  if (cf >= 1.17549435e-38f && uf >= 1.17549435e-38f && cf <= 3.0e38f) {
-    const float lc = __builtin_amdgcn_logf(cf) * 0.69314718f, lu = __builtin_amdgcn_logf(uf) * 0.69314718f;
-    const double d = (double)lc - (double)lu + 1.0 - c;
-    if (fabs(d) > 1e-6 * (1.0 + (double)fabsf(lc) + (double)fabsf(lu))) return d > 0.0;
-  }
-  return log(c / u) + 1.0 >= c;
 }
 
 template <int ND, bool MEAS>
