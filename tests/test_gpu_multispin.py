@@ -104,3 +104,33 @@ def test_captured_steps_equal_single_steps(sc):
     eb, mb = b.series()
     assert ea.shape == eb.shape and np.array_equal(ea, eb) and np.array_equal(ma, mb)
     assert a.stats() == b.stats()
+
+
+def test_bench_shape_equals_single_replica_runs(sc):
+    """The tempering workload's own shape: 3-D L=128, 64 temperatures
+    4.30 .. 4.72 (geometric) in one bundle.  A row holds 64 ranks, so the
+    sweeps run the one-row kernels on their full grid (4,096 waves per
+    launch, the tie queue and the counter copies at scale).  Two measured
+    steps without exchange: every bit equals the single-replica kernels'
+    run of that replica, and each step's energy and |m| of every replica
+    equal that run's measured series."""
+    L, seed, n = 128, 42, 2
+    print(f"case multispin {L}^3 x 64 bits", flush=True)
+    lat = sc.Lattice.hypercubic([L, L, L])
+    betas = 1.0 / (4.30 * (4.72 / 4.30) ** (np.arange(64) / 63.0))
+    ms = sc.MultiSpin(lat, 1.0, 1, seed)
+    ms.set_betas(betas)
+    ms.init(True)
+    ms.step(n, exchange=False, measure=True)
+    e_ms, m_ms = ms.series()
+    st = sc.State(lat, sc.model(sc.ISING), 64, seed)
+    for r in range(64):
+        st.init(True, r)
+        st.set_beta(float(betas[r]), r)
+    st.sweep(sc.METROPOLIS, n, measure=True)
+    w = ms.download()
+    for r in range(64):
+        assert np.array_equal(_bits(w, r), st.download_i8(r)), f"replica {r}"
+        e, m = st.series(r)
+        assert np.array_equal(e_ms[:, r], e) and np.array_equal(m_ms[:, r], m), f"series of replica {r}"
+    st.close()

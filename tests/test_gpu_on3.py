@@ -229,6 +229,53 @@ def test_xy_rejects_other_components(sc):
             sc.State(lat, sc.model(sc.ON, 1.0, 2, n_comp), 1, 1)
 
 
+def test_xy_bench_shape_against_fp64_oracle(sc, orc):
+    """The XY workload at its own shape: 96^3 periodic, 32 replicas at
+    T = 2.15 + 0.1 r / 31.  The heat bath runs the four-sites-per-lane
+    kernel on its full grid; three replicas (first, middle, last) are
+    replayed from the device's spins by the FP64 oracle (sample_von_mises,
+    models.rs:666-689, then microcanonical.rs:42-80) for one hybrid sweep.
+    The sampler's accept/reject follows the f64 rule, so the spins differ
+    only by FP32 rounding: of the stored components and of the
+    over-relaxation, which reaches a spin as ~1e-7 / |h|, and a neighbour's
+    error enters the field a site is drawn about.  Chains of small fields
+    give a tail far wider than the bulk (the largest error, 1.8e-4, sits at
+    a site with |h| ~ 1 next to sites of small fields), so the bars are the
+    mean (1e-6), the 99.99% quantile (5e-5) and the maximum (1e-3, far below
+    the O(1) error of a decision that went the other way), and the
+    replica's energy and |M| (1e-6)."""
+    L, R, seed = 96, 32, 42
+    print(f"case xy {L}^3 x {R} replicas", flush=True)
+    dims = (L, L, L)
+    lat = sc.Lattice.hypercubic(list(dims))
+    st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 2), R, seed)
+    betas = [1.0 / (2.15 + 0.1 * r / 31) for r in range(R)]
+    for r in range(R):
+        st.init(True, r)
+        st.set_beta(betas[r], r)
+    st.sweep(sc.HYBRID_HB_OR, 2)
+    replicas = (0, 13, 31)
+    starts = {r: st.download_f64(r) for r in replicas}
+    sweep = st.sweeps
+    st.sweep(sc.HYBRID_HB_OR, 1, measure=True)
+    olat = orc.Lattice.hypercubic(list(dims))
+    m = orc.model(orc.ON, 1.0, 2, 2)
+    for r in replicas:
+        chain = orc.Chain(olat, m, starts[r].copy())
+        chain.on2_heat_bath(betas[r], orc.replica_key(seed, r), sweep)
+        chain.over_relaxation()
+        got = st.download_f64(r)
+        err = np.abs(got - chain.spins).reshape(-1, 2).max(axis=1)
+        e, mag = st.series(r)
+        e_ref, m_ref = chain.total_energy(), chain.magnetization()
+        rel_e = abs(e[-1] - e_ref) / abs(e_ref)
+        print(f"replica {r}: per-site max {err.max():.2e}, mean {err.mean():.2e}, "
+              f"99.99% {np.quantile(err, 0.9999):.2e}; E rel {rel_e:.2e}, |M| abs {abs(mag[-1] - m_ref):.2e}")
+        assert err.max() < 1e-3 and np.quantile(err, 0.9999) < 5e-5 and err.mean() < 1e-6
+        assert rel_e < 1e-6 and abs(mag[-1] - m_ref) < 1e-6
+    st.close()
+
+
 def _fields(spins, L):
     """Local field h = sum of the six neighbours of every site of a periodic
     L^3 lattice, from site-major f64 spins (x fastest)."""
