@@ -317,6 +317,11 @@ typedef struct {
                                  rate of three real chunks of one set of resident jobs, once
                                  the set holds 1 Mi colour-class sites (8 Mi for the grid) */
   uint32_t use_graphs;        /* 1 = hipGraph per chunk, 0 = plain launches */
+  uint32_t int8_only;         /* 0 = Ising and Potts q <= 4 jobs run on bit planes (packed from
+                                 and back to their int8 spins at every chunk; needs another half
+                                 of the spin budget), 1 = every job on the int8 kernel; both give
+                                 the same results */
+  uint32_t reserved;          /* 0 */
 } scamper_pool_config;
 
 typedef struct {

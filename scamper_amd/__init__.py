@@ -47,7 +47,8 @@ class Model(C.Structure):
 
 class PoolConfig(C.Structure):
     _fields_ = [("spin_budget_bytes", C.c_uint64), ("max_jobs", C.c_uint32),
-                ("chunk_sweeps", C.c_uint32), ("grid", C.c_uint32), ("use_graphs", C.c_uint32)]
+                ("chunk_sweeps", C.c_uint32), ("grid", C.c_uint32), ("use_graphs", C.c_uint32),
+                ("int8_only", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class JobSpec(C.Structure):
@@ -540,8 +541,8 @@ class Pool:
     (Scheduler::run_parallel, Carlo.rs/src/scheduler.rs:345-405)."""
 
     def __init__(self, spin_budget_bytes: int = 1 << 30, max_jobs: int = 256, chunk_sweeps: int = 32,
-                 grid: int = 0, use_graphs: bool = True):
-        cfg = PoolConfig(spin_budget_bytes, max_jobs, chunk_sweeps, grid, int(use_graphs))
+                 grid: int = 0, use_graphs: bool = True, bit_planes: bool = True):
+        cfg = PoolConfig(spin_budget_bytes, max_jobs, chunk_sweeps, grid, int(use_graphs), int(not bit_planes), 0)
         h = C.c_void_p()
         check(lib().scamper_pool_create(C.byref(cfg), C.byref(h)))
         self.h = h.value

@@ -235,6 +235,13 @@ struct alignas(16) TileJob {
   // word): top and second byte of K, both 0xff where K = 2^32 (a draw whose
   // top 16 bits are all ones then ties, and the tie is settled exactly)
   uint32_t lut_k8[4], lut_m8[4];
+  // Bit-plane form (Ising, Potts q <= 4; null bits: int8 only): four planes
+  // (colour, bit) of nwp words each, a spare word before and after
+  // ceil(nh / 32) words of 32 ranks; the job's first tile of 256 words in
+  // the bit kernel's list; the top byte of K over t = index + 7 - z.
+  uint32_t *bits;
+  uint32_t nwp, bit_tile_start;
+  uint32_t lut_t8[4];
 };
 
 // Fill a row's byte look-up tables from its thresholds.
@@ -250,9 +257,10 @@ inline void fill_tile_luts(TileJob &t) {
 
 struct alignas(16) TileHeader {
   uint32_t n_jobs;
-  uint32_t n_tiles;
-  uint32_t pad[2];
+  uint32_t n_tiles;  // tiles of the int8 kernel
+  uint32_t pad[2];   // pad[0]: tiles of 256 words of the bit-plane kernel
 };
+
 
 // Where a launch adds its measured counters: for job row r, step k and
 // block b the record (em[0], em[1], em[2 .. 2+q) as for the stencil) at
@@ -264,6 +272,16 @@ struct EmLayout {
   uint32_t sub_mask;
   uint32_t sub_stride;
 };
+
+// Bit-plane jobs of a table: the tile kernel on their planes (the same
+// draws, decisions and measured records as launch_tile_sweep), and the
+// conversion of every bit-plane job's int8 colour arrays to planes (unpack 0)
+// or back (unpack 1).  The planes of a job must hold 16 (ceil(nh / 32) + 2)
+// bytes at `bits`.
+void fill_bit_lut(TileJob &t);
+hipError_t launch_tile_bits(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint64_t step, uint32_t colour,
+                            int measure, EmLayout em, hipStream_t stream);
+hipError_t launch_bits_pack(const TileHeader *hdr, const TileJob *jobs, int unpack, hipStream_t stream);
 
 hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint32_t lane_sites,
                              uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream);

@@ -148,6 +148,9 @@ struct scamper_pool {
   hipStream_t cap_stream = nullptr;  // graph capture only; graphs run on `stream`
   int8_t *spins = nullptr;        // spins_alloc + kSpinSlack; jobs at spins + offset
   int8_t *spins_alloc = nullptr;
+  // bit planes of the jobs that run on them: a job's at byte offset off / 2
+  // (its int8 region's half, which holds its four planes)
+  uint32_t *bits = nullptr;
   Arena arena;
   uint64_t in_use = 0;
   size_t table_bytes = 0;
@@ -198,6 +201,7 @@ struct scamper_pool {
       if (ev_end[p]) hipEventDestroy(ev_end[p]);
     }
     hipFree(spins_alloc);
+    hipFree(bits);
     if (cap_stream) hipStreamDestroy(cap_stream);
     if (stream) hipStreamDestroy(stream);
   }
@@ -209,32 +213,40 @@ namespace {
 // rows of this length, so the layout follows each chunk's own K).
 uint64_t ring_row_counters(uint32_t K) { return (uint64_t)K * kSubs * kRec; }
 
-// The chunk's launches: zero the ring, K steps of (colour 0, colour 1), fold.
-int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, uint32_t K, hipStream_t st) {
+// The chunk's launches: zero the ring, pack the bit-plane jobs, K steps of
+// (colour 0, colour 1) on the int8 and the bit-plane kernels, unpack, fold.
+int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, uint32_t K, bool int8_jobs,
+                       bool bit_jobs, hipStream_t st) {
   const TileHeader *hdr = reinterpret_cast<const TileHeader *>(P->d_table[p]);
   const TileJob *rows = reinterpret_cast<const TileJob *>(hdr + 1);
   const EmLayout em{P->d_ring[p], ring_row_counters(K), (uint64_t)kSubs * kRec, kSubs - 1, kRec};
   hipError_t e = launch_tile_zero(hdr, P->d_ring[p], ring_row_counters(K), st);
+  if (e == hipSuccess && bit_jobs) e = launch_bits_pack(hdr, rows, 0, st);
   for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
-    e = launch_tile_sweep(hdr, rows, grid, lane, k, 0, 0, em, st);
-    if (e == hipSuccess) e = launch_tile_sweep(hdr, rows, grid, lane, k, 1, 1, em, st);
+    for (uint32_t c = 0; c < 2 && e == hipSuccess; ++c) {
+      if (int8_jobs) e = launch_tile_sweep(hdr, rows, grid, lane, k, c, (int)c, em, st);
+      if (e == hipSuccess && bit_jobs) e = launch_tile_bits(hdr, rows, grid, k, c, (int)c, em, st);
+    }
   }
+  if (e == hipSuccess && bit_jobs) e = launch_bits_pack(hdr, rows, 1, st);
   if (e == hipSuccess) e = launch_tile_fold(hdr, P->d_ring[p], P->d_compact[p], K, st);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "chunk launch: %s", hipGetErrorString(e));
   return SCAMPER_OK;
 }
 
-// The captured chunk for launch shape (grid, lane, K) on table buffer p,
-// captured on first use.
-int chunk_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane, uint32_t K, hipGraphExec_t *exec) {
-  const uint64_t key = ((uint64_t)grid << 32) | ((uint64_t)K << 8) | lane;
+// The captured chunk for launch shape (grid, lane, K) and the kernels the
+// table needs, on table buffer p, captured on first use.
+int chunk_graph(scamper_pool *P, int p, uint32_t grid, uint32_t lane, uint32_t K, bool int8_jobs, bool bit_jobs,
+                hipGraphExec_t *exec) {
+  const uint64_t key = ((uint64_t)grid << 32) | ((uint64_t)K << 8) | lane | (int8_jobs ? 1ull << 5 : 0) |
+                       (bit_jobs ? 1ull << 6 : 0);
   auto it = P->graphs[p].find(key);
   if (it != P->graphs[p].end()) {
     *exec = it->second.second;
     return SCAMPER_OK;
   }
   SC_HIP(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_chunk_body(P, p, grid, lane, K, P->cap_stream);
+  const int rc = enqueue_chunk_body(P, p, grid, lane, K, int8_jobs, bit_jobs, P->cap_stream);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(P->cap_stream, &g);
   if (rc) {
@@ -497,7 +509,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   InFlight &F = P->inflight[p];
   F.rows.clear();
   F.updates = 0;
-  uint32_t tiles = 0;
+  uint32_t tiles = 0, bit_tiles = 0;
   const uint32_t lane = lane_for(shape, total_sites);
   for (uint64_t id : P->running) {
     Job &J = *P->jobs.at(id);
@@ -520,11 +532,20 @@ int step(scamper_pool *P, uint32_t *n_completed) {
     t.kind = (uint32_t)J.spec.model.kind;
     t.q = (uint32_t)J.spec.model.q;
     t.tile_start = tiles;
-    t.n_tiles = (uint32_t)((J.geom.nh + tile_sites(lane) - 1) / tile_sites(lane));
+    t.bit_tile_start = bit_tiles;
     t.em_row = (uint32_t)F.rows.size();
     std::memcpy(t.thr, J.thr, sizeof t.thr);
     fill_tile_luts(t);
-    tiles += t.n_tiles;
+    if (P->bits && (J.spec.model.kind == kIsing || J.spec.model.q <= 4)) {
+      const uint32_t words = (uint32_t)((J.geom.nh + 31) / 32);
+      t.bits = P->bits + J.off / 8;  // byte offset off / 2
+      t.nwp = words + 2;
+      fill_bit_lut(t);
+      bit_tiles += (words + 255) / 256;
+    } else {
+      t.n_tiles = (uint32_t)((J.geom.nh + tile_sites(lane) - 1) / tile_sites(lane));
+      tiles += t.n_tiles;
+    }
     F.rows.push_back(RowInfo{id, J.enqueued, steps});
     F.updates += J.n_sites * steps;
   }
@@ -536,22 +557,25 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   }
   hdr->n_jobs = (uint32_t)F.rows.size();
   hdr->n_tiles = tiles;
-  hdr->pad[0] = hdr->pad[1] = 0;
+  hdr->pad[0] = bit_tiles;
+  hdr->pad[1] = 0;
+  const bool int8_jobs = tiles > 0, bit_jobs = bit_tiles > 0;
   uint32_t grid = shape.grid;
   // no more blocks than tiles (rounded up to a power of two, so that few
   // launch shapes and captured graphs arise as the table shrinks and grows)
+  const uint32_t work = std::max(tiles, bit_tiles);
   uint32_t fit = 8;
-  while (fit < tiles && fit < grid) fit <<= 1;
+  while (fit < work && fit < grid) fit <<= 1;
   grid = std::min(grid, fit);
   hipGraphExec_t exec = nullptr;
   if (P->cfg.use_graphs)
-    if (int rc = chunk_graph(P, p, grid, lane, K, &exec)) return rc;
+    if (int rc = chunk_graph(P, p, grid, lane, K, int8_jobs, bit_jobs, &exec)) return rc;
   SC_HIP(hipMemcpyAsync(P->d_table[p], P->h_table[p], sizeof(TileHeader) + F.rows.size() * sizeof(TileJob),
                         hipMemcpyHostToDevice, P->stream));
   SC_HIP(hipEventRecord(P->ev_start[p], P->stream));
   if (P->cfg.use_graphs) {
     SC_HIP(hipGraphLaunch(exec, P->stream));
-  } else if (int rc = enqueue_chunk_body(P, p, grid, lane, K, P->stream)) {
+  } else if (int rc = enqueue_chunk_body(P, p, grid, lane, K, int8_jobs, bit_jobs, P->stream)) {
     return rc;
   }
   SC_HIP(hipEventRecord(P->ev_graph[p], P->stream));
@@ -608,6 +632,8 @@ int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out) {
   if (hipMalloc(&P->spins_alloc, c.spin_budget_bytes + 2 * kSpinSlack) != hipSuccess)
     return fail(SCAMPER_ENOMEM, "spin budget of %llu bytes", (unsigned long long)c.spin_budget_bytes);
   P->spins = P->spins_alloc + kSpinSlack;
+  if (!c.int8_only && hipMalloc(&P->bits, c.spin_budget_bytes / 2 + 256) != hipSuccess)
+    return fail(SCAMPER_ENOMEM, "bit planes of %llu bytes", (unsigned long long)(c.spin_budget_bytes / 2));
   P->arena.reset(c.spin_budget_bytes);
   P->table_bytes = sizeof(TileHeader) + (size_t)c.max_jobs * sizeof(TileJob);
   const size_t ring = (size_t)c.max_jobs * c.chunk_sweeps * kSubs * kRec * sizeof(long long);

@@ -113,10 +113,11 @@ class ServeResult:
 
 
 def serve(requests: list[Request], spin_budget_bytes: int = 8 << 30, max_jobs: int = 512,
-          chunk_sweeps: int = 32, grid: int = 0, keep: int = 0) -> ServeResult:
+          chunk_sweeps: int = 32, grid: int = 0, keep: int = 0, bit_planes: bool = True) -> ServeResult:
     """Play the request stream in real time against one pool.  `keep` =
     number of leading requests whose estimates are returned."""
-    pool = Pool(spin_budget_bytes=spin_budget_bytes, max_jobs=max_jobs, chunk_sweeps=chunk_sweeps, grid=grid)
+    pool = Pool(spin_budget_bytes=spin_budget_bytes, max_jobs=max_jobs, chunk_sweeps=chunk_sweeps, grid=grid,
+                bit_planes=bit_planes)
     n = len(requests)
     id_to_req = {}
     done_at = np.zeros(n)
@@ -161,6 +162,31 @@ def plain(requests: list[Request], keep: int = 0) -> ServeResult:
         run_s[k] = time.perf_counter() - t
         if k < keep:
             estimates[k] = est
+    finish = np.zeros(len(requests))
+    clock = 0.0
+    for k, r in enumerate(requests):
+        clock = max(clock, r.arrival) + run_s[k]
+        finish[k] = clock
+    lat = finish - np.array([r.arrival for r in requests])
+    return ServeResult(len(requests), float(run_s.sum()), sum(r.updates for r in requests), lat, estimates)
+
+
+def plain_pool(requests: list[Request], keep: int = 0, chunk_sweeps: int = 32, bit_planes: bool = True) -> ServeResult:
+    """The same requests as plain per-job launches of the pool's kernels: one
+    job at a time, alone in a pool of one row, with every sweep's launches
+    issued from the host (no graphs), in arrival order."""
+    run_s = np.zeros(len(requests))
+    estimates = {}
+    for k, r in enumerate(requests):
+        pool = Pool(spin_budget_bytes=max(1 << 20, 4 * r.sites), max_jobs=1, chunk_sweeps=chunk_sweeps, grid=1024,
+                    use_graphs=False, bit_planes=bit_planes)
+        t = time.perf_counter()
+        jid = pool.submit(_spec(r))
+        pool.run()
+        run_s[k] = time.perf_counter() - t
+        if k < keep:
+            estimates[k] = pool.result(jid)[0]
+        del pool
     finish = np.zeros(len(requests))
     clock = 0.0
     for k, r in enumerate(requests):
