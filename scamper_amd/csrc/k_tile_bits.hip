@@ -41,6 +41,18 @@ constexpr int kThreads = 256;
 constexpr uint32_t kTileWords = 256;  // words per tile: one per lane of a block
 constexpr uint32_t kMsb = 0x80808080u;
 
+// Plane words through global-memory pointers (a generic pointer read from
+// the table would make every access a flat one).
+typedef __attribute__((address_space(1))) uint32_t g32;
+typedef __attribute__((address_space(1))) const char gbyte;
+
+// Word i of a plane, addressed by a 32-bit byte offset from its uniform base
+// (planes are far below 4 GiB), so that the load takes a scalar base and one
+// offset register.
+__device__ __forceinline__ uint32_t ld(const g32 *p, uint32_t i) {
+  return *reinterpret_cast<const g32 *>(reinterpret_cast<gbyte *>(p) + (i << 2));
+}
+
 __device__ __forceinline__ uint32_t div_by(uint32_t n, uint64_t magic) {
   return magic ? (uint32_t)__umul64hi((uint64_t)n, magic) : n;
 }
@@ -194,10 +206,11 @@ __device__ __forceinline__ uint32_t job_of_bit_tile(const TileJob *__restrict__ 
 struct Pair {
   uint32_t lo = 0, hi = 0;
 };
-__device__ __forceinline__ Pair pair_at(const uint32_t *__restrict__ p, int32_t o, int32_t hi) {
-  o = min(max(o, -32), hi);
-  const int32_t k = o >> 5;
-  return Pair{p[k + 1], p[k + 2]};
+__device__ __forceinline__ Pair pair_at(const g32 *__restrict__ p, int32_t o, int32_t hi) {
+  // storage index of word o >> 5, never negative after the clamp: an
+  // unsigned 32-bit offset from the plane's uniform base
+  const uint32_t i = (uint32_t)((min(max(o, -32), hi) >> 5) + 1);
+  return Pair{ld(p, i), ld(p, i + 1u)};
 }
 __device__ __forceinline__ uint32_t bits_of(const Pair &q, int32_t o) {
   return __builtin_amdgcn_alignbit(q.hi, q.lo, (uint32_t)o & 31u);
@@ -208,12 +221,12 @@ struct Raw {
   uint32_t before, at, after;  // words w - 1, w, w + 1
   Pair wl, wr, up, dn, y0, yl, bk, fw, z0, zl;
 
-  __device__ __forceinline__ void load(const uint32_t *__restrict__ p, uint32_t w, int32_t o, int32_t hi, int32_t x2,
+  __device__ __forceinline__ void load(const g32 *__restrict__ p, uint32_t w, int32_t o, int32_t hi, int32_t x2,
                                        int32_t pl, int32_t all, uint32_t need_wl, uint32_t need_wr, uint32_t m_y0,
                                        uint32_t m_yl, uint32_t m_z0, uint32_t m_zl) {
-    before = p[w];
-    at = p[w + 1];
-    after = p[w + 2];
+    before = ld(p, w);
+    at = ld(p, w + 1u);
+    after = ld(p, w + 2u);
     if (need_wl) wl = pair_at(p, o - 1 + x2, hi);
     if (need_wr) wr = pair_at(p, o + 1 - x2, hi);
     if (ND >= 2) {
@@ -265,10 +278,11 @@ __device__ __forceinline__ void run_word(const TileJob &J, uint32_t w, uint64_t 
   const uint32_t vm = nv == 32u ? ~0u : (1u << nv) - 1u;
   const uint64_t key = J.key;
   const uint32_t nwp = J.nwp;
-  uint32_t *__restrict__ own0 = J.bits + (colour * 2u) * nwp;
-  uint32_t *__restrict__ own1 = own0 + nwp;
-  const uint32_t *__restrict__ oth0 = J.bits + ((1u - colour) * 2u) * nwp;
-  const uint32_t *__restrict__ oth1 = oth0 + nwp;
+  g32 *const base = (g32 *)J.bits;
+  g32 *__restrict__ own0 = base + (colour * 2u) * nwp;
+  g32 *__restrict__ own1 = own0 + nwp;
+  const g32 *__restrict__ oth0 = base + ((1u - colour) * 2u) * nwp;
+  const g32 *__restrict__ oth1 = oth0 + nwp;
   const int32_t hi = (int32_t)(32u * n_words) - 1;
 
   // rows the word covers: masks of the sites with row parity 1, at a row's
@@ -314,8 +328,8 @@ __device__ __forceinline__ void run_word(const TileJob &J, uint32_t w, uint64_t 
   Raw<ND> A0, A1;
   A0.load(oth0, w, o, hi, x2, pl, all, m_first & ~m_p1, m_last & m_p1, m_y0, m_yl, m_z0, m_zl);
   if (TWO) A1.load(oth1, w, o, hi, x2, pl, all, m_first & ~m_p1, m_last & m_p1, m_y0, m_yl, m_z0, m_zl);
-  const uint32_t S0 = own0[w + 1];
-  const uint32_t S1 = TWO ? own1[w + 1] : 0u;
+  const uint32_t S0 = ld(own0, w + 1u);
+  const uint32_t S1 = TWO ? ld(own1, w + 1u) : 0u;
 
   const uint32_t g0 = j0 >> 4;
   const U4 T0 = byte_block(key, g0, sweep, colour, kStreamAccept, 0);

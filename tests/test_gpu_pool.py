@@ -1,7 +1,9 @@
 """The job pool (continuous batching of independent runs) on the GPU.
 
 A job's draws are keyed by its own (seed, sweep, colour, site), so its result
-must not depend on what else shares the GPU: each check below compares
+must not depend on what else shares the GPU, nor on whether the pool runs it
+on bit planes (Ising and Potts q <= 4, the default) or on int8 spins: each
+check below compares
 measured series, estimates and final spins bit for bit -- against
 run_one on the same spec (plain per-job launches), against the same job
 alone in a pool, across graph and plain launches, grid sizes and chunk
@@ -63,9 +65,13 @@ def _same(a, b):
         assert ea[k].mean == eb[k].mean and ea[k].stderr == eb[k].stderr, k
 
 
-def test_pool_equals_run_one(sc):
+@pytest.mark.parametrize("bit_planes", [True, False])
+def test_pool_equals_run_one(sc, bit_planes):
+    """Bit planes (Ising and Potts q <= 4 jobs; the q = 6 job stays on int8
+    spins in the same tables) or int8 spins throughout: every job equals its
+    run_one on the int8 kernel."""
     specs = [_spec(sc, d, k, q, b, t, m, 100 + i) for i, (d, k, q, b, t, m) in enumerate(MIX)]
-    res, pool = _run_pool(sc, specs, chunk_sweeps=16, max_jobs=64, spin_budget_bytes=1 << 24)
+    res, pool = _run_pool(sc, specs, chunk_sweeps=16, max_jobs=64, spin_budget_bytes=1 << 24, bit_planes=bit_planes)
     st = pool.stats()
     assert st.done == len(MIX) and st.queued == 0 and st.spin_bytes_in_use == 0
     for i, (d, k, q, b, t, m) in enumerate(MIX):
