@@ -242,6 +242,9 @@ struct alignas(16) TileJob {
   uint32_t *bits;
   uint32_t nwp, bit_tile_start;
   uint32_t lut_t8[4];
+  // the rest of K over t for the sites settled one by one: second byte, low
+  // 16 bits as two bytes, and bit t set where K = 2^32
+  uint32_t lut_tm8[4], lut_tk16lo[4], lut_tk16hi[4], tal;
 };
 
 // Fill a row's byte look-up tables from its thresholds.
@@ -277,11 +280,12 @@ struct EmLayout {
 // draws, decisions and measured records as launch_tile_sweep), and the
 // conversion of every bit-plane job's int8 colour arrays to planes (unpack 0)
 // or back (unpack 1).  The planes of a job must hold 16 (ceil(nh / 32) + 2)
-// bytes at `bits`.
+// bytes at `bits`; job_of_tile[t] is the row of bit tile t.
 void fill_bit_lut(TileJob &t);
-hipError_t launch_tile_bits(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint64_t step, uint32_t colour,
-                            int measure, EmLayout em, hipStream_t stream);
-hipError_t launch_bits_pack(const TileHeader *hdr, const TileJob *jobs, int unpack, hipStream_t stream);
+hipError_t launch_tile_bits(const TileHeader *hdr, const TileJob *jobs, const uint32_t *job_of_tile, uint32_t grid,
+                            uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream);
+hipError_t launch_bits_pack(const TileHeader *hdr, const TileJob *jobs, const uint32_t *job_of_tile, int unpack,
+                            hipStream_t stream);
 
 hipError_t launch_tile_sweep(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint32_t lane_sites,
                              uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream);

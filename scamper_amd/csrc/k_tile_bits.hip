@@ -109,13 +109,22 @@ __device__ uint32_t propose_exact(uint32_t top, uint32_t m, uint64_t key, uint32
   return (uint32_t)(((b2 | site_low(key, j, sweep, colour, kStreamPropose)) * m) >> 32);
 }
 
-// Exact "word < K" from the top byte, reading the finer draws on a tie.
-__device__ bool decide_exact(uint32_t top, const Threshold &t, uint64_t key, uint32_t j, uint64_t sweep,
+// Byte t (0..15) of a 16-entry byte table held in four uniform words.
+__device__ __forceinline__ uint32_t lut_byte(const uint32_t (&L)[4], uint32_t t) {
+  const uint32_t lo = __builtin_amdgcn_perm(L[1], L[0], t & 7u), hi = __builtin_amdgcn_perm(L[3], L[2], t & 7u);
+  return ((t & 8u) ? hi : lo) & 0xFFu;
+}
+
+// Exact "word < K" for look-up index t (K = 2^32 where bit t of tal is set),
+// from the top byte, reading the finer draws on a tie.
+__device__ bool decide_exact(uint32_t top, const TileJob &J, uint32_t t, uint64_t key, uint32_t j, uint64_t sweep,
                              uint32_t colour) {
-  if (top != t.k8) return top < t.k8;
-  const uint32_t mid = site_byte(key, j, sweep, colour, kStreamAccept, 1);
-  if (mid != t.m8) return mid < t.m8;
-  return site_low(key, j, sweep, colour, kStreamAccept) < t.k16;
+  if ((J.tal >> t) & 1u) return true;
+  const uint32_t k8 = lut_byte(J.lut_t8, t);
+  if (top != k8) return top < k8;
+  const uint32_t mid = site_byte(key, j, sweep, colour, kStreamAccept, 1), m8 = lut_byte(J.lut_tm8, t);
+  if (mid != m8) return mid < m8;
+  return site_low(key, j, sweep, colour, kStreamAccept) < (lut_byte(J.lut_tk16lo, t) | lut_byte(J.lut_tk16hi, t) << 8);
 }
 
 // Bit-sliced count of Z one-bit inputs: c0 + 2 c1 + 4 c2.
@@ -187,16 +196,6 @@ __device__ __forceinline__ void flush_block(BLds &L, BMeas &M, int nc, bool isin
   }
   M = BMeas{};
   ++flushes;
-}
-
-// The last job whose bit tiles start at or before t.
-__device__ __forceinline__ uint32_t job_of_bit_tile(const TileJob *__restrict__ jobs, uint32_t n_jobs, uint32_t t) {
-  uint32_t a = 0, b = n_jobs;
-  while (b - a > 1) {
-    const uint32_t c = (a + b) >> 1;
-    if (jobs[c].bit_tile_start <= t) a = c; else b = c;
-  }
-  return a;
 }
 
 // The words of one plane a lane needs: its own word and those beside it (the
@@ -451,9 +450,9 @@ __device__ __forceinline__ void run_word(const TileJob &J, uint32_t w, uint64_t 
       ns += (uint32_t)(n == s);
       nr += (uint32_t)(n == r);
     }
-    const uint32_t idx = POTTS ? Z + nr - ns : Z - ns;
+    const uint32_t tt = POTTS ? nr + 7u - ns : 7u - ns;  // index + 7 - z
     const uint32_t top = byte32(T0, T1, k);
-    const uint32_t v = decide_exact(top, J.thr[idx], key, j, sweep, colour) ? r : s;
+    const uint32_t v = decide_exact(top, J, tt, key, j, sweep, colour) ? r : s;
     const uint32_t b = 1u << k;
     NS0 = (v & 1u) ? NS0 | b : NS0 & ~b;
     if (TWO) NS1 = (v & 2u) ? NS1 | b : NS1 & ~b;
@@ -496,10 +495,11 @@ __device__ __forceinline__ void run_word(const TileJob &J, uint32_t w, uint64_t 
 // that XCD's L2; it adds up its measured counters over consecutive tiles of
 // one job and flushes them when the job changes and at the end.
 __global__ __launch_bounds__(kThreads) void k_tile_bits(const TileHeader *__restrict__ hdr,
-                                                        const TileJob *__restrict__ jobs, uint64_t step,
+                                                        const TileJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ job_of_tile, uint64_t step,
                                                         uint32_t colour, int measure, EmLayout em) {
   __shared__ BLds L;
-  const uint32_t n_tiles = hdr->pad[0], n_jobs = hdr->n_jobs;
+  const uint32_t n_tiles = hdr->pad[0];
   const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3, per = gridDim.x >> 3;
   const uint32_t lo = (uint32_t)(((uint64_t)n_tiles * xcd) >> 3);
   const uint32_t hi = (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_bits(const TileHeader *__rest
   bool ising = false, pending = false;
   uint32_t cur = ~0u, flushes = 0;
   for (uint32_t t = lo + local; t < hi; t += per) {
-    const uint32_t a = job_of_bit_tile(jobs, n_jobs, t);
+    const uint32_t a = job_of_tile[t];
     const TileJob &J = jobs[a];
     const uint64_t sweep = J.sweep_base + step;
     if (sweep >= J.sweep_end) continue;
@@ -557,13 +557,14 @@ __device__ __forceinline__ uint32_t planes_of4(uint32_t x, bool ising, uint32_t 
   return gather4(x & 0x01010101u);
 }
 
-__global__ void k_bits_pack(const TileHeader *__restrict__ hdr, const TileJob *__restrict__ jobs) {
-  const uint32_t n_tiles = hdr->pad[0], n_jobs = hdr->n_jobs;
+__global__ void k_bits_pack(const TileHeader *__restrict__ hdr, const TileJob *__restrict__ jobs,
+                              const uint32_t *__restrict__ job_of_tile) {
+  const uint32_t n_tiles = hdr->pad[0];
   const uint64_t n = (uint64_t)n_tiles * kTileWords * 2u;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t colour = (uint32_t)(i & 1u), word = (uint32_t)(i >> 1);
     const uint32_t t = word / kTileWords;
-    const TileJob &J = jobs[job_of_bit_tile(jobs, n_jobs, t)];
+    const TileJob &J = jobs[job_of_tile[t]];
     const uint32_t w = (t - J.bit_tile_start) * kTileWords + word % kTileWords;
     const uint32_t nh = J.nh;
     if (w >= ((nh + 31u) >> 5)) continue;
@@ -592,13 +593,14 @@ __global__ void k_bits_pack(const TileHeader *__restrict__ hdr, const TileJob *_
   }
 }
 
-__global__ void k_bits_unpack(const TileHeader *__restrict__ hdr, const TileJob *__restrict__ jobs) {
-  const uint32_t n_tiles = hdr->pad[0], n_jobs = hdr->n_jobs;
+__global__ void k_bits_unpack(const TileHeader *__restrict__ hdr, const TileJob *__restrict__ jobs,
+                              const uint32_t *__restrict__ job_of_tile) {
+  const uint32_t n_tiles = hdr->pad[0];
   const uint64_t n = (uint64_t)n_tiles * kTileWords * 2u;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t colour = (uint32_t)(i & 1u), word = (uint32_t)(i >> 1);
     const uint32_t t = word / kTileWords;
-    const TileJob &J = jobs[job_of_bit_tile(jobs, n_jobs, t)];
+    const TileJob &J = jobs[job_of_tile[t]];
     const uint32_t w = (t - J.bit_tile_start) * kTileWords + word % kTileWords;
     const uint32_t nh = J.nh;
     if (w >= ((nh + 31u) >> 5)) continue;
@@ -626,27 +628,35 @@ __global__ void k_bits_unpack(const TileHeader *__restrict__ hdr, const TileJob 
 // Byte look-up table over t = index + 7 - z (the bit kernel's index).
 void fill_bit_lut(TileJob &t) {
   const int z = 2 * (int)t.nd;
-  for (int w = 0; w < 4; ++w) t.lut_t8[w] = 0;
+  for (int w = 0; w < 4; ++w) t.lut_t8[w] = t.lut_tm8[w] = t.lut_tk16lo[w] = t.lut_tk16hi[w] = 0;
+  t.tal = 0;
   for (int v = 0; v < 16; ++v) {
     const int idx = v - 7 + z;
     if (idx < 0 || idx >= kStencilTable) continue;
-    const uint32_t k8 = t.thr[idx].k8;
-    t.lut_t8[v >> 2] |= (k8 > 0xFFu ? 0xFFu : k8) << (8u * (uint32_t)(v & 3));
+    const Threshold &th = t.thr[idx];
+    const uint32_t sh = 8u * (uint32_t)(v & 3);
+    t.lut_t8[v >> 2] |= (th.k8 > 0xFFu ? 0xFFu : th.k8) << sh;
+    t.lut_tm8[v >> 2] |= (uint32_t)(th.m8 & 0xFFu) << sh;
+    t.lut_tk16lo[v >> 2] |= (uint32_t)(th.k16 & 0xFFu) << sh;
+    t.lut_tk16hi[v >> 2] |= (uint32_t)(th.k16 >> 8) << sh;
+    if (th.k8 > 0xFFu) t.tal |= 1u << v;
   }
 }
 
-hipError_t launch_tile_bits(const TileHeader *hdr, const TileJob *jobs, uint32_t grid, uint64_t step, uint32_t colour,
-                            int measure, EmLayout em, hipStream_t stream) {
+hipError_t launch_tile_bits(const TileHeader *hdr, const TileJob *jobs, const uint32_t *job_of_tile, uint32_t grid,
+                            uint64_t step, uint32_t colour, int measure, EmLayout em, hipStream_t stream) {
   if (grid == 0 || (grid & 7u)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_tile_bits, dim3(grid), dim3(kThreads), 0, stream, hdr, jobs, step, colour, measure, em);
+  hipLaunchKernelGGL(k_tile_bits, dim3(grid), dim3(kThreads), 0, stream, hdr, jobs, job_of_tile, step, colour, measure,
+                     em);
   return hipGetLastError();
 }
 
-hipError_t launch_bits_pack(const TileHeader *hdr, const TileJob *jobs, int unpack, hipStream_t stream) {
+hipError_t launch_bits_pack(const TileHeader *hdr, const TileJob *jobs, const uint32_t *job_of_tile, int unpack,
+                            hipStream_t stream) {
   if (unpack)
-    hipLaunchKernelGGL(k_bits_unpack, dim3(1024), dim3(kThreads), 0, stream, hdr, jobs);
+    hipLaunchKernelGGL(k_bits_unpack, dim3(1024), dim3(kThreads), 0, stream, hdr, jobs, job_of_tile);
   else
-    hipLaunchKernelGGL(k_bits_pack, dim3(1024), dim3(kThreads), 0, stream, hdr, jobs);
+    hipLaunchKernelGGL(k_bits_pack, dim3(1024), dim3(kThreads), 0, stream, hdr, jobs, job_of_tile);
   return hipGetLastError();
 }
 

@@ -159,6 +159,10 @@ struct scamper_pool {
   long long *d_ring[2] = {nullptr, nullptr};
   long long *d_compact[2] = {nullptr, nullptr};
   long long *h_compact[2] = {nullptr, nullptr};
+  // row of every bit-plane tile of the table (one load finds a tile's job)
+  uint32_t *d_tmap[2] = {nullptr, nullptr};
+  uint32_t *h_tmap[2] = {nullptr, nullptr};
+  uint64_t tmap_cap = 0;
   // captured chunks per table buffer, keyed by launch shape (grid << 8 | lane)
   std::map<uint64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs[2];
   hipEvent_t ev_start[2] = {nullptr, nullptr}, ev_graph[2] = {nullptr, nullptr}, ev_end[2] = {nullptr, nullptr};
@@ -194,6 +198,8 @@ struct scamper_pool {
       hipFree(d_table[p]);
       hipFree(d_ring[p]);
       hipFree(d_compact[p]);
+      hipFree(d_tmap[p]);
+      if (h_tmap[p]) hipHostFree(h_tmap[p]);
       if (h_table[p]) hipHostFree(h_table[p]);
       if (h_compact[p]) hipHostFree(h_compact[p]);
       if (ev_start[p]) hipEventDestroy(ev_start[p]);
@@ -221,14 +227,14 @@ int enqueue_chunk_body(scamper_pool *P, int p, uint32_t grid, uint32_t lane, uin
   const TileJob *rows = reinterpret_cast<const TileJob *>(hdr + 1);
   const EmLayout em{P->d_ring[p], ring_row_counters(K), (uint64_t)kSubs * kRec, kSubs - 1, kRec};
   hipError_t e = launch_tile_zero(hdr, P->d_ring[p], ring_row_counters(K), st);
-  if (e == hipSuccess && bit_jobs) e = launch_bits_pack(hdr, rows, 0, st);
+  if (e == hipSuccess && bit_jobs) e = launch_bits_pack(hdr, rows, P->d_tmap[p], 0, st);
   for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
     for (uint32_t c = 0; c < 2 && e == hipSuccess; ++c) {
       if (int8_jobs) e = launch_tile_sweep(hdr, rows, grid, lane, k, c, (int)c, em, st);
-      if (e == hipSuccess && bit_jobs) e = launch_tile_bits(hdr, rows, grid, k, c, (int)c, em, st);
+      if (e == hipSuccess && bit_jobs) e = launch_tile_bits(hdr, rows, P->d_tmap[p], grid, k, c, (int)c, em, st);
     }
   }
-  if (e == hipSuccess && bit_jobs) e = launch_bits_pack(hdr, rows, 1, st);
+  if (e == hipSuccess && bit_jobs) e = launch_bits_pack(hdr, rows, P->d_tmap[p], 1, st);
   if (e == hipSuccess) e = launch_tile_fold(hdr, P->d_ring[p], P->d_compact[p], K, st);
   if (e != hipSuccess) return fail(SCAMPER_EDEVICE, "chunk launch: %s", hipGetErrorString(e));
   return SCAMPER_OK;
@@ -541,7 +547,10 @@ int step(scamper_pool *P, uint32_t *n_completed) {
       t.bits = P->bits + J.off / 8;  // byte offset off / 2
       t.nwp = words + 2;
       fill_bit_lut(t);
-      bit_tiles += (words + 255) / 256;
+      const uint32_t n = (words + 255) / 256;
+      if (bit_tiles + (uint64_t)n > P->tmap_cap) return fail(SCAMPER_EINVARIANT, "bit tile map overflow");
+      for (uint32_t k = 0; k < n; ++k) P->h_tmap[p][bit_tiles + k] = (uint32_t)F.rows.size();
+      bit_tiles += n;
     } else {
       t.n_tiles = (uint32_t)((J.geom.nh + tile_sites(lane) - 1) / tile_sites(lane));
       tiles += t.n_tiles;
@@ -570,6 +579,9 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   hipGraphExec_t exec = nullptr;
   if (P->cfg.use_graphs)
     if (int rc = chunk_graph(P, p, grid, lane, K, int8_jobs, bit_jobs, &exec)) return rc;
+  if (bit_tiles)
+    SC_HIP(hipMemcpyAsync(P->d_tmap[p], P->h_tmap[p], (size_t)bit_tiles * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          P->stream));
   SC_HIP(hipMemcpyAsync(P->d_table[p], P->h_table[p], sizeof(TileHeader) + F.rows.size() * sizeof(TileJob),
                         hipMemcpyHostToDevice, P->stream));
   SC_HIP(hipEventRecord(P->ev_start[p], P->stream));
@@ -634,6 +646,14 @@ int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out) {
   P->spins = P->spins_alloc + kSpinSlack;
   if (!c.int8_only && hipMalloc(&P->bits, c.spin_budget_bytes / 2 + 256) != hipSuccess)
     return fail(SCAMPER_ENOMEM, "bit planes of %llu bytes", (unsigned long long)(c.spin_budget_bytes / 2));
+  // resident jobs hold at most budget / 2 colour-class sites in all, in
+  // words of 32 (one partial word per job) and tiles of 256 words (one
+  // partial tile per job)
+  P->tmap_cap = c.int8_only ? 0 : c.spin_budget_bytes / (2ull * 8192) + 2ull * c.max_jobs + 16;
+  for (int p = 0; p < 2 && P->tmap_cap; ++p)
+    if (hipMalloc(&P->d_tmap[p], P->tmap_cap * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc(&P->h_tmap[p], P->tmap_cap * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+      return fail(SCAMPER_ENOMEM, "bit tile map");
   P->arena.reset(c.spin_budget_bytes);
   P->table_bytes = sizeof(TileHeader) + (size_t)c.max_jobs * sizeof(TileJob);
   const size_t ring = (size_t)c.max_jobs * c.chunk_sweeps * kSubs * kRec * sizeof(long long);
