@@ -245,6 +245,10 @@ struct alignas(16) TileJob {
   // the rest of K over t for the sites settled one by one: second byte, low
   // 16 bits as two bytes, and bit t set where K = 2^32
   uint32_t lut_tm8[4], lut_tk16lo[4], lut_tk16hi[4], tal;
+  // bit 0: pack the int8 spins into the planes at this chunk's start (the
+  // job's first chunk), bit 1: unpack them at its end (its last); between
+  // them the planes are the job's state
+  uint32_t bit_io;
 };
 
 // Fill a row's byte look-up tables from its thresholds.

@@ -546,8 +546,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_bits(const TileHeader *__rest
   if (pending) flush_block(L, M, nc, ising, rec, flushes);
 }
 
-// int8 colour arrays <-> bit planes, one thread per (word, colour) of every
-// bit-plane job in the table.
+// int8 colour arrays <-> bit planes, one thread per (word, colour) of the
+// bit-plane jobs in the table that start (pack) or end (unpack) in this chunk.
 __device__ __forceinline__ uint32_t planes_of4(uint32_t x, bool ising, uint32_t &b1) {
   if (ising) {
     b1 = 0;
@@ -567,7 +567,7 @@ __global__ void k_bits_pack(const TileHeader *__restrict__ hdr, const TileJob *_
     const TileJob &J = jobs[job_of_tile[t]];
     const uint32_t w = (t - J.bit_tile_start) * kTileWords + word % kTileWords;
     const uint32_t nh = J.nh;
-    if (w >= ((nh + 31u) >> 5)) continue;
+    if (!(J.bit_io & 1u) || w >= ((nh + 31u) >> 5)) continue;
     const bool ising = J.kind == kIsing, two = !ising && J.q > 2u;
     const int8_t *src = J.spins + (uint64_t)colour * J.cs + 32ull * w;
     uint32_t p0 = 0, p1 = 0;
@@ -603,7 +603,7 @@ __global__ void k_bits_unpack(const TileHeader *__restrict__ hdr, const TileJob 
     const TileJob &J = jobs[job_of_tile[t]];
     const uint32_t w = (t - J.bit_tile_start) * kTileWords + word % kTileWords;
     const uint32_t nh = J.nh;
-    if (w >= ((nh + 31u) >> 5)) continue;
+    if (!(J.bit_io & 2u) || w >= ((nh + 31u) >> 5)) continue;
     const bool ising = J.kind == kIsing, two = !ising && J.q > 2u;
     const uint32_t *src = J.bits + (colour * 2u) * J.nwp + 1u + w;
     const uint32_t p0 = src[0], p1 = two ? src[J.nwp] : 0u;

@@ -546,6 +546,8 @@ int step(scamper_pool *P, uint32_t *n_completed) {
       const uint32_t words = (uint32_t)((J.geom.nh + 31) / 32);
       t.bits = P->bits + J.off / 8;  // byte offset off / 2
       t.nwp = words + 2;
+      // planes are the job's state from its first chunk to its last
+      t.bit_io = (J.enqueued == 0 ? 1u : 0u) | (J.enqueued + steps == J.total ? 2u : 0u);
       fill_bit_lut(t);
       const uint32_t n = (words + 255) / 256;
       if (bit_tiles + (uint64_t)n > P->tmap_cap) return fail(SCAMPER_EINVARIANT, "bit tile map overflow");
