@@ -490,26 +490,37 @@ __device__ __forceinline__ void run_word(const TileJob &J, uint32_t w, uint64_t 
   }
 }
 
-// Block b takes tiles (256 words of one job) of its XCD's share of the
-// flattened list (blocks b, b + 8, ... run on one XCD), so halo words stay in
-// that XCD's L2; it adds up its measured counters over consecutive tiles of
-// one job and flushes them when the job changes and at the end.
+// Blocks take tiles of 256 words of one job; a block adds up its measured
+// counters over consecutive tiles of one job and flushes them when the job
+// changes and at the end.
 __global__ __launch_bounds__(kThreads) void k_tile_bits(const TileHeader *__restrict__ hdr,
                                                         const TileJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ job_of_tile, uint64_t step,
                                                         uint32_t colour, int measure, EmLayout em) {
   __shared__ BLds L;
+  // One job: block b walks a contiguous share of its XCD's eighth of the
+  // tiles (blocks b, b + 8, ... run on one XCD), so halo words stay in that
+  // XCD's L2.  Several jobs: block b takes tiles b, b + grid, ..., which
+  // spreads every job over all XCDs (their costs differ by q and
+  // dimension), and its first tile's row is read together with the header
+  // (the map holds at least one entry per block).
+  const uint32_t first = job_of_tile[blockIdx.x];
   const uint32_t n_tiles = hdr->pad[0];
-  const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3, per = gridDim.x >> 3;
-  const uint32_t lo = (uint32_t)(((uint64_t)n_tiles * xcd) >> 3);
-  const uint32_t hi = (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
+  const bool spread = hdr->n_jobs > 1;
+  uint32_t t0 = blockIdx.x, end = n_tiles, stride = gridDim.x;
+  if (!spread) {
+    const uint32_t xcd = blockIdx.x & 7u;
+    t0 = (uint32_t)(((uint64_t)n_tiles * xcd) >> 3) + (blockIdx.x >> 3);
+    end = (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
+    stride = gridDim.x >> 3;
+  }
   BMeas M;
   long long *rec = nullptr;
   int nc = 0;
   bool ising = false, pending = false;
   uint32_t cur = ~0u, flushes = 0;
-  for (uint32_t t = lo + local; t < hi; t += per) {
-    const uint32_t a = job_of_tile[t];
+  for (uint32_t t = t0; t < end; t += stride) {
+    const uint32_t a = !spread ? 0u : t == blockIdx.x ? first : job_of_tile[t];
     const TileJob &J = jobs[a];
     const uint64_t sweep = J.sweep_base + step;
     if (sweep >= J.sweep_end) continue;

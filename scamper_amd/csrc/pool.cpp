@@ -651,7 +651,11 @@ int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out) {
   // resident jobs hold at most budget / 2 colour-class sites in all, in
   // words of 32 (one partial word per job) and tiles of 256 words (one
   // partial tile per job)
-  P->tmap_cap = c.int8_only ? 0 : c.spin_budget_bytes / (2ull * 8192) + 2ull * c.max_jobs + 16;
+  // (and at least one entry per block of any launch: the kernel reads the
+  // entry of its block's first tile early)
+  P->tmap_cap = c.int8_only ? 0
+                            : std::max<uint64_t>(c.spin_budget_bytes / (2ull * 8192) + 2ull * c.max_jobs + 16,
+                                                 std::max<uint32_t>(c.grid, kGridCandidates[2]));
   for (int p = 0; p < 2 && P->tmap_cap; ++p)
     if (hipMalloc(&P->d_tmap[p], P->tmap_cap * sizeof(uint32_t)) != hipSuccess ||
         hipHostMalloc(&P->h_tmap[p], P->tmap_cap * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
