@@ -529,10 +529,11 @@ def tri(sc, args) -> dict:
 
 
 def potts3d(sc, args) -> dict:
-    """The serving pool's kernel on one large lattice through the plain state
-    path (no graphs, so a profiler sees every launch): 3-state Potts, 3-D
-    L=256, J=1, beta=1.0 (T_c is 1/0.550565), Metropolis with the fused
-    per-sweep measurement on every sweep."""
+    """The tiled int8 kernel on one large lattice through the state path (no
+    graphs, so a profiler sees every launch): 3-state Potts, 3-D L=256, J=1,
+    beta=1.0 (T_c is 1/0.550565), Metropolis with the fused per-sweep
+    measurement on every sweep; then the same lattice as a job of the pool,
+    which runs it on bit planes (pool_bit_planes)."""
     L, q = 256, 3
     lat = sc.Lattice.hypercubic([L, L, L], True)
     st = sc.State(lat, sc.model(sc.POTTS, 1.0, q), 1, 42)
