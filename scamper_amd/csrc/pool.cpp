@@ -638,8 +638,10 @@ int scamper_pool_create(const scamper_pool_config *cfg, scamper_pool **out) {
   c.spin_budget_bytes = (c.spin_budget_bytes + kAlign - 1) / kAlign * kAlign;
   P->cfg = c;
   // a configured grid fixes the shape (lanes by size, the configured
-  // chunk); otherwise the autotuner starts from 1024 blocks
-  P->shape = Shape{c.grid ? c.grid : 1024u, kLaneCrossovers[0], c.chunk_sweeps};
+  // chunk); otherwise the autotuner starts from 4096 blocks (capped at the
+  // table's tiles: blocks that finish early take the remaining tiles, which
+  // balances jobs of different costs)
+  P->shape = Shape{c.grid ? c.grid : kGridCandidates[2], kLaneCrossovers[0], c.chunk_sweeps};
   if (c.grid) P->tuned = true;
   SC_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
   SC_HIP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));

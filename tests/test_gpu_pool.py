@@ -173,7 +173,8 @@ def test_autotuner_settles_grid_lane_and_chunk(sc):
 
 def test_autotuner_waits_for_enough_work(sc):
     """A resident set below 1 Mi colour-class sites runs the starting shape
-    (1024 blocks, 16 sites per lane from 2 Mi sites, chunk_sweeps) untuned."""
+    (4096 blocks capped at the tiles, 16 sites per lane from 2 Mi sites,
+    chunk_sweeps) untuned."""
     specs = [_spec(sc, (64, 64), 1, 3, 1.0, 10, 90 + 7 * i, 720 + i) for i in range(3)]
     got, pool = _run_pool(sc, specs, chunk_sweeps=8, max_jobs=16, spin_budget_bytes=1 << 24, grid=0)
     st = pool.stats()
