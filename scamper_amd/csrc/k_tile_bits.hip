@@ -14,9 +14,9 @@
 // word w holding rank 32w + k; a Potts state v is planes (v & 1, v >> 1), an
 // Ising spin one plane (1 for -1).  Each plane has one spare word before and
 // after it, so that 32 bits starting anywhere in [-32, 32 words) are two
-// aligned loads and a funnel shift.  The int8 arrays stay the jobs' state
-// between chunks: a chunk packs them into planes, runs its sweeps on the
-// planes and unpacks them at its end.
+// aligned loads and a funnel shift.  A job is admitted with int8 arrays;
+// they are packed into its planes at its first chunk, the planes are its
+// state until its last chunk unpacks them.
 //
 // A lane updates one word (32 ranks, two Philox groups).  Its neighbours in
 // the other colour are that colour's bits at rank offsets 0, -1 / +1 (by row
