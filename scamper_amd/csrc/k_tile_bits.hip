@@ -436,21 +436,24 @@ __device__ __forceinline__ void run_word(const TileJob &J, uint32_t w, uint64_t 
     open &= open - 1u;
     const uint32_t j = j0 + k;
     const uint32_t s = bit_of(S0, k) | (TWO ? bit_of(S1, k) << 1 : 0u);
-    uint32_t r;
-    if (!POTTS || QK == 0) {
-      r = s ^ 1u;
+    uint32_t r, tt;
+    if (QK != 4) {
+      // a tie (q <= 3 proposals never straddle): the planes' proposal and
+      // look-up index are exact
+      r = bit_of(R0, k) | (TWO ? bit_of(R1, k) << 1 : 0u);
+      tt = bit_of(t0, k) | bit_of(t1, k) << 1 | bit_of(t2, k) << 2 | bit_of(t3, k) << 3;
     } else {
       r = propose_exact(byte32(P0, P1, k), (uint32_t)QK - 1u, key, j, sweep, colour);
       r += r >= s;
-    }
-    uint32_t ns = 0, nr = 0;
+      uint32_t ns = 0, nr = 0;
 #pragma unroll
-    for (int c = 0; c < Z; ++c) {
-      const uint32_t n = bit_of(N0[c], k) | (TWO ? bit_of(N1[c], k) << 1 : 0u);
-      ns += (uint32_t)(n == s);
-      nr += (uint32_t)(n == r);
+      for (int c = 0; c < Z; ++c) {
+        const uint32_t n = bit_of(N0[c], k) | (TWO ? bit_of(N1[c], k) << 1 : 0u);
+        ns += (uint32_t)(n == s);
+        nr += (uint32_t)(n == r);
+      }
+      tt = nr + 7u - ns;  // index + 7 - z
     }
-    const uint32_t tt = POTTS ? nr + 7u - ns : 7u - ns;  // index + 7 - z
     const uint32_t top = byte32(T0, T1, k);
     const uint32_t v = decide_exact(top, J, tt, key, j, sweep, colour) ? r : s;
     const uint32_t b = 1u << k;
