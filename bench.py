@@ -658,7 +658,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU "
-              "(multi-GPU sharding is described in DESIGN.md, not launched).", file=sys.stderr)
+              "(multi-GPU sharding is designed but not launched).", file=sys.stderr)
         return 2
 
     import scamper_amd as sc

@@ -1269,7 +1269,7 @@ int cluster_sweeps(scamper_state *st, int algorithm, uint64_t n_sweeps, int meas
 }  // namespace
 
 // Bytes of O(N) stencil spins swept as one group of replicas (see
-// scamper_sweep), inside the 256 MB Infinity Cache: C4 (340 MB) runs as two
+// scamper_sweep), inside the 256 MB Infinity Cache: O(3) at 96^3 x 32 (340 MB) runs as two
 // groups of 16 replicas (123 -> 147 site-updates/ns); a state that already
 // fits (XY at 96^3 x 32, 226 MB) stays one group, splitting it cost 2%.
 constexpr uint64_t kReplicaBlockBytes = 240ull << 20;
@@ -1295,7 +1295,7 @@ int scamper_sweep(scamper_state *st, int algorithm, uint64_t n_sweeps, int measu
   if (st->continuous && !st->csr) {
     // O(N) on a stencil: every sweep of a group of replicas before the next
     // group, each group small enough to stay in the 256 MB Infinity Cache
-    // across its half-sweeps (C4's 32 replicas at 96^3 are 340 MB, which
+    // across its half-sweeps (32 O(3) replicas at 96^3 are 340 MB, which
     // would stream from HBM at every launch).  Draws are keyed by (replica,
     // sweep, colour, rank), so the order of groups changes no result.
     const int d = st->model.spin_dim;

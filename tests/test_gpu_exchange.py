@@ -69,7 +69,7 @@ def test_exchange_trajectory_matches_oracle(sc, orc, builder, args, kind, q, j, 
 
 
 def test_exchange_rule_on3(sc, orc):
-    """O(3), 32 replicas at T = 1.40 .. 1.49 (the C4 ladder) on 8^3: hybrid
+    """O(3), 32 replicas at T = 1.40 .. 1.49 (the 96^3 workload's ladder) on 8^3: hybrid
     sweeps and exchange rounds.  The labels follow the oracle's rule applied
     to the device's energies; those energies equal the FP64 recount of the
     device's spins to FP32 accuracy."""

@@ -300,7 +300,7 @@ def test_c4_shape_against_fp64_oracle(sc, orc):
     bars are the maximum (2e-4), the 99.99% quantile (5e-5) and the mean
     (1e-6) per component, and the replica's energy and |M| (1e-6)."""
     L, R, seed = 96, 32, 42
-    print(f"case on3 C4 shape {L}^3 x {R} replicas", flush=True)
+    print(f"case on3 bench shape {L}^3 x {R} replicas", flush=True)
     lat = sc.Lattice.hypercubic([L, L, L])
     st = sc.State(lat, sc.model(sc.ON, 1.0, 2, 3), R, seed)
     betas = [1.0 / (1.40 + 0.09 * r / 31) for r in range(R)]
