@@ -265,7 +265,8 @@ inline void fill_tile_luts(TileJob &t) {
 struct alignas(16) TileHeader {
   uint32_t n_jobs;
   uint32_t n_tiles;  // tiles of the int8 kernel
-  uint32_t pad[2];   // pad[0]: tiles of 256 words of the bit-plane kernel
+  uint32_t pad[2];   // pad[0]: tiles of 256 words of the bit-plane kernel; pad[1] bit 0: the int8
+                     // kernel deals the tiles over all XCDs (the pool sets it for several jobs)
 };
 
 

@@ -721,10 +721,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   __shared__ TileLds L;
   const uint32_t n_tiles = hdr->n_tiles, n_jobs = hdr->n_jobs;
   // XCD-aware walk: blocks b, b+8, ... run on one XCD and take one
-  // contiguous eighth of the tiles, so halo rows stay in that XCD's L2.
-  const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3, per = gridDim.x >> 3;
-  const uint32_t lo = (uint32_t)(((uint64_t)n_tiles * xcd) >> 3);
-  const uint32_t hi = (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
+  // contiguous eighth of the tiles, so halo rows stay in that XCD's L2.  A
+  // table of jobs of different costs (header flag pad[1] bit 0, set by the
+  // pool for several jobs) is dealt tile by tile over all blocks instead.
+  const bool spread = (hdr->pad[1] & 1u) != 0u;
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint32_t lo = spread ? blockIdx.x : (uint32_t)(((uint64_t)n_tiles * xcd) >> 3) + (blockIdx.x >> 3);
+  const uint32_t hi = spread ? n_tiles : (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
+  const uint32_t per = spread ? gridDim.x : gridDim.x >> 3;
   // A block adds up its measured counters over consecutive tiles of one job
   // and flushes them when the job changes, before a lane's 8-bit state
   // counters can overflow (they gain at most 2 * LANE per tile) and at the end.
@@ -734,7 +738,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   int nc = 0;
   bool ising = false;
   uint32_t cur = ~0u, pending = 0, flushes = 0;
-  for (uint32_t t = lo + local; t < hi; t += per) {
+  for (uint32_t t = lo; t < hi; t += per) {
     uint32_t a = 0, b = n_jobs;  // last job with tile_start <= t
     while (b - a > 1) {
       const uint32_t c = (a + b) >> 1;

@@ -569,7 +569,7 @@ int step(scamper_pool *P, uint32_t *n_completed) {
   hdr->n_jobs = (uint32_t)F.rows.size();
   hdr->n_tiles = tiles;
   hdr->pad[0] = bit_tiles;
-  hdr->pad[1] = 0;
+  hdr->pad[1] = F.rows.size() > 1 ? 1u : 0u;  // the int8 kernel deals several jobs' tiles over all XCDs
   const bool int8_jobs = tiles > 0, bit_jobs = bit_tiles > 0;
   uint32_t grid = shape.grid;
   // no more blocks than tiles (rounded up to a power of two, so that few
